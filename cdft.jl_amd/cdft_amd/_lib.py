@@ -1,0 +1,211 @@
+"""ctypes binding of libcdft_hip.so (declarations: include/cdft_hip.h).
+
+The product path has no CPU fallback: if the HIP library is missing, or no GPU is
+visible, every compute call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBPATH = os.path.join(_PKG, "libcdft_hip.so")
+
+CDFT_OK = 0
+CDFT_FP64 = 0
+CDFT_MIXED = 1
+MAXC = 8
+
+_dp = C.POINTER(C.c_double)
+
+# every symbol include/cdft_hip.h declares
+EXPORTS = [
+    "cdft_ctx_create", "cdft_ctx_destroy", "cdft_last_error", "cdft_system_create",
+    "cdft_system_destroy", "cdft_system_npoints", "cdft_functional_eval", "cdft_converge",
+    "cdft_converge_batch", "cdft_converge_batch_device", "cdft_last_kernel_ms",
+]
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [
+        ("nc", C.c_int), ("ndim", C.c_int), ("ngrid", C.c_int * 3),
+        ("lb", C.c_double * 3), ("ub", C.c_double * 3),
+        ("T", C.c_double), ("p", C.c_double), ("L", C.c_double), ("psi", C.c_double),
+        ("m", _dp), ("d", _dp), ("sigma", _dp), ("eps", _dp),
+        ("rho_bulk", _dp), ("mu_res", _dp), ("vext", _dp), ("precision", C.c_int),
+    ]
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("maxit", C.c_int), ("beta", C.c_double), ("tol", C.c_double),
+                ("anderson_start", C.c_double), ("m", C.c_int), ("hist_cap", C.c_int),
+                ("hist", _dp)]
+
+
+class Result(C.Structure):
+    _fields_ = [("idid", C.c_int), ("errcode", C.c_int), ("iters", C.c_int),
+                ("nevals", C.c_int), ("nhist", C.c_int), ("resnorm", C.c_double),
+                ("tol", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CDFTError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise CDFTError(f"{LIBPATH} not found: build it with __graft_entry__.build() "
+                        "(make -C cdft.jl_amd/csrc); there is no CPU fallback")
+    L = C.CDLL(LIBPATH)
+    vp = C.c_void_p
+    L.cdft_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.cdft_ctx_destroy.argtypes = [vp]
+    L.cdft_last_error.argtypes = [vp]
+    L.cdft_last_error.restype = C.c_char_p
+    L.cdft_system_create.argtypes = [vp, C.POINTER(SystemDesc), C.POINTER(vp)]
+    L.cdft_system_destroy.argtypes = [vp]
+    L.cdft_system_npoints.argtypes = [vp]
+    L.cdft_functional_eval.argtypes = [vp, _dp, _dp, _dp, _dp]
+    L.cdft_converge.argtypes = [vp, _dp, C.POINTER(SolverOpts), C.POINTER(Result)]
+    L.cdft_converge_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(_dp),
+                                      C.POINTER(SolverOpts), C.POINTER(Result)]
+    L.cdft_converge_batch_device.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(_dp),
+                                             C.POINTER(SolverOpts), C.POINTER(Result)]
+    L.cdft_last_kernel_ms.argtypes = [vp]
+    L.cdft_last_kernel_ms.restype = C.c_double
+    _lib = L
+    return L
+
+
+def ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def check(rc, ctx=None):
+    if rc != CDFT_OK:
+        msg = load().cdft_last_error(ctx)
+        raise CDFTError(f"cdft error {rc}: {msg.decode() if msg else ''}")
+
+
+class Context:
+    """One device + HIP stream (cdft_ctx)."""
+
+    def __init__(self, device=0):
+        L = load()
+        h = C.c_void_p()
+        check(L.cdft_ctx_create(int(device), C.byref(h)), None)
+        self.handle = h
+        self.device = device
+
+    def __del__(self):
+        try:
+            if self.handle:
+                load().cdft_ctx_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def last_kernel_ms(self):
+        return load().cdft_last_kernel_ms(self.handle)
+
+
+_default_ctx = {}
+
+
+def default_context(device=0):
+    if device not in _default_ctx:
+        _default_ctx[device] = Context(device)
+    return _default_ctx[device]
+
+
+class DeviceSystem:
+    """A cdft_sys handle plus the host arrays its descriptor pointed to."""
+
+    def __init__(self, ctx, *, nc, ndim, ngrid, lb, ub, T, p, L, psi, m, d, sigma, eps,
+                 rho_bulk, mu_res, vext, precision):
+        self.ctx = ctx
+        keep = {k: np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+                for k, v in dict(m=m, d=d, sigma=sigma, eps=eps, rho_bulk=rho_bulk,
+                                 mu_res=mu_res).items()}
+        desc = SystemDesc()
+        desc.nc, desc.ndim = int(nc), int(ndim)
+        for a in range(3):
+            desc.ngrid[a] = int(ngrid[a]) if a < ndim else 1
+            desc.lb[a] = float(lb[a]) if a < ndim else 0.0
+            desc.ub[a] = float(ub[a]) if a < ndim else 0.0
+        desc.T, desc.p, desc.L, desc.psi = float(T), float(p), float(L), float(psi)
+        for k, v in keep.items():
+            setattr(desc, k, ptr(v))
+        if vext is not None:
+            keep["vext"] = np.ascontiguousarray(vext, dtype=np.float64).reshape(-1)
+            desc.vext = ptr(keep["vext"])
+        desc.precision = int(precision)
+        h = C.c_void_p()
+        check(load().cdft_system_create(ctx.handle, C.byref(desc), C.byref(h)), ctx.handle)
+        self.handle = h
+        self.N = int(np.prod([int(ngrid[a]) for a in range(ndim)]))
+        self.nc = int(nc)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                load().cdft_system_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def functional_eval(self, rho, want_fval=False):
+        rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        assert rho.size == self.N * self.nc
+        out = np.zeros(self.N * self.nc)
+        fres = np.zeros(1)
+        fval = np.zeros(self.N) if want_fval else None
+        check(load().cdft_functional_eval(self.handle, ptr(rho), ptr(out), ptr(fres),
+                                          ptr(fval) if want_fval else None), self.ctx.handle)
+        return out, float(fres[0]), fval
+
+    def converge(self, rho_inout, *, maxit=10000, beta=1e-2, tol=1e-4, anderson_start=1e-1,
+                 m=5, hist_cap=0):
+        assert rho_inout.dtype == np.float64 and rho_inout.flags.c_contiguous
+        assert rho_inout.size == self.N * self.nc
+        hist = np.zeros(max(hist_cap, 1))
+        o = SolverOpts(int(maxit), float(beta), float(tol), float(anderson_start), int(m),
+                       int(hist_cap), ptr(hist) if hist_cap else None)
+        r = Result()
+        check(load().cdft_converge(self.handle, ptr(rho_inout), C.byref(o), C.byref(r)),
+              self.ctx.handle)
+        out = r.as_dict()
+        if hist_cap:
+            out["history"] = hist[:min(r.nhist, hist_cap)].copy()
+        return out
+
+
+def converge_batch(ctx, systems, rhos, *, maxit=10000, beta=1e-2, tol=1e-4, anderson_start=1e-1,
+                   m=5, device_ptrs=False):
+    """Batched persistent solve of many systems (cdft_converge_batch[_device]).
+
+    rhos: list of float64 numpy arrays (in/out), or device pointers (ints) when
+    device_ptrs is True."""
+    n = len(systems)
+    hs = (C.c_void_p * n)(*[s.handle.value for s in systems])
+    if device_ptrs:
+        ps = (_dp * n)(*[C.cast(C.c_void_p(int(p)), _dp) for p in rhos])
+    else:
+        for r, s in zip(rhos, systems):
+            assert r.dtype == np.float64 and r.flags.c_contiguous and r.size == s.N * s.nc
+        ps = (_dp * n)(*[ptr(r) for r in rhos])
+    o = SolverOpts(int(maxit), float(beta), float(tol), float(anderson_start), int(m), 0, None)
+    res = (Result * n)()
+    fn = load().cdft_converge_batch_device if device_ptrs else load().cdft_converge_batch
+    check(fn(ctx.handle, n, hs, ps, C.byref(o), res), ctx.handle)
+    return [r.as_dict() for r in res]

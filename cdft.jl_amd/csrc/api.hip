@@ -1,0 +1,494 @@
+// api.hip -- C ABI of the cDFT HIP library (include/cdft_hip.h): contexts, system
+// set-up, functional evaluation and the plain Euler-Lagrange / Anderson solve.
+// This is the single translation unit of the library; the kernel files are
+// included below.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "plain.hip"
+#include "batched.hip"
+
+using namespace cdft;
+
+namespace {
+
+thread_local std::string g_err;   // errors raised before a context exists
+
+int fail(cdft_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg; else g_err = msg;
+    return code;
+}
+
+inline int nblocks(long n, int bs = 256) { return (int)((n + bs - 1) / bs); }
+
+// Householder QR of the m x (m-1) block R[:, 2:m] with LAPACK's conventions
+// (dgeqrf/dlarfg, then the thin Q of dorg2r); R column-major, leading dim m.
+// Used by the Anderson downdate (downdate_aaqr!, src/utils/anderson.jl:552-583).
+void householder_downdate(double *R, int m, double *Qx)
+{
+    int nr = m, ncol = m - 1;
+    double A[CDFT_MAXM * CDFT_MAXM], tau[CDFT_MAXM], v[CDFT_MAXM];
+    for (int j = 0; j < ncol; j++)
+        for (int i = 0; i < nr; i++) A[j * CDFT_MAXM + i] = R[(j + 1) * m + i];
+    for (int j = 0; j < ncol; j++) {
+        double alpha = A[j * CDFT_MAXM + j], xn = 0.0;
+        for (int i = j + 1; i < nr; i++) xn += A[j * CDFT_MAXM + i] * A[j * CDFT_MAXM + i];
+        xn = std::sqrt(xn);
+        if (xn == 0.0) { tau[j] = 0.0; continue; }
+        double beta = -std::copysign(std::hypot(alpha, xn), alpha);
+        tau[j] = (beta - alpha) / beta;
+        double sc = 1.0 / (alpha - beta);
+        for (int i = j + 1; i < nr; i++) A[j * CDFT_MAXM + i] *= sc;
+        A[j * CDFT_MAXM + j] = beta;
+        for (int jj = j + 1; jj < ncol; jj++) {
+            double w = A[jj * CDFT_MAXM + j];
+            for (int i = j + 1; i < nr; i++) w += A[j * CDFT_MAXM + i] * A[jj * CDFT_MAXM + i];
+            w *= tau[j];
+            A[jj * CDFT_MAXM + j] -= w;
+            for (int i = j + 1; i < nr; i++) A[jj * CDFT_MAXM + i] -= w * A[j * CDFT_MAXM + i];
+        }
+    }
+    for (int j = 0; j < ncol; j++)
+        for (int i = 0; i < nr; i++) Qx[j * nr + i] = (i == j) ? 1.0 : 0.0;
+    for (int h = ncol - 1; h >= 0; h--) {
+        if (tau[h] == 0.0) continue;
+        v[h] = 1.0;
+        for (int i = h + 1; i < nr; i++) v[i] = A[h * CDFT_MAXM + i];
+        for (int j = 0; j < ncol; j++) {
+            double w = 0.0;
+            for (int i = h; i < nr; i++) w += v[i] * Qx[j * nr + i];
+            w *= tau[h];
+            for (int i = h; i < nr; i++) Qx[j * nr + i] -= w * v[i];
+        }
+    }
+    // R[1:m-1, 1:m-1] = Rd; R[:, m] = 0
+    for (int j = 0; j < ncol; j++)
+        for (int i = 0; i < ncol; i++) R[j * m + i] = (i <= j) ? A[j * CDFT_MAXM + i] : 0.0;
+    for (int i = 0; i < m; i++) R[(m - 1) * m + i] = 0.0;
+}
+
+// Composite Simpson exactly as _int (src/utils/integrals.jl:28-40), 1D.
+double simpson1d(const double *f, int n, double dz)
+{
+    double s = 0.0;
+    for (int k = 1; k <= n; k++) {
+        double c = (k == 1 || k == n) ? 1.0 : ((k % 2 == 0) ? 4.0 : 2.0);
+        s += c * f[k - 1];
+    }
+    return s * (dz / 3.0);
+}
+
+void free_sys_buffers(cdft_sys *s)
+{
+    double **bufs[] = {&s->h, &s->vext, &s->cpd, &s->rho, &s->n, &s->dphi, &s->part, &s->fval, &s->out, &s->sol,
+                       &s->gx, &s->df, &s->dg, &s->res, &s->Q, &s->Qalt, &s->DG, &s->red, &s->fin};
+    for (double **b : bufs)
+        if (*b) { (void)hipFree(*b); *b = nullptr; }
+    if (s->h32) { (void)hipFree(s->h32); s->h32 = nullptr; }
+}
+
+template <int NC>
+void launch_phi(cdft_sys *s, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_phi<NC, 1>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, s->n, s->dphi, s->fval, s->N,
+                       s->NA, s->pc);
+}
+
+// Weighted densities, Phi + gradient and transposed convolutions into s->dphi / s->part.
+void launch_functional(cdft_sys *s)
+{
+    hipStream_t st = s->ctx->stream;
+    int N = s->N, F = s->F, nc = s->nc;
+    dim3 g(nblocks(N), nc * (F - 1));
+    // forward: src = rho_c, dst = n[(c*F+f)*N]
+    hipLaunchKernelGGL(k_circ1d<double>, g, dim3(256), 0, st, s->rho, (long)N, 0L, s->h, s->n, (long)F * N,
+                       (long)N, 0, N, F, s->NA, 0);
+    switch (nc) {
+    case 1: launch_phi<1>(s, st); break;
+    case 2: launch_phi<2>(s, st); break;
+    case 3: launch_phi<3>(s, st); break;
+    default: launch_phi<4>(s, st); break;
+    }
+    // transposed: src = dphi[(c*F+f)*N], dst = part[(c*(F-1)+f-1)*N], minus on the vector slot
+    hipLaunchKernelGGL(k_circ1d<double>, g, dim3(256), 0, st, s->dphi, (long)F * N, (long)N, s->h, s->part,
+                       (long)(F - 1) * N, (long)N, 1, N, F, 1.0, 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cdft_last_error(const cdft_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int cdft_ctx_create(int device, cdft_ctx **out)
+{
+    if (!out) return fail(nullptr, CDFT_ERR_ARG, "cdft_ctx_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return fail(nullptr, CDFT_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(nullptr, CDFT_ERR_ARG, "device index out of range");
+    auto *c = new cdft_ctx;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return fail(nullptr, CDFT_ERR_HIP, "stream/event creation failed");
+    }
+    *out = c;
+    return CDFT_OK;
+}
+
+int cdft_ctx_destroy(cdft_ctx *ctx)
+{
+    if (!ctx) return CDFT_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    delete ctx;
+    return CDFT_OK;
+}
+
+double cdft_last_kernel_ms(const cdft_ctx *ctx) { return ctx ? ctx->last_kernel_ms : 0.0; }
+
+int cdft_system_npoints(const cdft_sys *sys) { return sys ? sys->N : 0; }
+
+int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
+{
+    if (!ctx || !d || !out) return fail(ctx, CDFT_ERR_ARG, "cdft_system_create: NULL argument");
+    *out = nullptr;
+    if (d->nc < 1 || d->nc > 4) return fail(ctx, CDFT_ERR_UNSUPPORTED, "nc must be in 1..4");
+    if (d->ndim != 1) return fail(ctx, CDFT_ERR_UNSUPPORTED, "only 1D Cartesian grids are supported");
+    if (d->ngrid[0] < 3) return fail(ctx, CDFT_ERR_ARG, "ngrid must be >= 3");
+    if (!(d->ub[0] > d->lb[0])) return fail(ctx, CDFT_ERR_ARG, "bounds must satisfy ub > lb");
+    if (!(d->T > 0) || !(d->L > 0)) return fail(ctx, CDFT_ERR_ARG, "T and L must be positive");
+    if (!d->m || !d->d || !d->sigma || !d->eps || !d->rho_bulk || !d->mu_res)
+        return fail(ctx, CDFT_ERR_ARG, "parameter array is NULL");
+    if (d->precision != CDFT_FP64 && d->precision != CDFT_MIXED)
+        return fail(ctx, CDFT_ERR_ARG, "unknown precision mode");
+    for (int i = 0; i < d->nc; i++)
+        if (!(d->rho_bulk[i] > 0) || !(d->d[i] > 0) || !(d->m[i] > 0))
+            return fail(ctx, CDFT_ERR_ARG, "rho_bulk, d and m must be positive");
+
+    auto *s = new cdft_sys;
+    s->ctx = ctx;
+    s->nc = d->nc;
+    s->nd = d->ndim;
+    s->N = d->ngrid[0];
+    s->F = 6 + d->ndim;
+    s->ngrid[0] = d->ngrid[0];
+    s->lb[0] = d->lb[0];
+    s->ub[0] = d->ub[0];
+    s->T = d->T;
+    s->p = d->p;
+    s->L = d->L;
+    s->psi = d->psi > 0 ? d->psi : 1.3862;
+    s->NA = 6.02214076e23 * (s->L * s->L * s->L);   // N_A * density_scale^3 (weighted_densities.jl:123)
+    s->precision = d->precision;
+    int nc = s->nc;
+    for (int i = 0; i < nc; i++) {
+        s->m[i] = d->m[i];
+        s->d[i] = d->d[i];
+        s->rho_bulk[i] = d->rho_bulk[i];
+        s->mu_res[i] = d->mu_res[i];
+        s->cp[i] = d->mu_res[i] + std::log(d->rho_bulk[i]);
+    }
+    for (int i = 0; i < nc * nc; i++) { s->sigma[i] = d->sigma[i]; s->eps[i] = d->eps[i]; }
+    PhiConst &P = s->pc;
+    std::memset(&P, 0, sizeof(P));
+    P.nc = nc;
+    P.T = s->T;
+    for (int i = 0; i < nc; i++) { P.m[i] = s->m[i]; P.dd[i] = s->d[i] / s->L; }
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) {
+            double sr = s->sigma[i * nc + j] / s->L;
+            P.cm[i * nc + j] = s->m[i] * s->m[j] * sr * sr * sr;
+            P.eT[i * nc + j] = s->eps[i * nc + j] / s->T;
+        }
+
+    int N = s->N, F = s->F;
+    long n = (long)N * nc;
+    s->nblk = nblocks(n);
+    (void)hipSetDevice(ctx->device);
+    auto alloc = [&](double **p, size_t count) { return hipMalloc((void **)p, sizeof(double) * count); };
+    hipError_t e = hipSuccess;
+    double *ddev = nullptr;
+    if ((e = alloc(&s->h, (size_t)nc * F * N)) != hipSuccess || (e = alloc(&s->rho, n)) != hipSuccess ||
+        (e = alloc(&s->n, (size_t)nc * F * N)) != hipSuccess || (e = alloc(&s->dphi, (size_t)nc * F * N)) != hipSuccess ||
+        (e = alloc(&s->part, (size_t)nc * (F - 1) * N)) != hipSuccess || (e = alloc(&s->fval, N)) != hipSuccess ||
+        (e = alloc(&s->out, n)) != hipSuccess || (e = alloc(&s->sol, n)) != hipSuccess ||
+        (e = alloc(&s->gx, n)) != hipSuccess || (e = alloc(&s->df, n)) != hipSuccess ||
+        (e = alloc(&s->dg, n)) != hipSuccess || (e = alloc(&s->res, n)) != hipSuccess ||
+        (e = alloc(&s->Q, (size_t)CDFT_MAXM * n)) != hipSuccess ||
+        (e = alloc(&s->Qalt, (size_t)CDFT_MAXM * n)) != hipSuccess ||
+        (e = alloc(&s->DG, (size_t)CDFT_MAXM * n)) != hipSuccess ||
+        (e = alloc(&s->red, (size_t)2 * CDFT_MAXM * s->nblk)) != hipSuccess ||
+        (e = alloc(&s->fin, (size_t)4 * CDFT_NQ)) != hipSuccess || (e = alloc(&s->cpd, nc)) != hipSuccess ||
+        (e = alloc(&ddev, nc)) != hipSuccess ||
+        (e = hipMemcpy(s->cpd, s->cp, sizeof(double) * nc, hipMemcpyHostToDevice)) != hipSuccess) {
+        free_sys_buffers(s);
+        if (ddev) (void)hipFree(ddev);
+        delete s;
+        return fail(ctx, e == hipErrorOutOfMemory ? CDFT_ERR_NOMEM : CDFT_ERR_HIP,
+                    std::string("device allocation failed: ") + hipGetErrorString(e));
+    }
+    if (d->vext) {
+        s->has_vext = true;
+        if ((e = alloc(&s->vext, n)) != hipSuccess ||
+            (e = hipMemcpy(s->vext, d->vext, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) {
+            free_sys_buffers(s);
+            (void)hipFree(ddev);
+            delete s;
+            return fail(ctx, CDFT_ERR_HIP, std::string("vext upload failed: ") + hipGetErrorString(e));
+        }
+    }
+    // real-space kernels from the analytic weights
+    double dd[CDFT_MAXC];
+    for (int i = 0; i < nc; i++) dd[i] = P.dd[i];
+    double mult = ((double)N / (s->ub[0] - s->lb[0])) / (double)N;   // fftfreq spacing
+    e = hipMemcpy(ddev, dd, sizeof(double) * nc, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(s->h, 0, sizeof(double) * nc * F * N, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_setup_h1d, dim3(nblocks(N), nc * F), dim3(256), 0, ctx->stream, s->h, N, nc, ddev,
+                           s->psi, mult, s->L);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ddev);
+    if (e != hipSuccess) {
+        free_sys_buffers(s);
+        delete s;
+        return fail(ctx, CDFT_ERR_HIP, std::string("kernel set-up failed: ") + hipGetErrorString(e));
+    }
+    if (s->precision == CDFT_MIXED) {
+        int rc = batched_prepare_mixed(s);
+        if (rc != CDFT_OK) {
+            free_sys_buffers(s);
+            delete s;
+            return rc;
+        }
+    }
+    *out = s;
+    return CDFT_OK;
+}
+
+int cdft_system_destroy(cdft_sys *s)
+{
+    if (!s) return CDFT_ERR_ARG;
+    (void)hipSetDevice(s->ctx->device);
+    free_sys_buffers(s);
+    delete s;
+    return CDFT_OK;
+}
+
+int cdft_functional_eval(cdft_sys *s, const double *rho, double *dfdrho, double *f_res, double *fval)
+{
+    if (!s || !rho || !dfdrho) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_functional_eval: NULL argument");
+    cdft_ctx *ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    long n = (long)s->N * s->nc;
+    HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(s->rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    launch_functional(s);
+    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, s->dphi, s->part, (const double *)nullptr,
+                       s->N, s->nc, s->F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
+                       (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0);
+    HIPCHECK_CTX(ctx, hipGetLastError());
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(dfdrho, s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    std::vector<double> fv;
+    if (f_res || fval) {
+        fv.resize(s->N);
+        HIPCHECK_CTX(ctx, hipMemcpyAsync(fv.data(), s->fval, sizeof(double) * s->N, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+    if (fval) std::memcpy(fval, fv.data(), sizeof(double) * s->N);
+    if (f_res) {
+        double dz = (s->ub[0] - s->lb[0]) / s->N;                     // structure_dz
+        *f_res = simpson1d(fv.data(), s->N, dz) / (s->L * s->L * s->L);  // F_res, models.jl:86-91
+    }
+    return CDFT_OK;
+}
+
+// converge!(prob, ::AASol, rho) with aasol (src/utils/anderson.jl:301-443): the
+// fixed-point map and every N-length vector operation run on the device; the
+// m x m QR downdate and the triangular solve run on the host, as in the reference.
+int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdft_result *r)
+{
+    if (!s || !rho_inout || !o || !r) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_converge: NULL argument");
+    cdft_ctx *ctx = s->ctx;
+    const int m = o->m;
+    const double beta = o->beta;
+    if (m < 0 || m > CDFT_MAXM - 1) return fail(ctx, CDFT_ERR_ARG, "anderson m must be in 0..7");
+    if (!(std::fabs(beta) > 0 && std::fabs(beta) <= 1)) return fail(ctx, CDFT_ERR_ARG, "abs(beta) must be in (0,1]");
+    if (o->maxit < 0) return fail(ctx, CDFT_ERR_ARG, "maxit must be >= 0");
+    hipStream_t st = ctx->stream;
+    const long n = (long)s->N * s->nc;
+    const int nb = s->nblk;
+    const int maxit = o->maxit, picard_maxit = o->maxit;
+    const double rtol = o->tol, atol = o->tol, picard_beta = beta;
+    const double picard_rtol = o->anderson_start, picard_atol = o->anderson_start;
+    HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
+
+    double *fin_h = nullptr;
+    HIPCHECK_CTX(ctx, hipHostMalloc((void **)&fin_h, sizeof(double) * 4 * CDFT_NQ));
+    struct HostFree { double *p; ~HostFree() { if (p) (void)hipHostFree(p); } } hf{fin_h};
+    int nh = 0, nevals = 0;
+    auto hist = [&](double v) { if (o->hist && nh < o->hist_cap) o->hist[nh] = v; nh++; };
+
+    // G evaluation + post-processing; returns the finalized reductions in fin_h[0..2]
+    auto evalG = [&](int mode, double b) -> int {
+        launch_functional(s);
+        hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, s->dphi, s->part, s->vext, s->N, s->nc, s->F,
+                           s->cpd, s->out, s->sol, s->gx, s->res, s->df, s->dg, b, mode, s->red, nb);
+        hipLaunchKernelGGL(k_finalize, dim3(3), dim3(256), 0, st, s->red, nb, s->fin);
+        HIPCHECK_CTX(ctx, hipGetLastError());
+        HIPCHECK_CTX(ctx, hipMemcpyAsync(fin_h, s->fin, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+        HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+        nevals++;
+        return CDFT_OK;
+    };
+#define EVAL(mode, b) do { int rc_ = evalG((mode), (b)); if (rc_ != CDFT_OK) return rc_; } while (0)
+
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(s->out, rho_inout, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_init_x, dim3(nb), dim3(256), 0, st, s->out, s->sol, s->rho, (int)n);
+
+    int k = 0;
+    EVAL(1, beta);
+    double resnorm = std::sqrt(fin_h[0]);
+    const double up_bd = 1.e4 * resnorm;
+    const double tol = rtol * resnorm + atol;
+    const double picard_tol = picard_rtol * resnorm + picard_atol;
+    hist(resnorm);
+    bool toosoon = (resnorm <= tol);
+
+    if (!toosoon && picard_maxit > 0) {
+        hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, s->rho, (int)n);
+        for (int it = 0; it < picard_maxit - 1; it++) {
+            EVAL(1, picard_beta);
+            resnorm = std::sqrt(fin_h[0]);
+            hist(resnorm);
+            if (resnorm <= picard_tol || resnorm >= up_bd) break;
+            hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, s->rho, (int)n);
+        }
+        toosoon = (resnorm <= tol);
+    }
+
+    double R[CDFT_MAXM * CDFT_MAXM] = {0};
+    int ring[CDFT_MAXM];
+    for (int j = 0; j < CDFT_MAXM; j++) ring[j] = j;
+    double dfn2 = 0.0, dfres = 0.0;
+    if (!toosoon) {
+        hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, s->rho, (int)n);
+        k += 1;
+        EVAL(2, beta);
+        resnorm = std::sqrt(fin_h[0]);
+        dfn2 = fin_h[1];
+        dfres = fin_h[2];
+        hist(resnorm);
+    }
+
+    double *Qc = s->Q, *Qa = s->Qalt;
+    while (k < maxit + picard_maxit && resnorm > tol && !toosoon && resnorm < up_bd) {
+        if (m == 0) {
+            hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, s->rho, (int)n);
+        } else {
+            // BuildDG!(DG, m, k+1, dg) with a ring of physical columns
+            int kk = k + 1, slot;
+            if (m == 1) {
+                slot = ring[0];
+            } else if (kk > m + 1) {
+                int first = ring[0];
+                for (int j = 0; j < m - 1; j++) ring[j] = ring[j + 1];
+                ring[m - 1] = first;
+                slot = ring[m - 1];
+            } else {
+                slot = ring[kk - 2];
+            }
+            HIPCHECK_CTX(ctx, hipMemcpyAsync(s->DG + (size_t)slot * n, s->dg, sizeof(double) * n,
+                                             hipMemcpyDeviceToDevice, st));
+            // aa_qr_update!(Q, R, df, m, k-1, Qd)
+            int kq = k - 1, qcol;
+            double nrm, tres[CDFT_MAXM];
+            if (kq == 0) {
+                nrm = std::sqrt(dfn2);
+                R[0] = nrm;
+                qcol = 0;
+                tres[0] = dfres / nrm;
+            } else {
+                AAMat qx;
+                std::memset(&qx, 0, sizeof(qx));
+                int down = (kq > m - 1);
+                if (down) householder_downdate(R, m, qx.a);
+                int kqq = std::min(kq, m - 1);
+                hipLaunchKernelGGL(k_aa_proj1, dim3(nb), dim3(256), 0, st, (int)n, Qc, Qa, qx, down, m, kqq, s->df,
+                                   s->res, s->red, nb);
+                hipLaunchKernelGGL(k_finalize, dim3(2 * CDFT_MAXM), dim3(256), 0, st, s->red, nb, s->fin);
+                if (down) std::swap(Qc, Qa);
+                hipLaunchKernelGGL(k_aa_proj2, dim3(nb), dim3(256), 0, st, (int)n, Qc, kqq, s->df, s->res, s->fin, 0,
+                                   s->red, nb);
+                hipLaunchKernelGGL(k_finalize, dim3(CDFT_MAXM), dim3(256), 0, st, s->red, nb, s->fin + CDFT_NQ);
+                hipLaunchKernelGGL(k_aa_proj2, dim3(nb), dim3(256), 0, st, (int)n, Qc, kqq, s->df, s->res,
+                                   s->fin + CDFT_NQ, 1, s->red, nb);
+                hipLaunchKernelGGL(k_finalize, dim3(2), dim3(256), 0, st, s->red, nb, s->fin + 2 * CDFT_NQ);
+                HIPCHECK_CTX(ctx, hipGetLastError());
+                HIPCHECK_CTX(ctx, hipMemcpyAsync(fin_h, s->fin, sizeof(double) * 3 * CDFT_NQ, hipMemcpyDeviceToHost, st));
+                HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+                const double *h1 = fin_h, *tq = fin_h + CDFT_MAXM, *h2 = fin_h + CDFT_NQ;
+                for (int j = 0; j < kqq; j++) R[kqq * m + j] = h1[j] + h2[j];
+                nrm = std::sqrt(fin_h[2 * CDFT_NQ]);
+                R[kqq * m + kqq] = nrm;
+                qcol = kqq;
+                for (int j = 0; j < kqq; j++) tres[j] = tq[j];
+                tres[kqq] = fin_h[2 * CDFT_NQ + 1] / nrm;
+            }
+            int mk = std::min(m, k);
+            AAMat th;
+            std::memset(&th, 0, sizeof(th));
+            for (int j = mk - 1; j >= 0; j--) {          // theta = RA \ tres
+                double acc = tres[j];
+                for (int l = j + 1; l < mk; l++) acc -= R[l * m + j] * th.a[l];
+                th.a[j] = acc / R[j * m + j];
+            }
+            for (int j = 0; j < mk; j++) th.col[j] = ring[j];
+            hipLaunchKernelGGL(k_aa_newpoint, dim3(nb), dim3(256), 0, st, (int)n, s->df, nrm,
+                               Qc + (size_t)qcol * n, s->sol, s->gx, s->DG, th, mk, s->rho);
+        }
+        k += 1;
+        EVAL(2, beta);
+        resnorm = std::sqrt(fin_h[0]);
+        dfn2 = fin_h[1];
+        dfres = fin_h[2];
+        hist(resnorm);
+    }
+#undef EVAL
+
+    int idid, errcode;
+    if (toosoon) { idid = 1; errcode = -1; }
+    else if (resnorm >= up_bd) { idid = 0; errcode = -2; }
+    else if (resnorm <= tol) { idid = 1; errcode = 0; }
+    else { idid = 0; errcode = 10; }
+
+    hipLaunchKernelGGL(k_exp, dim3(nb), dim3(256), 0, st, s->sol, s->out, (int)n);
+    HIPCHECK_CTX(ctx, hipGetLastError());
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(rho_inout, s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+    r->idid = idid;
+    r->errcode = errcode;
+    r->iters = k;
+    r->nevals = nevals;
+    r->nhist = nh;
+    r->resnorm = resnorm;
+    r->tol = tol;
+    return CDFT_OK;
+}
+
+}  // extern "C"

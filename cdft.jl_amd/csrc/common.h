@@ -1,0 +1,62 @@
+// common.h -- shared host/device definitions of the cDFT HIP library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "cdft_hip.h"
+#include "phi.h"
+
+#define CDFT_MAXM 8          // largest Anderson depth supported
+#define CDFT_NQ 16           // reduction slots per vector pass
+
+struct cdft_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    double last_kernel_ms = 0.0;
+};
+
+// Per-system state.  Host copies of the descriptor plus device buffers.
+struct cdft_sys {
+    cdft_ctx *ctx = nullptr;
+    int nc = 0, nd = 0, N = 0, F = 0;
+    int ngrid[3] = {1, 1, 1};
+    double lb[3] = {0, 0, 0}, ub[3] = {0, 0, 0};
+    double T = 0, p = 0, L = 0, psi = 1.3862, NA = 0;
+    double m[CDFT_MAXC], d[CDFT_MAXC], rho_bulk[CDFT_MAXC], mu_res[CDFT_MAXC];
+    double sigma[CDFT_MAXC * CDFT_MAXC], eps[CDFT_MAXC * CDFT_MAXC];
+    double cp[CDFT_MAXC];      // mu_res + ln rho_bulk (converge.jl:121)
+    int precision = CDFT_FP64;
+    bool has_vext = false;
+    PhiConst pc;
+
+    // device buffers (FP64)
+    double *h = nullptr;       // real-space kernels [(c*F+f)*N + j], f >= 1
+    float *h32 = nullptr;      // FP32 copy for the mixed-precision batched path
+    double *vext = nullptr;    // [c*N + i]
+    double *cpd = nullptr;     // device copy of cp [nc]
+    double *rho = nullptr;     // current profile
+    double *n = nullptr;       // weighted densities [(c*F+f)*N + i]
+    double *dphi = nullptr;    // dPhi/dn, same layout
+    double *part = nullptr;    // per-field transposed convolutions [(c*(F-1)+f-1)*N + i]
+    double *fval = nullptr;    // Phi [N]
+    double *out = nullptr;     // dF/drho or lnG [c*N + i]
+    // Anderson storage (anderson.jl:665-703)
+    double *sol = nullptr, *gx = nullptr, *df = nullptr, *dg = nullptr, *res = nullptr;
+    double *Q = nullptr, *Qalt = nullptr, *DG = nullptr;   // m columns each
+    double *red = nullptr;     // block partials
+    double *fin = nullptr;     // finalized reductions [CDFT_NQ]
+    int nblk = 0;
+};
+
+#define HIPCHECK_CTX(ctx, call)                                                              \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                 \
+            return CDFT_ERR_HIP;                                                              \
+        }                                                                                     \
+    } while (0)

@@ -1,0 +1,141 @@
+/*
+ * cdft_hip.h -- C ABI of the MI355X-native PC-SAFT classical-DFT hot path.
+ *
+ * The library evaluates the PC-SAFT free-energy functional (weighted densities,
+ * pointwise Helmholtz density and its gradient, transposed convolutions) and runs
+ * the Euler-Lagrange Picard/Anderson fixed-point loop on an AMD Instinct MI355X
+ * (gfx950).  It is the drop-in for this call chain of cDFT.jl:
+ *
+ *   converge!(system, rho; maxit, beta, tol, anderson_start, anderson_m)
+ *                                                    src/methods/converge.jl:214-254
+ *     -> aasol(GFix!, ln_X0, method)                 src/utils/anderson.jl:301-443
+ *       -> get_new_profile!(system, rho, ...)        src/methods/converge.jl:108-153
+ *         -> dFdrho_res!(system, rho, ...)            src/models/models.jl:131-186
+ *
+ * Conventions
+ *  - Every function returns CDFT_OK (0) or a negative error code; the message of
+ *    the last error on a context is returned by cdft_last_error().
+ *  - Arrays are column-major like the reference's Julia arrays: a profile
+ *    rho(ngrid..., nc) is rho[c*N + idx] with idx = i0 + n0*(i1 + n1*i2).
+ *  - Host pointers are read during the call only; the library keeps no caller
+ *    pointer after it returns.  The *_device variants take device pointers.
+ *  - One HIP stream per context.  A context is not re-entrant; separate contexts
+ *    may be used from separate threads.
+ *  - Physical inputs use the reference's units: lengths in m, temperature in K,
+ *    densities in mol/m^3, chemical potentials as beta*mu (dimensionless).
+ */
+#ifndef CDFT_HIP_H
+#define CDFT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CDFT_OK 0
+#define CDFT_ERR_ARG (-1)
+#define CDFT_ERR_HIP (-2)
+#define CDFT_ERR_UNSUPPORTED (-3)
+#define CDFT_ERR_NOMEM (-4)
+
+/* precision modes (DFTOptions(...; precision), src/base/devices.jl:29-40) */
+#define CDFT_FP64 0   /* everything in FP64 (the reference's default)                */
+#define CDFT_MIXED 1  /* FP32 convolutions and Helmholtz density; FP64 n3-singular   */
+                      /* terms, exponent assembly, Anderson and residual norms       */
+
+#define CDFT_MAXC 8
+
+typedef struct cdft_ctx cdft_ctx;
+typedef struct cdft_sys cdft_sys;
+
+/* Everything DFTSystem(model, structure[, external_field][, options]) carries that
+ * the hot path reads (src/base/base.jl:50-89).  The bulk quantities come from the
+ * caller's equation of state, as the reference takes them from Clapeyron
+ * (get_species, src/models/DFT/SAFT/PCSAFT/PCSAFT.jl:65-72). */
+typedef struct {
+    int nc;                 /* components, one bead each (PC-SAFT)               */
+    int ndim;               /* 1 or 3 Cartesian dimensions                       */
+    int ngrid[3];           /* grid points per axis (unused axes ignored)        */
+    double lb[3], ub[3];    /* bounds per axis, m                                */
+    double T;               /* temperature, K                                    */
+    double p;               /* pressure, Pa (carried for post-processing)       */
+    double L;               /* length scale max(sigma), m (PCSAFT.jl:79-81)      */
+    double psi;             /* dispersion width factor, 1.3862 (PCSAFT.jl:53)   */
+    const double *m;        /* [nc] segment numbers                              */
+    const double *d;        /* [nc] hard-sphere diameters at T, m                */
+    const double *sigma;    /* [nc*nc] sigma_ij, m                               */
+    const double *eps;      /* [nc*nc] eps_ij / k_B, K                           */
+    const double *rho_bulk; /* [nc] bulk densities, mol/m^3                      */
+    const double *mu_res;   /* [nc] beta mu_res of the bulk                      */
+    const double *vext;     /* [N*nc] beta V_ext (external_field.jl:6-19), NULL  */
+    int precision;          /* CDFT_FP64 or CDFT_MIXED                           */
+} cdft_system_desc;
+
+/* converge! keywords (src/methods/converge.jl:3-13, :51-60). */
+typedef struct {
+    int maxit;             /* 10000                                               */
+    double beta;           /* 1e-2 damping (Picard and Anderson)                  */
+    double tol;            /* 1e-4; used as rtol and atol                         */
+    double anderson_start; /* 1e-1; picard_rtol and picard_atol                   */
+    int m;                 /* 5 Anderson depth (0 = damped Picard)                */
+    int hist_cap;          /* capacity of hist (0: no history)                    */
+    double *hist;          /* residual-norm history out (host), or NULL           */
+} cdft_solver_opts;
+
+/* aasol's outcome (AndersonOK, src/utils/anderson.jl:727-737). */
+typedef struct {
+    int idid;              /* 1 converged, 0 not                                  */
+    int errcode;           /* 0 ok, -1 initial iterate within tol, -2 diverged,   */
+                           /* 10 maxit reached                                    */
+    int iters;             /* Anderson iteration counter k                        */
+    int nevals;            /* fixed-point map evaluations                          */
+    int nhist;             /* residual norms recorded                             */
+    double resnorm;        /* final ||beta (G(x) - x)||_2                          */
+    double tol;            /* rtol*res0 + atol                                    */
+} cdft_result;
+
+/* Context: device selection and one HIP stream.  (No counterpart: the
+ * reference's device is the KernelAbstractions backend, devices.jl:29-40.) */
+int cdft_ctx_create(int device, cdft_ctx **out);
+int cdft_ctx_destroy(cdft_ctx *ctx);
+const char *cdft_last_error(const cdft_ctx *ctx);
+
+/* System set-up: grid, analytic Fourier weights and their real-space kernels,
+ * external potential, device buffers.  Replaces DFTSystem(...) + preallocate
+ * (src/base/base.jl:61-89, src/base/devices.jl:51-102,
+ * src/fields/weighted_densities.jl:28-80,193-246). */
+int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *desc, cdft_sys **out);
+int cdft_system_destroy(cdft_sys *sys);
+int cdft_system_npoints(const cdft_sys *sys);
+
+/* One functional evaluation: dfdrho = delta F_res / delta rho (no external
+ * field), and optionally F_res (Simpson integral of Phi / L^3) and the pointwise
+ * Phi (fval, length N).  Replaces dFdrho_res(system, rho) + F_res
+ * (src/models/models.jl:86-91,131-200). */
+int cdft_functional_eval(cdft_sys *sys, const double *rho, double *dfdrho, double *f_res, double *fval);
+
+/* Euler-Lagrange fixed point: rho is overwritten with the converged profile.
+ * Plain path: one system, per-operator kernels, Anderson control on the host
+ * with device reductions.  Replaces converge!(prob, ::AASol, rho)
+ * (src/methods/converge.jl:220-254) and aasol (src/utils/anderson.jl:301-443). */
+int cdft_converge(cdft_sys *sys, double *rho_inout, const cdft_solver_opts *opts, cdft_result *result);
+
+/* Batched path: nsys independent systems (any mix of 1D grid sizes), one
+ * persistent launch, one workgroup per system with its Anderson history resident
+ * on the device; systems retire independently.  Each system follows exactly the
+ * same algorithm as cdft_converge.  (No batched counterpart in the reference: it
+ * replaces a loop of converge! calls, e.g. over adsorption(...) state points,
+ * src/methods/adsorption.jl:48-66.)  hist in opts is ignored. */
+int cdft_converge_batch(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_inout,
+                        const cdft_solver_opts *opts, cdft_result *results);
+/* Same with rho_inout as device pointers (inputs resident in HBM). */
+int cdft_converge_batch_device(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_dev,
+                               const cdft_solver_opts *opts, cdft_result *results);
+
+/* Kernel time of the last cdft_converge_batch* launch in ms, measured with HIP
+ * events on the context's stream. */
+double cdft_last_kernel_ms(const cdft_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
