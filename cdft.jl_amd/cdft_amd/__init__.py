@@ -1,0 +1,13 @@
+"""cdft_amd -- MI355X-native PC-SAFT classical DFT hot path (functional evaluation and
+Euler-Lagrange Picard/Anderson fixed point) behind cDFT.jl's DFTSystem /
+initialize_profiles / converge! API.  Compute runs in libcdft_hip.so (HIP, gfx950);
+there is no CPU fallback."""
+from .bulk import KB, NA, RGAS, PCSAFT
+from .structure import Steele, Structure, TwoPhase1DCart, Uniform1DCart
+from .system import (DFTOptions, DFTSystem, F_ideal, F_res, converge_, dFdrho_res, free_energy,
+                     initialize_profiles, simpson)
+from .methods import adsorption, surface_tension
+
+__all__ = ["PCSAFT", "Steele", "Structure", "Uniform1DCart", "TwoPhase1DCart", "DFTOptions", "DFTSystem",
+           "initialize_profiles", "converge_", "dFdrho_res", "F_res", "F_ideal", "free_energy", "simpson",
+           "surface_tension", "adsorption", "NA", "KB", "RGAS"]

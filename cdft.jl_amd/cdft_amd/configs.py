@@ -1,0 +1,63 @@
+"""The benchmark workloads, built with the host API.
+
+  methane_wall    methane at 150 K, 1e7 Pa next to a graphite Steele wall pair 5 sigma
+                  apart; 201 points over [0.5 sigma, 4.5 sigma] (README example,
+                  examples/docs_figures/01_getting_started.jl)
+  c1c10_interface methane / n-decane planar vapour-liquid interface at 300 K with liquid
+                  composition x = [0.5, 0.5] at its bubble point (surface_tension(model, T, x)
+                  semantics, src/methods/surface_tension.jl:13-29); 2048 points over
+                  [-20 L, 20 L]; reference cos_prof seed (two_phase.jl:8-35)
+  slit_pores      512 methane slit pores between two graphite Steele walls at 150 K,
+                  widths W ~ U[2 sigma, 12 sigma], p ~ logU[1e3, 1e7] Pa from seed 0,
+                  stable bulk root, N = round(40 W / sigma) points over [0.5 L, W - 0.5 L]
+"""
+import numpy as np
+
+from .bulk import PCSAFT
+from .structure import Steele, TwoPhase1DCart, Uniform1DCart
+from .system import DFTOptions, DFTSystem, initialize_profiles
+
+
+def methane_wall(options=None, ngrid=201):
+    model = PCSAFT(["methane"])
+    T, p = 150.0, 1e7
+    v = model.volume(p, T, [1.0], phase="liquid")
+    L = model.length_scale()
+    width = 5 * L
+    surface = Steele(["graphite"], width)
+    structure = Uniform1DCart((p, T), [1 / v], [0.5 * L, width - 0.5 * L], ngrid)
+    system = DFTSystem(model, structure, surface, options)
+    return system, initialize_profiles(system)
+
+
+def c1c10_interface(options=None, ngrid=2048, box=20.0, x=(0.5, 0.5)):
+    model = PCSAFT(["methane", "n-decane"])
+    T = 300.0
+    x = np.asarray(x, dtype=np.float64)
+    p, vl, vv, y = model.bubble_pressure(T, x)
+    L = model.length_scale()
+    structure = TwoPhase1DCart((p, T), x / vl, y / vv, [-box * L, box * L], ngrid)
+    system = DFTSystem(model, structure, None, options)
+    return system, initialize_profiles(system)
+
+
+def slit_pore_params(n=512, seed=0):
+    rng = np.random.default_rng(seed)
+    W = rng.uniform(2.0, 12.0, size=n)            # in units of sigma
+    p = 10 ** rng.uniform(3.0, 7.0, size=n)       # Pa
+    return W, p
+
+
+def slit_pores(n=512, seed=0, options=None, T=150.0):
+    model = PCSAFT(["methane"])
+    sigma = model.length_scale()
+    Ws, ps = slit_pore_params(n, seed)
+    out = []
+    for W, p in zip(Ws, ps):
+        width = W * sigma
+        v = model.volume(p, T, [1.0])
+        N = int(round(40 * W))
+        structure = Uniform1DCart((p, T), [1 / v], [0.5 * sigma, width - 0.5 * sigma], N)
+        system = DFTSystem(model, structure, Steele(["graphite"], width), options)
+        out.append((system, initialize_profiles(system)))
+    return out

@@ -1,0 +1,69 @@
+"""Post-processing callers of converge!: surface tension and adsorption.
+
+  surface_tension(model, T, x)        src/methods/surface_tension.jl:13-29
+  surface_tension(system, rho)        src/methods/surface_tension.jl:31-53
+  adsorption(model, surface, p, T, n) src/methods/adsorption.jl:48-66
+  adsorption(system, rho)             src/methods/adsorption.jl:22-32
+"""
+import numpy as np
+
+from .bulk import KB
+from .structure import TwoPhase1DCart, Uniform1DCart
+from .system import DFTSystem, converge_, free_energy, initialize_profiles, simpson
+
+
+def surface_tension_system(system, rho):
+    """F k_B T - sum_i mu_i int rho_i + p int 1 (for the periodic box: two interfaces)."""
+    rho = np.asarray(rho, dtype=np.float64).reshape(system.nc, system.N)
+    st = system.structure
+    p, T = st.conditions
+    rl = st.rho_bulk
+    x = rl / rl.sum()
+    vl = 1.0 / rl.sum()
+    F = free_energy(system, rho)
+    mu = system.model.VT_chemical_potential(vl, T, x)
+    dz = st.dz()
+    chem = sum(mu[i] * simpson(rho[i], dz) for i in range(system.nc))
+    return F * KB * T - chem + p * simpson(np.ones(system.N), dz)
+
+
+def surface_tension(model_or_system, T_or_rho, x=None, *, ngrid=201, box=10.0, options=None, **kw):
+    """surface_tension(model, T, x=[1.0]) -> gamma (N/m), or surface_tension(system, rho)."""
+    if isinstance(model_or_system, DFTSystem):
+        return surface_tension_system(model_or_system, T_or_rho)
+    model, T = model_or_system, float(T_or_rho)
+    x = np.array([1.0]) if x is None else np.asarray(x, dtype=np.float64)
+    L = model.length_scale()
+    p, vl, vv, y = model.bubble_pressure(T, x)
+    rho1 = x / vl
+    rho2 = y / vv
+    structure = TwoPhase1DCart((p, T), rho1, rho2, [-box * L, box * L], (ngrid,))
+    system = DFTSystem(model, structure, None, options)
+    rho = initialize_profiles(system)
+    converge_(system, rho, **kw)
+    return surface_tension_system(system, rho) / 2
+
+
+def adsorption_system(system, rho):
+    rho = np.asarray(rho, dtype=np.float64).reshape(system.nc, system.N)
+    lb, ub = system.structure.bounds[0]
+    V = ub - lb
+    return np.array([simpson(rho[i], system.structure.dz()) / V for i in range(system.nc)])
+
+
+def adsorption(model_or_system, surface_or_rho, p=None, T=None, n=(1.0,), *, options=None, **kw):
+    """adsorption(model, surface, p, T, n=[1.0]) or adsorption(system, rho)."""
+    if isinstance(model_or_system, DFTSystem):
+        return adsorption_system(model_or_system, surface_or_rho)
+    model, surface = model_or_system, surface_or_rho
+    L = model.length_scale()
+    width = surface.width
+    bounds = [0.7 * L, width - 0.7 * L]
+    n = np.asarray(n, dtype=np.float64)
+    v = model.volume(p, T, n)
+    rho_b = n / v
+    structure = Uniform1DCart((p, T), rho_b, bounds, (201,))
+    system = DFTSystem(model, structure, surface, options)
+    rho = initialize_profiles(system)
+    converge_(system, rho, **kw)
+    return adsorption_system(system, rho)

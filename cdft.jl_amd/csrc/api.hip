@@ -92,7 +92,8 @@ void free_sys_buffers(cdft_sys *s)
                        &s->gx, &s->df, &s->dg, &s->res, &s->Q, &s->Qalt, &s->DG, &s->red, &s->fin};
     for (double **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
-    if (s->h32) { (void)hipFree(s->h32); s->h32 = nullptr; }
+    if (s->H64) { (void)hipFree(s->H64); s->H64 = nullptr; }
+    if (s->H32s) { (void)hipFree(s->H32s); s->H32s = nullptr; }
 }
 
 template <int NC>
@@ -154,6 +155,10 @@ int cdft_ctx_destroy(cdft_ctx *ctx)
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (auto &kv : ctx->twiddles) {
+        (void)hipFree(kv.second.first);
+        (void)hipFree(kv.second.second);
+    }
     delete ctx;
     return CDFT_OK;
 }
@@ -268,14 +273,6 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
         free_sys_buffers(s);
         delete s;
         return fail(ctx, CDFT_ERR_HIP, std::string("kernel set-up failed: ") + hipGetErrorString(e));
-    }
-    if (s->precision == CDFT_MIXED) {
-        int rc = batched_prepare_mixed(s);
-        if (rc != CDFT_OK) {
-            free_sys_buffers(s);
-            delete s;
-            return rc;
-        }
     }
     *out = s;
     return CDFT_OK;
@@ -488,7 +485,192 @@ int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdf
     r->nhist = nh;
     r->resnorm = resnorm;
     r->tol = tol;
+    r->nevals_fp32 = 0;
+    r->reserved = 0;
     return CDFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched persistent path
+// ---------------------------------------------------------------------------
+static int fft_length(int N)
+{
+    if ((N & (N - 1)) == 0) return N;          // power of two: the period itself
+    int M = 1;
+    while (M < 2 * N - 1) M <<= 1;
+    return M;
+}
+
+static size_t batched_lds_bytes(const cdft_sys *s)
+{
+    return (size_t)2 * s->M * 16 + (size_t)s->F * s->N * s->nc * 8 + 64 * 8;
+}
+
+static int batched_prepare(cdft_ctx *ctx, cdft_sys *s)
+{
+    hipStream_t st = ctx->stream;
+    if (s->M == 0) {
+        int M = fft_length(s->N);
+        if (M > 4096) return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched path: grid too large (FFT length > 4096)");
+        size_t cnt = (size_t)s->nc * (s->F - 1) * M * 2;
+        HIPCHECK_CTX(ctx, hipMalloc((void **)&s->H64, sizeof(double) * cnt));
+        HIPCHECK_CTX(ctx, hipMalloc((void **)&s->H32s, sizeof(float) * cnt));
+        hipLaunchKernelGGL(k_setup_spec, dim3(nblocks(M), s->nc * (s->F - 1)), dim3(256), 0, st, s->h, s->N, M, s->F,
+                           s->H64, s->H32s);
+        HIPCHECK_CTX(ctx, hipGetLastError());
+        s->M = M;
+    }
+    if (!ctx->twiddles.count(s->M)) {
+        double *t64 = nullptr;
+        float *t32 = nullptr;
+        HIPCHECK_CTX(ctx, hipMalloc((void **)&t64, sizeof(double) * 2 * s->M));
+        HIPCHECK_CTX(ctx, hipMalloc((void **)&t32, sizeof(float) * 2 * s->M));
+        hipLaunchKernelGGL(k_setup_tw, dim3(nblocks(s->M)), dim3(256), 0, st, s->M, t64, t32);
+        HIPCHECK_CTX(ctx, hipGetLastError());
+        ctx->twiddles[s->M] = {t64, t32};
+    }
+    return CDFT_OK;
+}
+
+static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho, const cdft_solver_opts *o,
+                      cdft_result *results, bool device_ptrs)
+{
+    if (!ctx || nsys < 0 || (nsys > 0 && (!sys || !rho || !results)) || !o)
+        return fail(ctx, CDFT_ERR_ARG, "cdft_converge_batch: NULL argument");
+    if (nsys == 0) return CDFT_OK;
+    if (o->m < 0 || o->m > CDFT_MAXM - 1) return fail(ctx, CDFT_ERR_ARG, "anderson m must be in 0..7");
+    if (!(std::fabs(o->beta) > 0 && std::fabs(o->beta) <= 1)) return fail(ctx, CDFT_ERR_ARG, "abs(beta) must be in (0,1]");
+    if (o->maxit < 0) return fail(ctx, CDFT_ERR_ARG, "maxit must be >= 0");
+    const int nc = sys[0] ? sys[0]->nc : 0;
+    const int prec = sys[0] ? sys[0]->precision : 0;
+    for (int i = 0; i < nsys; i++) {
+        if (!sys[i] || !rho[i]) return fail(ctx, CDFT_ERR_ARG, "cdft_converge_batch: NULL system or profile");
+        if (sys[i]->ctx != ctx) return fail(ctx, CDFT_ERR_ARG, "all systems must belong to the context");
+        if (sys[i]->nc != nc) return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched systems must share nc");
+        if (sys[i]->precision != prec) return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched systems must share precision");
+        for (int j = 0; j < i; j++)
+            if (sys[j] == sys[i]) return fail(ctx, CDFT_ERR_ARG, "a system appears twice in the batch");
+    }
+    HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    int maxlds = 0;
+    HIPCHECK_CTX(ctx, hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+    size_t lds = 0;
+    for (int i = 0; i < nsys; i++) {
+        int rc = batched_prepare(ctx, sys[i]);
+        if (rc != CDFT_OK) return rc;
+        lds = std::max(lds, batched_lds_bytes(sys[i]));
+    }
+    size_t lds_limit = std::max<size_t>((size_t)maxlds, 160 * 1024);
+    if (lds + sizeof(AState) > lds_limit)
+        return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched path: system too large for one workgroup's LDS");
+    // largest systems first: the tail of the launch is then the short ones
+    std::vector<int> order(nsys);
+    for (int i = 0; i < nsys; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sys[a]->N > sys[b]->N; });
+    std::vector<BSys> hb(nsys);
+    for (int q = 0; q < nsys; q++) {
+        cdft_sys *s = sys[order[q]];
+        BSys &b = hb[q];
+        std::memset(&b, 0, sizeof(b));
+        b.N = s->N;
+        b.nc = s->nc;
+        b.M = s->M;
+        int lg = 0;
+        while ((1 << lg) < s->M) lg++;
+        b.log2M = lg;
+        b.NA = s->NA;
+        for (int c = 0; c < s->nc; c++) b.cp[c] = s->cp[c];
+        b.vext = s->vext;
+        b.H64 = s->H64;
+        b.H32 = s->H32s;
+        b.tw64 = ctx->twiddles[s->M].first;
+        b.tw32 = ctx->twiddles[s->M].second;
+        long n = (long)s->N * s->nc;
+        if (device_ptrs) {
+            b.rho = rho[order[q]];
+        } else {
+            HIPCHECK_CTX(ctx, hipMemcpyAsync(s->rho, rho[order[q]], sizeof(double) * n, hipMemcpyHostToDevice, st));
+            b.rho = s->rho;
+        }
+        b.sol = s->sol;
+        b.gx = s->gx;
+        b.df = s->df;
+        b.dg = s->dg;
+        b.res = s->res;
+        b.Q = s->Q;
+        b.DG = s->DG;
+        b.pc = s->pc;
+    }
+    BSys *db = nullptr;
+    BResult *dr = nullptr;
+    HIPCHECK_CTX(ctx, hipMalloc((void **)&db, sizeof(BSys) * nsys));
+    struct Free { void *p; ~Free() { if (p) (void)hipFree(p); } } f1{db};
+    HIPCHECK_CTX(ctx, hipMalloc((void **)&dr, sizeof(BResult) * nsys));
+    Free f2{dr};
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(db, hb.data(), sizeof(BSys) * nsys, hipMemcpyHostToDevice, st));
+    BOpts bo;
+    bo.maxit = o->maxit;
+    bo.m = o->m;
+    bo.beta = o->beta;
+    bo.tol = o->tol;
+    bo.anderson_start = o->anderson_start;
+    bo.mixed = (prec == CDFT_MIXED);
+    bo.switch_factor = 4.0;
+    bo.stall_window = 40;
+    auto launch = [&](auto kern) -> int {
+        HIPCHECK_CTX(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev0, st));
+        hipLaunchKernelGGL(kern, dim3(nsys), dim3(BT), lds, st, db, bo, dr);
+        HIPCHECK_CTX(ctx, hipGetLastError());
+        HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev1, st));
+        return CDFT_OK;
+    };
+    int rc;
+    switch (nc) {
+    case 1: rc = launch(k_batched<1>); break;
+    case 2: rc = launch(k_batched<2>); break;
+    case 3: rc = launch(k_batched<3>); break;
+    default: rc = launch(k_batched<4>); break;
+    }
+    if (rc != CDFT_OK) return rc;
+    std::vector<BResult> hr(nsys);
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(hr.data(), dr, sizeof(BResult) * nsys, hipMemcpyDeviceToHost, st));
+    if (!device_ptrs)
+        for (int q = 0; q < nsys; q++) {
+            cdft_sys *s = sys[order[q]];
+            HIPCHECK_CTX(ctx, hipMemcpyAsync(rho[order[q]], s->rho, sizeof(double) * s->N * s->nc,
+                                             hipMemcpyDeviceToHost, st));
+        }
+    HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+    float ms = 0.0f;
+    HIPCHECK_CTX(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_kernel_ms = ms;
+    for (int q = 0; q < nsys; q++) {
+        cdft_result &r = results[order[q]];
+        r.idid = hr[q].idid;
+        r.errcode = hr[q].errcode;
+        r.iters = hr[q].iters;
+        r.nevals = hr[q].nevals;
+        r.nhist = 0;
+        r.resnorm = hr[q].resnorm;
+        r.tol = hr[q].tol;
+        r.nevals_fp32 = hr[q].nevals32;
+        r.reserved = 0;
+    }
+    return CDFT_OK;
+}
+
+int cdft_converge_batch(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_inout,
+                        const cdft_solver_opts *opts, cdft_result *results)
+{
+    return batch_impl(ctx, nsys, sys, rho_inout, opts, results, false);
+}
+
+int cdft_converge_batch_device(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_dev,
+                               const cdft_solver_opts *opts, cdft_result *results)
+{
+    return batch_impl(ctx, nsys, sys, rho_dev, opts, results, true);
 }
 
 }  // extern "C"

@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <map>
 #include <string>
+#include <utility>
 
 #include "cdft_hip.h"
 #include "phi.h"
@@ -17,6 +19,7 @@ struct cdft_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     double last_kernel_ms = 0.0;
+    std::map<int, std::pair<double *, float *>> twiddles;   // FFT length M -> W_M^t tables
 };
 
 // Per-system state.  Host copies of the descriptor plus device buffers.
@@ -35,7 +38,6 @@ struct cdft_sys {
 
     // device buffers (FP64)
     double *h = nullptr;       // real-space kernels [(c*F+f)*N + j], f >= 1
-    float *h32 = nullptr;      // FP32 copy for the mixed-precision batched path
     double *vext = nullptr;    // [c*N + i]
     double *cpd = nullptr;     // device copy of cp [nc]
     double *rho = nullptr;     // current profile
@@ -47,6 +49,10 @@ struct cdft_sys {
     // Anderson storage (anderson.jl:665-703)
     double *sol = nullptr, *gx = nullptr, *df = nullptr, *dg = nullptr, *res = nullptr;
     double *Q = nullptr, *Qalt = nullptr, *DG = nullptr;   // m columns each
+    // batched path: kernel spectra at the padded FFT length M
+    int M = 0;
+    double *H64 = nullptr;
+    float *H32s = nullptr;
     double *red = nullptr;     // block partials
     double *fin = nullptr;     // finalized reductions [CDFT_NQ]
     int nblk = 0;

@@ -19,6 +19,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #ifndef CDFT_MAXC
 #define CDFT_MAXC 8
 #endif
@@ -119,9 +121,17 @@ __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + N
     TS dn = TS(1) - n3;
     TS log1dn = log(fabs(dn));
     TS inv1dn = TS(1) / dn;
-    TS bkt = log1dn / (twelvepi * n3 * n3) + TS(1) / (twelvepi * n3 * dn * dn);
-    TS dB = (TS(-1) / (n3 * n3 * dn) - TS(2) * log1dn / (n3 * n3 * n3) +
-             (TS(3) * n3 - TS(1)) / (n3 * n3 * dn * dn * dn)) / twelvepi;
+    TS bkt, dB;
+    if (std::is_same<T, float>::value && fabs(n3) < TS(1e-5)) {
+        // FP32 inputs: n3 is at rounding-noise level inside walls and may be 0.
+        // Series of the same bracket, sum_k ((k+2)^2 - 1)/(k+2) n3^k / (12 pi).
+        bkt = (TS(1.5) + n3 * (TS(8.0 / 3.0) + n3 * (TS(3.75) + n3 * TS(4.8)))) / twelvepi;
+        dB = (TS(8.0 / 3.0) + n3 * (TS(7.5) + n3 * TS(14.4))) / twelvepi;
+    } else {
+        bkt = log1dn / (twelvepi * n3 * n3) + TS(1) / (twelvepi * n3 * dn * dn);
+        dB = (TS(-1) / (n3 * n3 * dn) - TS(2) * log1dn / (n3 * n3 * n3) +
+              (TS(3) * n3 - TS(1)) / (n3 * n3 * dn * dn * dn)) / twelvepi;
+    }
     T lg = T(log1dn), iv = T(inv1dn), bk = T(bkt);
     T n2c = n2 * n2 * n2 / T(3) - n2 * nv2sq;
     T phi = -n0 * lg + (n1 * n2 - nv1v2) * iv + n2c * bk;
@@ -159,14 +169,20 @@ __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + N
 #pragma unroll
     for (int i = 0; i < NC; i++) {
         T mi1 = T(P.m[i]) - T(1);
+        if (P.m[i] == 1.0) continue;     // the chain term and its gradient vanish exactly for m = 1
         T di = T(P.dd[i]);
         T rhoi = nl[i][0];
-        T lam = nl[i][IL] / (T(2) * di);
+        T lsl = nl[i][IL];
+        if (std::is_same<T, float>::value) {   // FP32: keep underflowed densities off 0/0
+            rhoi = fmax(rhoi, T(1e-30));
+            lsl = copysign(fmax(fabs(lsl), T(1e-30)), lsl);
+        }
+        T lam = lsl / (T(2) * di);
         T y = inv + T(1.5) * di * z2 * inv * inv + T(0.5) * di * di * z2 * z2 * inv * inv * inv;
         T l = T(log(fabs(TS(y * lam / rhoi))));
         phi += -rhoi * mi1 * l;
         g[i][0] += -mi1 * (l - T(1));
-        g[i][IL] += -rhoi * mi1 / nl[i][IL];
+        g[i][IL] += -rhoi * mi1 / lsl;
         T gy = -rhoi * mi1 / y;
         T dy3 = inv * inv + T(3) * di * z2 * inv * inv * inv + T(1.5) * di * di * z2 * z2 * inv * inv * inv * inv;
         T dy2 = T(1.5) * di * inv * inv + di * di * z2 * inv * inv * inv;
@@ -188,7 +204,8 @@ __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + N
         mt += rb[i] * T(P.m[i]);
         es += T(P.m[i]) * rb[i] * di * di * di;
     }
-    T mb = mt / S;
+    if (std::is_same<T, float>::value) S = fmax(S, T(1e-30));
+    T mb = (std::is_same<T, float>::value && NC == 1) ? T(P.m[0]) : mt / S;
     T eta = es * pi / T(6);
     T A1 = 0, A2 = 0, dA1[NC], dA2[NC];
 #pragma unroll

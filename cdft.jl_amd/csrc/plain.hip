@@ -318,14 +318,14 @@ __global__ __launch_bounds__(256) void k_aa_proj2(int n, const double *__restric
 }
 
 // New Q column, the Anderson point sol = gx - DG theta, and rho = exp(sol).
-__global__ __launch_bounds__(256) void k_aa_newpoint(int n, const double *__restrict__ df, double inv_nrm,
+__global__ __launch_bounds__(256) void k_aa_newpoint(int n, const double *__restrict__ df, double nrm,
                                                      double *__restrict__ qcol, double *__restrict__ sol,
                                                      const double *__restrict__ gx, const double *__restrict__ DG,
                                                      AAMat th, int mk, double *__restrict__ rho)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (qcol) qcol[i] = df[i] * inv_nrm;
+    if (qcol) qcol[i] = df[i] / nrm;   // vnew ./= hv[k+1] (anderson.jl:538)
     double a = 0.0;
     for (int j = 0; j < mk; j++) a += DG[(size_t)th.col[j] * n + i] * th.a[j];
     double x = gx[i] - a;

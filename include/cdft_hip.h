@@ -91,6 +91,8 @@ typedef struct {
     int nhist;             /* residual norms recorded                             */
     double resnorm;        /* final ||beta (G(x) - x)||_2                          */
     double tol;            /* rtol*res0 + atol                                    */
+    int nevals_fp32;       /* map evaluations done in FP32 (CDFT_MIXED)           */
+    int reserved;
 } cdft_result;
 
 /* Context: device selection and one HIP stream.  (No counterpart: the

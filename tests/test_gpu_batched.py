@@ -1,0 +1,113 @@
+"""Batched persistent path (cdft_converge_batch): same algorithm as the plain path,
+checked against it and against the oracle; the mixed-precision mode against FP64.
+Needs a GPU."""
+import sys
+
+import numpy as np
+import pytest
+
+from cdft_amd import DFTOptions, _lib, configs, converge_
+from cdft_amd.methods import adsorption_system
+from helpers import oracle_of
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+def batch(systems, rhos, **kw):
+    ctx = systems[0].device_system.ctx
+    return _lib.converge_batch(ctx, [s.device_system for s in systems], [r.reshape(-1) for r in rhos], **kw)
+
+
+@pytest.fixture(scope="module")
+def pores():
+    return configs.slit_pores(n=6, seed=0)
+
+
+def test_batched_fp64_matches_plain(pores):
+    say("batched FP64 vs plain FP64: 6 slit pores + methane wall, tol=1e-8")
+    items = list(pores) + [configs.methane_wall()]
+    systems = [s for s, _ in items]
+    seeds = [r for _, r in items]
+    rb = [r.copy() for r in seeds]
+    res = batch(systems, rb, tol=1e-8)
+    for s, r0, r_b, rr in zip(systems, seeds, rb, res):
+        rp = r0.copy()
+        converge_(s, rp, tol=1e-8)
+        rp_res = s.last_result
+        say(f"  N={s.N}: batched errcode {rr['errcode']} evals {rr['nevals']} | plain {rp_res['errcode']} "
+            f"evals {rp_res['nevals']} | rel {rel(r_b, rp):.2e}")
+        assert rr["errcode"] == rp_res["errcode"]
+        assert abs(rr["nevals"] - rp_res["nevals"]) <= max(5, 0.1 * rp_res["nevals"])
+        if rr["errcode"] == 0:
+            e_ = rel(r_b, rp)
+            assert e_ < 1e-5, e_
+
+
+def test_batched_fp64_tight_matches_oracle(pores):
+    say("batched FP64 tol=1e-12 vs oracle tol=1e-12 (2 pores)")
+    items = list(pores)[2:4]
+    systems = [s for s, _ in items]
+    rb = [r.copy() for _, r in items]
+    res = batch(systems, rb, tol=1e-12)
+    for (s, r0), r_b, rr in zip(items, rb, res):
+        rho_o, ro, _ = oracle_of(s).converge(r0.reshape(-1), tol=1e-12)
+        assert rr["errcode"] == 0 and ro.errcode == 0
+        e_ = rel(r_b.reshape(-1), rho_o)
+        assert e_ < 1e-10, e_
+
+
+def test_batched_mixed_precision(pores):
+    """FP32 map to its noise floor, then FP64 to tol: profile and adsorption agree with FP64."""
+    say("batched mixed vs batched FP64 (6 pores), tol=1e-8")
+    systems64 = [s for s, _ in pores]
+    seeds = [r for _, r in pores]
+    r64 = [r.copy() for r in seeds]
+    res64 = batch(systems64, r64, tol=1e-8)
+    mixed = configs.slit_pores(n=6, seed=0, options=DFTOptions(precision="mixed"))
+    sm = [s for s, _ in mixed]
+    rm = [r.copy() for _, r in mixed]
+    resm = batch(sm, rm, tol=1e-8)
+    for s, a, b, ra, rb_ in zip(systems64, r64, rm, res64, resm):
+        say(f"  N={s.N}: fp64 evals {ra['nevals']} mixed evals {rb_['nevals']} errcodes {ra['errcode']} {rb_['errcode']}")
+        assert rb_["errcode"] == ra["errcode"]
+        if ra["errcode"] == 0:
+            assert rb_["resnorm"] <= rb_["tol"]
+            e_ = rel(b, a)
+            assert e_ < 1e-5, e_
+            ads_a = adsorption_system(s, a)
+            ads_b = adsorption_system(s, b)
+            assert abs(ads_b[0] - ads_a[0]) <= 1e-6 * abs(ads_a[0])
+
+
+def test_batched_binary_interface():
+    say("batched FP64 binary interface N=256 vs plain")
+    s, r0 = configs.c1c10_interface(ngrid=256, box=10.0)
+    rb = r0.copy()
+    rr = batch([s], [rb], tol=1e-8)[0]
+    rp = r0.copy()
+    converge_(s, rp, tol=1e-8)
+    assert rr["errcode"] == 0 == s.last_result["errcode"]
+    e_ = rel(rb, rp)
+    assert e_ < 1e-5, e_
+
+
+def test_batched_errcode_maxit(pores):
+    say("batched maxit=2 -> errcode 10, equals plain")
+    s, r0 = pores[0]
+    rb = r0.copy()
+    rr = batch([s], [rb], tol=1e-8, maxit=2)[0]
+    rp = r0.copy()
+    converge_(s, rp, tol=1e-8, maxit=2)
+    assert rr["errcode"] == 10 == s.last_result["errcode"]
+    assert rr["nevals"] == s.last_result["nevals"]
+    e_ = rel(rb, rp)
+    assert e_ < 1e-12, e_
