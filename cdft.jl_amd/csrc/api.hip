@@ -501,10 +501,7 @@ static int fft_length(int N)
     return M;
 }
 
-static size_t batched_lds_bytes(const cdft_sys *s)
-{
-    return (size_t)2 * s->M * 16 + (size_t)s->F * s->N * s->nc * 8 + 64 * 8;
-}
+static size_t batched_lds_bytes(const cdft_sys *s) { return batched_lds_layout(s->N, s->M, s->nc); }
 
 static int batched_prepare(cdft_ctx *ctx, cdft_sys *s)
 {
@@ -556,13 +553,15 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
     int maxlds = 0;
     HIPCHECK_CTX(ctx, hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
     size_t lds = 0;
+    int maxM = 0;
     for (int i = 0; i < nsys; i++) {
         int rc = batched_prepare(ctx, sys[i]);
         if (rc != CDFT_OK) return rc;
         lds = std::max(lds, batched_lds_bytes(sys[i]));
+        maxM = std::max(maxM, sys[i]->M);
     }
     size_t lds_limit = std::max<size_t>((size_t)maxlds, 160 * 1024);
-    if (lds + sizeof(AState) > lds_limit)
+    if (lds + sizeof(AState) + sizeof(PhiConst) > lds_limit)
         return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched path: system too large for one workgroup's LDS");
     // largest systems first: the tail of the launch is then the short ones
     std::vector<int> order(nsys);
@@ -627,12 +626,16 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         return CDFT_OK;
     };
     int rc;
+    const int mb = maxM <= 1024 ? 1 : (maxM <= 2048 ? 2 : 4);
+#define CDFT_LAUNCH_NC(NCV)                                  \
+    (mb == 1 ? launch(k_batched<NCV, 1>) : mb == 2 ? launch(k_batched<NCV, 2>) : launch(k_batched<NCV, 4>))
     switch (nc) {
-    case 1: rc = launch(k_batched<1>); break;
-    case 2: rc = launch(k_batched<2>); break;
-    case 3: rc = launch(k_batched<3>); break;
-    default: rc = launch(k_batched<4>); break;
+    case 1: rc = CDFT_LAUNCH_NC(1); break;
+    case 2: rc = CDFT_LAUNCH_NC(2); break;
+    case 3: rc = CDFT_LAUNCH_NC(3); break;
+    default: rc = CDFT_LAUNCH_NC(4); break;
     }
+#undef CDFT_LAUNCH_NC
     if (rc != CDFT_OK) return rc;
     std::vector<BResult> hr(nsys);
     HIPCHECK_CTX(ctx, hipMemcpyAsync(hr.data(), dr, sizeof(BResult) * nsys, hipMemcpyDeviceToHost, st));

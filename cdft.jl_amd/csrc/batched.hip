@@ -61,32 +61,22 @@ constexpr int BT = 256;        // threads per workgroup
 
 // ---------------------------------------------------------------- helpers
 template <typename T>
-__device__ __forceinline__ c2<T> cmul(c2<T> a, c2<T> b)
+__device__ __attribute__((always_inline)) inline c2<T> cmul(c2<T> a, c2<T> b)
 {
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
-template <typename T>
-__device__ __forceinline__ const T *tw_of(const BSys &S);
-template <>
-__device__ __forceinline__ const double *tw_of<double>(const BSys &S) { return S.tw64; }
-template <>
-__device__ __forceinline__ const float *tw_of<float>(const BSys &S) { return S.tw32; }
-template <typename T>
-__device__ __forceinline__ const T *H_of(const BSys &S);
-template <>
-__device__ __forceinline__ const double *H_of<double>(const BSys &S) { return S.H64; }
-template <>
-__device__ __forceinline__ const float *H_of<float>(const BSys &S) { return S.H32; }
 
-// In-place Stockham FFT of length M (power of two, 4 <= M <= 4096) in LDS:
+#define CDFT_INLINE __device__ __attribute__((always_inline)) inline
+
+// In-place Stockham FFT of length M (power of two, 4 <= M <= 1024 * MAXB) in LDS:
 // radix-4 passes and a final radix-2 pass when log2 M is odd.  Each pass loads
 // all its butterfly inputs into registers, synchronises, then stores, so one
-// buffer suffices.  Unnormalised; sign -1 forward, +1 inverse.
-template <typename T>
-__device__ void fft_lds(c2<T> *a, int M, int log2M, const T *tw, bool inverse)
+// buffer suffices.  Twiddles W_M^t come from the LDS table tw.  Unnormalised;
+// sign -1 forward, +1 inverse.
+template <typename T, int MAXB>
+CDFT_INLINE void fft_lds(c2<T> *a, const c2<double> *tw, int M, int log2M, bool inverse)
 {
-    constexpr int MAXB = 4;                       // butterflies per thread (M <= 4096)
     const int tid = threadIdx.x;
     int Ns = 1;
     const int nr4 = log2M >> 1;
@@ -94,29 +84,27 @@ __device__ void fft_lds(c2<T> *a, int M, int log2M, const T *tw, bool inverse)
     for (int pass = 0; pass < nr4; pass++) {
         c2<T> v[MAXB][4];
         const int nb = M >> 2;
+        const int step = M / (Ns * 4);
 #pragma unroll
         for (int b = 0; b < MAXB; b++) {
             int j = tid + b * BT;
             if (j < nb) {
                 int k = j & (Ns - 1);
-                int step = M / (Ns * 4);
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     c2<T> x = a[j + r * nb];
                     if (r) {
-                        int t = (r * k * step) & (M - 1);
-                        c2<T> w = {tw[2 * t], inverse ? -tw[2 * t + 1] : tw[2 * t + 1]};
+                        c2<double> wd = tw[(r * k * step) & (M - 1)];
+                        c2<T> w = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
                         x = cmul(x, w);
                     }
                     v[b][r] = x;
                 }
-                // y_q = sum_r v_r (sg i)^{r q}
                 c2<T> s02 = {v[b][0].x + v[b][2].x, v[b][0].y + v[b][2].y};
                 c2<T> d02 = {v[b][0].x - v[b][2].x, v[b][0].y - v[b][2].y};
                 c2<T> s13 = {v[b][1].x + v[b][3].x, v[b][1].y + v[b][3].y};
                 c2<T> d13 = {v[b][1].x - v[b][3].x, v[b][1].y - v[b][3].y};
-                // (sg i) * d13 = (-sg d13.y, sg d13.x)
-                c2<T> id13 = {-sg * d13.y, sg * d13.x};
+                c2<T> id13 = {-sg * d13.y, sg * d13.x};   // (sg i) * d13
                 v[b][0] = {s02.x + s13.x, s02.y + s13.y};
                 v[b][1] = {d02.x + id13.x, d02.y + id13.y};
                 v[b][2] = {s02.x - s13.x, s02.y - s13.y};
@@ -145,8 +133,8 @@ __device__ void fft_lds(c2<T> *a, int M, int log2M, const T *tw, bool inverse)
             int j = tid + b * BT;
             if (j < nb) {
                 int k = j & (Ns - 1);
-                int t = (k * (M / (Ns * 2))) & (M - 1);
-                c2<T> w = {tw[2 * t], inverse ? -tw[2 * t + 1] : tw[2 * t + 1]};
+                c2<double> wd = tw[(k * (M / (Ns * 2))) & (M - 1)];
+                c2<T> w = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
                 c2<T> x0 = a[j], x1 = cmul(a[j + nb], w);
                 v[b][0] = {x0.x + x1.x, x0.y + x1.y};
                 v[b][1] = {x0.x - x1.x, x0.y - x1.y};
@@ -170,12 +158,13 @@ __device__ void fft_lds(c2<T> *a, int M, int log2M, const T *tw, bool inverse)
 // Sum of NQ values over the workgroup; every thread receives the same totals
 // (fixed summation order, so the result is reproducible and uniform).
 template <int NQ>
-__device__ __forceinline__ void wg_sum(double (&v)[NQ], double *red)
+CDFT_INLINE void wg_sum(double (&v)[NQ], double *red)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         double x = v[q];
+#pragma unroll
         for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
         v[q] = x;
     }
@@ -188,60 +177,82 @@ __device__ __forceinline__ void wg_sum(double (&v)[NQ], double *red)
     __syncthreads();
 }
 
-// LDS layout (bytes): work[M] c2<T> | acc[M] c2<T> | fld[F*N*nc] T | red[64] double | misc
+// One system's data, read once from its descriptor into registers.
+template <int NC>
+struct SysRegs {
+    int N, M, lg, n;
+    double NA;
+    double cp[NC];
+    const double *__restrict__ vext;
+    const double *__restrict__ H64;
+    const float *__restrict__ H32;
+    double *__restrict__ rho;
+    double *__restrict__ sol;
+    double *__restrict__ gx;
+    double *__restrict__ df;
+    double *__restrict__ dg;
+    double *__restrict__ res;
+    double *__restrict__ Q;
+    double *__restrict__ DG;
+};
+
 template <typename T>
-struct Lds {
-    c2<T> *A, *P;
-    T *fld;
-    double *red;
+struct HSel;
+template <>
+struct HSel<double> {
+    static CDFT_INLINE const double *get(const double *h64, const float *) { return h64; }
+};
+template <>
+struct HSel<float> {
+    static CDFT_INLINE const float *get(const double *, const float *h32) { return h32; }
 };
 
 // One evaluation of the fixed-point map plus the Anderson book-keeping of mode
-// (1: first/Picard step, 2: aa_point!).  Returns |res|^2 (and |df|^2, df.res).
-template <typename T, int NC>
-__device__ void evalG(const BSys &S, const Lds<T> &L, int mode, double b, double (&out)[3])
+// (1: first/Picard step, 2: aa_point!).  out = |res|^2, |df|^2, df.res.
+// LDS: tw (M, FP64), work A and accumulator P (M complex T each, at FP64
+// stride), fields fld (F*N*NC of T at FP64 stride), red (64 doubles).
+template <typename T, int NC, int MAXB>
+CDFT_INLINE void evalG(const SysRegs<NC> &S, const PhiConst &P, const c2<double> *tw, c2<T> *A, c2<T> *Pc, T *fld,
+                       double *red, int mode, double b, double (&out)[3])
 {
     constexpr int F = 7;
     const int tid = threadIdx.x;
-    const int N = S.N, M = S.M, lg = S.log2M;
+    const int N = S.N, M = S.M, lg = S.lg;
     const bool fold = (M != N);
-    const T *tw = tw_of<T>(S);
-    const T *H = H_of<T>(S);
-    const T NA = T(S.NA);
+    const T *H = HSel<T>::get(S.H64, S.H32);
     const T invM = T(1) / T(M);
 
     for (int c = 0; c < NC; c++) {
-        // rho = exp(sol); slot 0 of the fields and the padded FFT input
+        // rho = exp(sol): slot 0 of the fields and the zero-padded FFT input
         for (int k = tid; k < M; k += BT) {
             if (k < N) {
                 double r = exp(S.sol[c * N + k]);
-                L.fld[(c * F) * N + k] = T(S.NA * r);
-                L.A[k] = {T(r), T(0)};
+                fld[(c * F) * N + k] = T(S.NA * r);
+                Pc[k] = {T(r), T(0)};
             } else {
-                L.A[k] = {T(0), T(0)};
+                Pc[k] = {T(0), T(0)};
             }
         }
         __syncthreads();
-        fft_lds<T>(L.A, M, lg, tw, false);
-        for (int k = tid; k < M; k += BT) L.P[k] = L.A[k];
-        __syncthreads();
+        fft_lds<T, MAXB>(Pc, tw, M, lg, false);
         for (int q = 0; q < 3; q++) {
             const int f1 = 1 + 2 * q, f2 = 2 + 2 * q;
             const T *H1 = H + (size_t)(c * (F - 1) + f1 - 1) * M * 2;
             const T *H2 = H + (size_t)(c * (F - 1) + f2 - 1) * M * 2;
             for (int k = tid; k < M; k += BT) {
-                c2<T> p = L.P[k];
+                c2<T> p = Pc[k];
                 c2<T> a1 = cmul(p, c2<T>{H1[2 * k], H1[2 * k + 1]});
                 c2<T> a2 = cmul(p, c2<T>{H2[2 * k], H2[2 * k + 1]});
-                L.A[k] = {a1.x - a2.y, a1.y + a2.x};
+                A[k] = {a1.x - a2.y, a1.y + a2.x};          // P H1 + i P H2
             }
             __syncthreads();
-            fft_lds<T>(L.A, M, lg, tw, true);
+            fft_lds<T, MAXB>(A, tw, M, lg, true);
+            const T sc = T(S.NA) * invM;
             for (int i = tid; i < N; i += BT) {
-                c2<T> y = L.A[i];
-                if (fold && i + N < M) { y.x += L.A[i + N].x; y.y += L.A[i + N].y; }
-                L.fld[(c * F + f1) * N + i] = NA * (y.x * invM);
-                L.fld[(c * F + f2) * N + i] = NA * (y.y * invM);
+                c2<T> y = A[i];
+                if (fold && i + N < M) { y.x += A[i + N].x; y.y += A[i + N].y; }
+                fld[(c * F + f1) * N + i] = sc * y.x;
+                fld[(c * F + f2) * N + i] = sc * y.y;
             }
             __syncthreads();
         }
@@ -252,47 +263,45 @@ __device__ void evalG(const BSys &S, const Lds<T> &L, int mode, double b, double
 #pragma unroll
         for (int c = 0; c < NC; c++)
 #pragma unroll
-            for (int f = 0; f < F; f++) nl[c][f] = L.fld[(c * F + f) * N + i];
-        phi_grad<T, double, NC, 1>(S.pc, nl, g);
+            for (int f = 0; f < F; f++) nl[c][f] = fld[(c * F + f) * N + i];
+        phi_grad<T, double, NC, 1>(P, nl, g);
 #pragma unroll
         for (int c = 0; c < NC; c++)
 #pragma unroll
-            for (int f = 0; f < F; f++) L.fld[(c * F + f) * N + i] = g[c][f];
+            for (int f = 0; f < F; f++) fld[(c * F + f) * N + i] = g[c][f];
     }
     __syncthreads();
-    double v[3] = {0.0, 0.0, 0.0};
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
     for (int c = 0; c < NC; c++) {
-        for (int k = tid; k < M; k += BT) L.P[k] = {T(0), T(0)};
+        for (int k = tid; k < M; k += BT) Pc[k] = {T(0), T(0)};
         for (int q = 0; q < 3; q++) {
             const int f1 = 1 + 2 * q, f2 = 2 + 2 * q;
             const T s1 = (f1 == 3) ? T(-1) : T(1), s2 = (f2 == 3) ? T(-1) : T(1);   // vector slot subtracts
             const T *H1 = H + (size_t)(c * (F - 1) + f1 - 1) * M * 2;
             const T *H2 = H + (size_t)(c * (F - 1) + f2 - 1) * M * 2;
             for (int k = tid; k < M; k += BT)
-                L.A[k] = (k < N) ? c2<T>{L.fld[(c * F + f1) * N + k], L.fld[(c * F + f2) * N + k]} : c2<T>{T(0), T(0)};
+                A[k] = (k < N) ? c2<T>{fld[(c * F + f1) * N + k], fld[(c * F + f2) * N + k]} : c2<T>{T(0), T(0)};
             __syncthreads();
-            fft_lds<T>(L.A, M, lg, tw, false);
+            fft_lds<T, MAXB>(A, tw, M, lg, false);
             for (int k = tid; k < M; k += BT) {
                 int km = (M - k) & (M - 1);
-                c2<T> a = L.A[k], bb = L.A[km];
-                // U1 = (a + conj b)/2, U2 = (a - conj b)/(2i)
-                c2<T> u1 = {T(0.5) * (a.x + bb.x), T(0.5) * (a.y - bb.y)};
-                c2<T> dd = {a.x - bb.x, a.y + bb.y};
-                c2<T> u2 = {T(0.5) * dd.y, T(-0.5) * dd.x};
+                c2<T> a = A[k], bb = A[km];
+                c2<T> u1 = {T(0.5) * (a.x + bb.x), T(0.5) * (a.y - bb.y)};     // (a + conj b)/2
+                c2<T> u2 = {T(0.5) * (a.y + bb.y), T(-0.5) * (a.x - bb.x)};    // (a - conj b)/(2i)
                 c2<T> t1 = cmul(u1, c2<T>{H1[2 * k], H1[2 * k + 1]});
                 c2<T> t2 = cmul(u2, c2<T>{H2[2 * k], H2[2 * k + 1]});
-                c2<T> acc = L.P[k];
+                c2<T> acc = Pc[k];
                 acc.x += s1 * t1.x + s2 * t2.x;
                 acc.y += s1 * t1.y + s2 * t2.y;
-                L.P[k] = acc;
+                Pc[k] = acc;
             }
             __syncthreads();
         }
-        fft_lds<T>(L.P, M, lg, tw, true);
+        fft_lds<T, MAXB>(Pc, tw, M, lg, true);
         for (int i = tid; i < N; i += BT) {
-            T y = L.P[i].x;
-            if (fold && i + N < M) y += L.P[i + N].x;
-            double s = double(L.fld[(c * F) * N + i]) + double(y * invM);
+            T y = Pc[i].x;
+            if (fold && i + N < M) y += Pc[i + N].x;
+            double s = double(fld[(c * F) * N + i]) + double(y * invM);
             const int t = c * N + i;
             if (S.vext) s += S.vext[t];
             double lgv = S.cp[c] - s;
@@ -303,7 +312,7 @@ __device__ void evalG(const BSys &S, const Lds<T> &L, int mode, double b, double
                 S.gx[t] = gnew;
                 double r = gnew - x;
                 S.res[t] = r;
-                v[0] += r * r;
+                v0 += r * r;
             } else {
                 double gold = S.gx[t], rold = S.res[t];
                 S.dg[t] = -gold + gnew;
@@ -312,21 +321,22 @@ __device__ void evalG(const BSys &S, const Lds<T> &L, int mode, double b, double
                 S.res[t] = r;
                 double d = -rold + r;
                 S.df[t] = d;
-                v[0] += r * r;
-                v[1] += d * d;
-                v[2] += d * r;
+                v0 += r * r;
+                v1 += d * d;
+                v2 += d * r;
             }
         }
         __syncthreads();
     }
-    wg_sum<3>(v, L.red);
+    double v[3] = {v0, v1, v2};
+    wg_sum<3>(v, red);
     out[0] = v[0];
     out[1] = v[1];
     out[2] = v[2];
 }
 
 // Householder downdate of the Anderson R (same arithmetic as the host version).
-__device__ void dev_householder_downdate(double *R, int m, double *Qx)
+CDFT_INLINE void dev_householder_downdate(double *R, int m, double *Qx)
 {
     int nr = m, ncol = m - 1;
     double A[CDFT_MAXM * CDFT_MAXM], tau[CDFT_MAXM], v[CDFT_MAXM];
@@ -377,35 +387,58 @@ struct AState {
     int ring[CDFT_MAXM];
 };
 
-template <typename T, int NC>
-__device__ double evalG_dispatch(const BSys &S, const Lds<T> &L, const Lds<float> &L32, bool fp32, int mode, double b,
-                                 double (&o)[3])
+// Dynamic LDS bytes of one system (FP64 layout; the FP32 map uses the same slots).
+__host__ __device__ inline size_t batched_lds_layout(int N, int M, int nc)
 {
-    if (fp32) evalG<float, NC>(S, L32, mode, b, o);
-    else evalG<T, NC>(S, L, mode, b, o);
-    return o[0];
+    return (size_t)3 * M * 16 + (size_t)7 * N * nc * 8 + 64 * 8;
 }
 
-template <int NC>
+template <int NC, int MAXB>
 __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems, BOpts o, BResult *results)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ AState st;
-    const BSys &S = systems[blockIdx.x];
-    const int tid = threadIdx.x;
-    const int N = S.N, M = S.M, n = N * NC;
+    __shared__ PhiConst pcs;
     constexpr int F = 7;
-    // LDS views for both arithmetic types over the same bytes
-    Lds<double> L;
-    L.A = reinterpret_cast<c2<double> *>(smem);
-    L.P = L.A + M;
-    L.fld = reinterpret_cast<double *>(L.P + M);
-    L.red = L.fld + (size_t)F * N * NC;
-    Lds<float> L32;
-    L32.A = reinterpret_cast<c2<float> *>(smem);
-    L32.P = L32.A + M;
-    L32.fld = reinterpret_cast<float *>(L32.P + M);
-    L32.red = L.red;                          // reduction scratch stays FP64, after the FP64 layout
+    const int tid = threadIdx.x;
+
+    // ---- descriptor -> registers / LDS, once
+    const BSys *D = systems + blockIdx.x;
+    SysRegs<NC> S;
+    S.N = D->N;
+    S.M = D->M;
+    S.lg = D->log2M;
+    S.n = S.N * NC;
+    S.NA = D->NA;
+#pragma unroll
+    for (int c = 0; c < NC; c++) S.cp[c] = D->cp[c];
+    S.vext = D->vext;
+    S.H64 = D->H64;
+    S.H32 = D->H32;
+    S.rho = D->rho;
+    S.sol = D->sol;
+    S.gx = D->gx;
+    S.df = D->df;
+    S.dg = D->dg;
+    S.res = D->res;
+    S.Q = D->Q;
+    S.DG = D->DG;
+    const double *__restrict__ twg = D->tw64;
+    {
+        const double *src = reinterpret_cast<const double *>(&D->pc);
+        double *dst = reinterpret_cast<double *>(&pcs);
+        for (int i = tid; i < (int)(sizeof(PhiConst) / sizeof(double)); i += BT) dst[i] = src[i];
+    }
+    const int N = S.N, M = S.M, n = S.n;
+    c2<double> *tw = reinterpret_cast<c2<double> *>(smem);
+    c2<double> *A64 = tw + M;
+    c2<double> *P64 = A64 + M;
+    double *fld64 = reinterpret_cast<double *>(P64 + M);
+    double *red = fld64 + (size_t)F * N * NC;
+    c2<float> *A32 = reinterpret_cast<c2<float> *>(A64);
+    c2<float> *P32 = reinterpret_cast<c2<float> *>(P64);
+    float *fld32 = reinterpret_cast<float *>(fld64);
+    for (int t = tid; t < M; t += BT) tw[t] = {twg[2 * t], twg[2 * t + 1]};
 
     const int m = o.m;
     const double beta = o.beta;
@@ -416,191 +449,239 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
     int nevals = 0, nevals32 = 0;
     double fo[3];
 
-    // sol = log(rho)
-    for (int i = tid; i < n; i += BT) S.sol[i] = log(S.rho[i]);
+    for (int i = tid; i < n; i += BT) S.sol[i] = log(S.rho[i]);   // x0 = log(rho)
     __syncthreads();
 
-#define EVAL(mode, b) do { evalG_dispatch<double, NC>(S, L, L32, fp32, (mode), (b), fo); nevals++; if (fp32) nevals32++; } while (0)
-#define SET_SOL_FROM_GX() do { for (int i_ = tid; i_ < n; i_ += BT) S.sol[i_] = S.gx[i_]; __syncthreads(); } while (0)
-
-    int k = 0;
-    EVAL(1, beta);
-    double resnorm = sqrt(fo[0]);
-    const double up_bd = 1.e4 * resnorm;
-    const double tol = rtol * resnorm + atol;
-    const double picard_tol = picard_rtol * resnorm + picard_atol;
-    bool toosoon = (resnorm <= tol);
-
-    if (!toosoon && picard_maxit > 0) {
-        SET_SOL_FROM_GX();
-        for (int it = 0; it < picard_maxit - 1; it++) {
-            EVAL(1, picard_beta);
-            resnorm = sqrt(fo[0]);
-            if (resnorm <= picard_tol || resnorm >= up_bd) break;
-            SET_SOL_FROM_GX();
-        }
-        toosoon = (resnorm <= tol);
-    }
-
-    // Anderson phase; a precision switch restarts it (ka counts Anderson steps
-    // since the last (re)start, k counts all of them for maxit).
-    double dfn2 = 0.0, dfres = 0.0;
-    int ka = 0;
-    double best = resnorm;
-    int since_best = 0;
-    auto restart = [&]() {
-        if (tid == 0) {
-            for (int j = 0; j < CDFT_MAXM * CDFT_MAXM; j++) st.R[j] = 0.0;
-            for (int j = 0; j < CDFT_MAXM; j++) st.ring[j] = j;
-        }
-        __syncthreads();
-        SET_SOL_FROM_GX();
-        k += 1;
-        ka = 1;
-        EVAL(2, beta);
-        resnorm = sqrt(fo[0]);
-        dfn2 = fo[1];
-        dfres = fo[2];
-        best = resnorm;
-        since_best = 0;
-    };
-    if (!toosoon) restart();
-
-    while (k < maxit + picard_maxit && !toosoon && resnorm < up_bd) {
-        if (fp32) {
-            if (resnorm < 0.5 * best) { best = resnorm; since_best = 0; }
-            else since_best++;
-            if (resnorm <= o.switch_factor * tol || since_best >= o.stall_window) {
-                // FP32 noise floor reached: continue with the FP64 map
-                fp32 = false;
-                EVAL(1, beta);
-                resnorm = sqrt(fo[0]);
-                if (resnorm <= tol) break;
-                restart();
-                continue;
-            }
-        } else if (!(resnorm > tol)) {
-            break;
-        }
-        if (m == 0) {
-            SET_SOL_FROM_GX();
-        } else {
-            int kk = ka + 1;
-            int kq = ka - 1;
-            int slot, qcol;
-            int down = 0;
+    // aasol (anderson.jl:301-443) as a phase machine with a single map-evaluation
+    // site per precision.  ka is aasol's Anderson counter since the last (re)start;
+    // k counts every Anderson step for the maxit bound.
+    enum { PH_FIRST, PH_PICARD, PH_RESTART, PH_STEP, PH_SWITCH, PH_DONE };
+    int phase = PH_FIRST;
+    int k = 0, ka = 0, pit = 0, since_best = 0;
+    double resnorm = 0.0, up_bd = 0.0, tol = 0.0, picard_tol = 0.0, best = 0.0, dfn2 = 0.0, dfres = 0.0;
+    bool toosoon = false;
+    while (phase != PH_DONE) {
+        // ---------------- what to evaluate next
+        int mode = 1;
+        double bb = beta;
+        if (phase == PH_PICARD) {
+            bb = picard_beta;
+        } else if (phase == PH_RESTART) {
             if (tid == 0) {
-                if (m > 1 && kk > m + 1) {
+                for (int j = 0; j < CDFT_MAXM * CDFT_MAXM; j++) st.R[j] = 0.0;
+                for (int j = 0; j < CDFT_MAXM; j++) st.ring[j] = j;
+            }
+            for (int i = tid; i < n; i += BT) S.sol[i] = S.gx[i];   // copy!(sol, gx)
+            __syncthreads();
+            k += 1;
+            ka = 1;
+            mode = 2;
+        } else if (phase == PH_SWITCH) {
+            fp32 = false;                                           // FP32 noise floor: FP64 map from here
+        } else if (phase == PH_STEP) {
+            if (m == 0) {
+                for (int i = tid; i < n; i += BT) S.sol[i] = S.gx[i];
+                __syncthreads();
+            } else {
+                const int kk = ka + 1;
+                const int kq = ka - 1;
+                if (tid == 0 && m > 1 && kk > m + 1) {               // BuildDG! shift, as a ring
                     int first = st.ring[0];
                     for (int j = 0; j < m - 1; j++) st.ring[j] = st.ring[j + 1];
                     st.ring[m - 1] = first;
                 }
-            }
-            __syncthreads();
-            slot = (m == 1) ? st.ring[0] : (kk > m + 1 ? st.ring[m - 1] : st.ring[kk - 2]);
-            // BuildDG!: copy dg into its ring slot
-            for (int i = tid; i < n; i += BT) S.DG[(size_t)slot * n + i] = S.dg[i];
-            double nrm;
-            if (kq == 0) {
-                nrm = sqrt(dfn2);
-                if (tid == 0) {
-                    st.R[0] = nrm;
-                    st.tres[0] = dfres / nrm;
-                }
-                qcol = 0;
-            } else {
-                down = (kq > m - 1);
-                if (down && tid == 0) dev_householder_downdate(st.R, m, st.Qx);
                 __syncthreads();
-                const int kqq = kq < m - 1 ? kq : m - 1;
-                // pass 1: optional downdate of Q (pointwise, in place), h1 = Q^T df, t = Q^T res
-                double v1[2 * CDFT_MAXM];
-                for (int j = 0; j < 2 * CDFT_MAXM; j++) v1[j] = 0.0;
-                for (int i = tid; i < n; i += BT) {
-                    double qv[CDFT_MAXM];
-                    if (down) {
-                        double qo[CDFT_MAXM];
-                        for (int l = 0; l < m; l++) qo[l] = S.Q[(size_t)l * n + i];
-                        for (int j = 0; j < m - 1; j++) {
-                            double a = 0.0;
-                            for (int l = 0; l < m; l++) a += qo[l] * st.Qx[j * m + l];
-                            qv[j] = a;
-                            S.Q[(size_t)j * n + i] = a;
+                const int slot = (m == 1) ? st.ring[0] : (kk > m + 1 ? st.ring[m - 1] : st.ring[kk - 2]);
+                for (int i = tid; i < n; i += BT) S.DG[(size_t)slot * n + i] = S.dg[i];
+                double nrm;
+                int qcol;
+                if (kq == 0) {                                      // aa_qr_update! first column
+                    nrm = sqrt(dfn2);
+                    if (tid == 0) {
+                        st.R[0] = nrm;
+                        st.tres[0] = dfres / nrm;
+                    }
+                    qcol = 0;
+                } else {
+                    const int down = (kq > m - 1);
+                    if (down && tid == 0) dev_householder_downdate(st.R, m, st.Qx);
+                    __syncthreads();
+                    const int kqq = kq < m - 1 ? kq : m - 1;
+                    // pass 1: optional downdate of Q (pointwise, in place); h1 = Q^T df, t = Q^T res
+                    double v1[2 * CDFT_MAXM];
+#pragma unroll
+                    for (int j = 0; j < 2 * CDFT_MAXM; j++) v1[j] = 0.0;
+                    for (int i = tid; i < n; i += BT) {
+                        double qv[CDFT_MAXM];
+                        if (down) {
+                            double qo[CDFT_MAXM];
+#pragma unroll
+                            for (int l = 0; l < CDFT_MAXM; l++) qo[l] = (l < m) ? S.Q[(size_t)l * n + i] : 0.0;
+#pragma unroll
+                            for (int j = 0; j < CDFT_MAXM - 1; j++) {
+                                double a = 0.0;
+#pragma unroll
+                                for (int l = 0; l < CDFT_MAXM; l++)
+                                    if (l < m) a += qo[l] * st.Qx[j * m + l];
+                                qv[j] = a;
+                                if (j < m - 1) S.Q[(size_t)j * n + i] = a;
+                            }
+                            S.Q[(size_t)(m - 1) * n + i] = 0.0;
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < CDFT_MAXM - 1; j++) qv[j] = (j < kqq) ? S.Q[(size_t)j * n + i] : 0.0;
                         }
-                        S.Q[(size_t)(m - 1) * n + i] = 0.0;
-                    } else {
-                        for (int j = 0; j < kqq; j++) qv[j] = S.Q[(size_t)j * n + i];
+                        const double dv = S.df[i], rv = S.res[i];
+#pragma unroll
+                        for (int j = 0; j < CDFT_MAXM - 1; j++)
+                            if (j < kqq) {
+                                v1[j] += qv[j] * dv;
+                                v1[CDFT_MAXM + j] += qv[j] * rv;
+                            }
                     }
-                    double dv = S.df[i], rv = S.res[i];
-                    for (int j = 0; j < kqq; j++) {
-                        v1[j] += qv[j] * dv;
-                        v1[CDFT_MAXM + j] += qv[j] * rv;
+                    wg_sum<2 * CDFT_MAXM>(v1, red);
+                    // pass 2: df -= Q h1; h2 = Q^T df
+                    double v2[CDFT_MAXM];
+#pragma unroll
+                    for (int j = 0; j < CDFT_MAXM; j++) v2[j] = 0.0;
+                    for (int i = tid; i < n; i += BT) {
+                        double qv[CDFT_MAXM], a = 0.0;
+#pragma unroll
+                        for (int j = 0; j < CDFT_MAXM - 1; j++)
+                            if (j < kqq) {
+                                qv[j] = S.Q[(size_t)j * n + i];
+                                a += qv[j] * v1[j];
+                            }
+                        const double d = S.df[i] - a;
+                        S.df[i] = d;
+#pragma unroll
+                        for (int j = 0; j < CDFT_MAXM - 1; j++)
+                            if (j < kqq) v2[j] += qv[j] * d;
                     }
-                }
-                wg_sum<2 * CDFT_MAXM>(v1, L.red);
-                // pass 2: df -= Q h1; h2 = Q^T df
-                double v2[CDFT_MAXM];
-                for (int j = 0; j < CDFT_MAXM; j++) v2[j] = 0.0;
-                for (int i = tid; i < n; i += BT) {
-                    double a = 0.0, qv[CDFT_MAXM];
-                    for (int j = 0; j < kqq; j++) {
-                        qv[j] = S.Q[(size_t)j * n + i];
-                        a += qv[j] * v1[j];
+                    wg_sum<CDFT_MAXM>(v2, red);
+                    // pass 3: df -= Q h2; |df|^2, df.res
+                    double v3[2] = {0.0, 0.0};
+                    for (int i = tid; i < n; i += BT) {
+                        double a = 0.0;
+#pragma unroll
+                        for (int j = 0; j < CDFT_MAXM - 1; j++)
+                            if (j < kqq) a += S.Q[(size_t)j * n + i] * v2[j];
+                        const double d = S.df[i] - a;
+                        S.df[i] = d;
+                        v3[0] += d * d;
+                        v3[1] += d * S.res[i];
                     }
-                    double d = S.df[i] - a;
-                    S.df[i] = d;
-                    for (int j = 0; j < kqq; j++) v2[j] += qv[j] * d;
+                    wg_sum<2>(v3, red);
+                    nrm = sqrt(v3[0]);
+                    if (tid == 0) {
+                        for (int j = 0; j < kqq; j++) st.R[kqq * m + j] = v1[j] + v2[j];
+                        st.R[kqq * m + kqq] = nrm;
+                        for (int j = 0; j < kqq; j++) st.tres[j] = v1[CDFT_MAXM + j];
+                        st.tres[kqq] = v3[1] / nrm;
+                    }
+                    qcol = kqq;
                 }
-                wg_sum<CDFT_MAXM>(v2, L.red);
-                // pass 3: df -= Q h2; |df|^2, df.res
-                double v3[2] = {0.0, 0.0};
-                for (int i = tid; i < n; i += BT) {
-                    double a = 0.0;
-                    for (int j = 0; j < kqq; j++) a += S.Q[(size_t)j * n + i] * v2[j];
-                    double d = S.df[i] - a;
-                    S.df[i] = d;
-                    v3[0] += d * d;
-                    v3[1] += d * S.res[i];
-                }
-                wg_sum<2>(v3, L.red);
-                nrm = sqrt(v3[0]);
+                const int mk = m < ka ? m : ka;
                 if (tid == 0) {
-                    for (int j = 0; j < kqq; j++) st.R[kqq * m + j] = v1[j] + v2[j];
-                    st.R[kqq * m + kqq] = nrm;
-                    for (int j = 0; j < kqq; j++) st.tres[j] = v1[CDFT_MAXM + j];
-                    st.tres[kqq] = v3[1] / nrm;
+                    for (int j = mk - 1; j >= 0; j--) {             // theta = RA \ tres
+                        double acc = st.tres[j];
+                        for (int l = j + 1; l < mk; l++) acc -= st.R[l * m + j] * st.theta[l];
+                        st.theta[j] = acc / st.R[j * m + j];
+                    }
                 }
-                qcol = kqq;
-            }
-            const int mk = m < ka ? m : ka;
-            if (tid == 0) {
-                for (int j = mk - 1; j >= 0; j--) {            // theta = RA \ tres
-                    double acc = st.tres[j];
-                    for (int l = j + 1; l < mk; l++) acc -= st.R[l * m + j] * st.theta[l];
-                    st.theta[j] = acc / st.R[j * m + j];
+                __syncthreads();
+                double th[CDFT_MAXM];
+                int rg[CDFT_MAXM];
+#pragma unroll
+                for (int j = 0; j < CDFT_MAXM; j++) {
+                    th[j] = (j < mk) ? st.theta[j] : 0.0;
+                    rg[j] = (j < mk) ? st.ring[j] : 0;
                 }
+                // new Q column; sol = gx - DG theta
+                for (int i = tid; i < n; i += BT) {
+                    S.Q[(size_t)qcol * n + i] = S.df[i] / nrm;
+                    double a = 0.0;
+#pragma unroll
+                    for (int j = 0; j < CDFT_MAXM; j++)
+                        if (j < mk) a += S.DG[(size_t)rg[j] * n + i] * th[j];
+                    S.sol[i] = S.gx[i] - a;
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            // new Q column, sol = gx - DG theta
-            for (int i = tid; i < n; i += BT) {
-                S.Q[(size_t)qcol * n + i] = S.df[i] / nrm;
-                double a = 0.0;
-                for (int j = 0; j < mk; j++) a += S.DG[(size_t)st.ring[j] * n + i] * st.theta[j];
-                S.sol[i] = S.gx[i] - a;
-            }
-            __syncthreads();
+            k += 1;
+            ka += 1;
+            mode = 2;
         }
-        k += 1;
-        ka += 1;
-        EVAL(2, beta);
-        resnorm = sqrt(fo[0]);
-        dfn2 = fo[1];
-        dfres = fo[2];
-    }
-#undef EVAL
-#undef SET_SOL_FROM_GX
 
-    // a mixed solve that ran out of iterations in FP32 still reports FP64 residuals
+        // ---------------- the one map evaluation
+        if (fp32) {
+            evalG<float, NC, MAXB>(S, pcs, tw, A32, P32, fld32, red, mode, bb, fo);
+            nevals32++;
+        } else {
+            evalG<double, NC, MAXB>(S, pcs, tw, A64, P64, fld64, red, mode, bb, fo);
+        }
+        nevals++;
+        resnorm = sqrt(fo[0]);
+
+        // ---------------- book-keeping after it
+        bool check_loop = false;
+        if (phase == PH_FIRST) {
+            up_bd = 1.e4 * resnorm;
+            tol = rtol * resnorm + atol;
+            picard_tol = picard_rtol * resnorm + picard_atol;
+            toosoon = (resnorm <= tol);
+            if (toosoon) {
+                phase = PH_DONE;
+            } else if (picard_maxit > 0) {
+                for (int i = tid; i < n; i += BT) S.sol[i] = S.gx[i];
+                __syncthreads();
+                pit = 0;
+                phase = (picard_maxit - 1 > 0) ? PH_PICARD : PH_RESTART;
+            } else {
+                phase = PH_RESTART;
+            }
+        } else if (phase == PH_PICARD) {
+            pit++;
+            const bool brk = (resnorm <= picard_tol || resnorm >= up_bd);
+            if (!brk) {
+                for (int i = tid; i < n; i += BT) S.sol[i] = S.gx[i];
+                __syncthreads();
+            }
+            if (brk || pit >= picard_maxit - 1) {
+                toosoon = (resnorm <= tol);
+                phase = toosoon ? PH_DONE : PH_RESTART;
+            }
+        } else if (phase == PH_SWITCH) {
+            phase = (resnorm <= tol) ? PH_DONE : PH_RESTART;
+        } else {                                                    // PH_RESTART or PH_STEP
+            dfn2 = fo[1];
+            dfres = fo[2];
+            if (phase == PH_RESTART) {
+                best = resnorm;
+                since_best = 0;
+            }
+            check_loop = true;
+        }
+        if (check_loop) {
+            // while (k < maxit + picard_maxit && resnorm > tol && !toosoon && resnorm < up_bd)
+            phase = PH_STEP;
+            if (!(k < maxit + picard_maxit && resnorm < up_bd)) {
+                phase = PH_DONE;
+            } else if (fp32) {
+                if (resnorm < 0.5 * best) {
+                    best = resnorm;
+                    since_best = 0;
+                } else {
+                    since_best++;
+                }
+                if (resnorm <= o.switch_factor * tol || since_best >= o.stall_window) phase = PH_SWITCH;
+            } else if (!(resnorm > tol)) {
+                phase = PH_DONE;
+            }
+        }
+    }
+
+    // a mixed solve that ran out of iterations in FP32 reports non-convergence
     int idid, errcode;
     if (toosoon) { idid = 1; errcode = -1; }
     else if (resnorm >= up_bd) { idid = 0; errcode = -2; }

@@ -37,7 +37,7 @@ struct PhiConst {
 
 namespace cdft {
 
-__constant__ static const double kCorr1[7][3] = {
+constexpr double kCorr1[7][3] = {
     {0.9105631445, -0.3084016918, -0.0906148351},
     {0.6361281449, 0.1860531159, 0.4527842806},
     {2.6861347891, -2.5030047259, 0.5962700728},
@@ -45,7 +45,7 @@ __constant__ static const double kCorr1[7][3] = {
     {97.759208784, -65.25588533, -4.1302112531},
     {-159.59154087, 83.318680481, 13.77663187},
     {91.297774084, -33.74692293, -8.6728470368}};
-__constant__ static const double kCorr2[7][3] = {
+constexpr double kCorr2[7][3] = {
     {0.7240946941, -0.5755498075, 0.0976883116},
     {2.2382791861, 0.6995095521, -0.2557574982},
     {-4.0025849485, 3.892567339, -9.155856153},

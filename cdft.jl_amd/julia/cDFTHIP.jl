@@ -1,0 +1,140 @@
+# cDFTHIP.jl -- Julia binding of libcdft_hip.so for cDFT.jl.
+#
+# Adds a device type `HIPNative` for `DFTOptions` and a more specific
+# `converge!(prob::DFTProblem, method::AASol, rho)` for PC-SAFT systems built with
+# it (src/methods/converge.jl:220).  System construction, bulk thermodynamics
+# (Clapeyron) and the external potential stay in cDFT.jl on the host; the functional
+# evaluation and the Picard/Anderson loop run in the HIP library.
+#
+#   using Clapeyron, cDFT, cDFTHIP
+#   system = DFTSystem(model, structure, surface, DFTOptions(HIPNative(0)))
+#   rho = initialize_profiles(system)
+#   converge!(system, rho; tol = 1e-8)          # rho overwritten, returns nothing
+#   cDFTHIP.last_result()                        # (idid, errcode, iters, ...)
+module cDFTHIP
+
+using cDFT, Clapeyron
+import KernelAbstractions
+
+const libcdft = get(ENV, "CDFT_HIP_LIB", joinpath(@__DIR__, "..", "libcdft_hip.so"))
+
+"""Device backend whose arrays live on the host; the library moves them."""
+struct HIPNative <: KernelAbstractions.Backend
+    device::Int
+end
+HIPNative() = HIPNative(0)
+KernelAbstractions.allocate(::HIPNative, ::Type{T}, dims::Tuple) where {T} = Array{T}(undef, dims)
+KernelAbstractions.synchronize(::HIPNative) = nothing
+cDFT.DFTOptions(dev::HIPNative; precision = Float64) = cDFT.DFTOptions(dev, :reverse; precision)
+
+# --- C structs (include/cdft_hip.h) -------------------------------------------
+struct CdftSystemDesc
+    nc::Cint; ndim::Cint; ngrid::NTuple{3,Cint}
+    lb::NTuple{3,Cdouble}; ub::NTuple{3,Cdouble}
+    T::Cdouble; p::Cdouble; L::Cdouble; psi::Cdouble
+    m::Ptr{Cdouble}; d::Ptr{Cdouble}; sigma::Ptr{Cdouble}; eps::Ptr{Cdouble}
+    rho_bulk::Ptr{Cdouble}; mu_res::Ptr{Cdouble}; vext::Ptr{Cdouble}
+    precision::Cint
+end
+
+struct CdftSolverOpts
+    maxit::Cint; beta::Cdouble; tol::Cdouble; anderson_start::Cdouble
+    m::Cint; hist_cap::Cint; hist::Ptr{Cdouble}
+end
+
+struct CdftResult
+    idid::Cint; errcode::Cint; iters::Cint; nevals::Cint; nhist::Cint
+    resnorm::Cdouble; tol::Cdouble; nevals_fp32::Cint; reserved::Cint
+end
+
+const CTX = Ref{Ptr{Cvoid}}(C_NULL)
+const LAST = Ref{Any}(nothing)
+last_result() = LAST[]
+
+function _ctx(dev::Int)
+    if CTX[] == C_NULL
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = ccall((:cdft_ctx_create, libcdft), Cint, (Cint, Ref{Ptr{Cvoid}}), dev, h)
+        rc == 0 || error("cdft_ctx_create failed: ", unsafe_string(ccall((:cdft_last_error, libcdft), Cstring, (Ptr{Cvoid},), C_NULL)))
+        CTX[] = h[]
+    end
+    return CTX[]
+end
+
+_check(rc, ctx) = rc == 0 || error("cdft error $rc: ", unsafe_string(ccall((:cdft_last_error, libcdft), Cstring, (Ptr{Cvoid},), ctx)))
+
+function _system(system, ctx)
+    model = system.model
+    st = system.structure
+    nd = cDFT.dimension(st)
+    (p, T) = st.conditions
+    ngrid = ntuple(i -> i <= nd ? Cint(st.ngrid[i]) : Cint(1), 3)
+    lb = ntuple(i -> i <= nd ? st.bounds[i][1] : 0.0, 3)
+    ub = ntuple(i -> i <= nd ? st.bounds[i][2] : 0.0, 3)
+    m = Float64.(model.params.segment.values)
+    d = Float64.(system.species.size)
+    sigma = Matrix{Float64}(model.params.sigma.values)
+    eps = Matrix{Float64}(model.params.epsilon.values)
+    rhob = Float64.(system.species.bulk_density)
+    mu = Float64.(system.species.chempot_res)
+    vext = nothing
+    if system.external_field !== nothing
+        cache = cDFT.preallocate_external_potential(system, zeros(st.ngrid..., length(model)))
+        vext = sum(c[end] for c in cache)          # beta V_ext (steele.jl:49-81)
+    end
+    prec = cDFT.fptype(system.options) == Float64 ? Cint(0) : Cint(1)
+    GC.@preserve m d sigma eps rhob mu vext begin
+        desc = CdftSystemDesc(length(model), nd, ngrid, lb, ub, T, p, cDFT.length_scale(model), 1.3862,
+                              pointer(m), pointer(d), pointer(sigma), pointer(eps), pointer(rhob), pointer(mu),
+                              vext === nothing ? Ptr{Cdouble}(C_NULL) : pointer(vext), prec)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        _check(ccall((:cdft_system_create, libcdft), Cint, (Ptr{Cvoid}, Ref{CdftSystemDesc}, Ref{Ptr{Cvoid}}), ctx, desc, h), ctx)
+        return h[]
+    end
+end
+
+const HIPSystem = cDFT.DFTSystem{<:Clapeyron.PCSAFTModel,<:Any,<:Any,<:Any,<:Any,<:Any,<:cDFT.DFTOptions{HIPNative}}
+
+function cDFT.converge!(prob::cDFT.DFTProblem{<:HIPSystem}, method::cDFT.AASol, rho::AbstractArray)
+    system = prob.system
+    ctx = _ctx(system.options.device.device)
+    sys = _system(system, ctx)
+    try
+        opts = CdftSolverOpts(method.maxit, method.beta, method.rtol, method.picard_rtol, method.m, 0, C_NULL)
+        res = Ref{CdftResult}()
+        buf = Array{Float64}(rho)
+        _check(ccall((:cdft_converge, libcdft), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ref{CdftSolverOpts}, Ref{CdftResult}),
+                     sys, buf, opts, res), ctx)
+        copyto!(rho, buf)
+        LAST[] = res[]
+    finally
+        ccall((:cdft_system_destroy, libcdft), Cint, (Ptr{Cvoid},), sys)
+    end
+    return nothing
+end
+
+"""Batched solve of many HIPNative systems in one persistent launch."""
+function converge_batch!(systems::AbstractVector, rhos::AbstractVector; maxit = 10_000, beta = 1e-2, tol = 1e-4,
+                         anderson_start = 1e-1, anderson_m = 5)
+    ctx = _ctx(first(systems).options.device.device)
+    hs = [_system(s, ctx) for s in systems]
+    bufs = [Array{Float64}(r) for r in rhos]
+    res = Vector{CdftResult}(undef, length(systems))
+    opts = CdftSolverOpts(maxit, beta, tol, anderson_start, anderson_m, 0, C_NULL)
+    try
+        GC.@preserve bufs begin
+            ptrs = [pointer(b) for b in bufs]
+            _check(ccall((:cdft_converge_batch, libcdft), Cint,
+                         (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cdouble}}, Ref{CdftSolverOpts}, Ptr{CdftResult}),
+                         ctx, length(hs), hs, ptrs, opts, res), ctx)
+        end
+        foreach(copyto!, rhos, bufs)
+    finally
+        foreach(h -> ccall((:cdft_system_destroy, libcdft), Cint, (Ptr{Cvoid},), h), hs)
+    end
+    return res
+end
+
+export HIPNative, converge_batch!
+
+end # module
