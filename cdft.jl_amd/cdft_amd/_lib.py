@@ -45,7 +45,8 @@ class SolverOpts(C.Structure):
 class Result(C.Structure):
     _fields_ = [("idid", C.c_int), ("errcode", C.c_int), ("iters", C.c_int),
                 ("nevals", C.c_int), ("nhist", C.c_int), ("resnorm", C.c_double),
-                ("tol", C.c_double), ("nevals_fp32", C.c_int), ("reserved", C.c_int)]
+                ("tol", C.c_double), ("nevals_fp32", C.c_int), ("reserved", C.c_int),
+                ("device_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

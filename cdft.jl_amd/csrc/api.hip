@@ -487,6 +487,7 @@ int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdf
     r->tol = tol;
     r->nevals_fp32 = 0;
     r->reserved = 0;
+    r->device_ms = 0.0;
     return CDFT_OK;
 }
 
@@ -649,6 +650,8 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
     float ms = 0.0f;
     HIPCHECK_CTX(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_kernel_ms = ms;
+    int wall_rate_khz = 0;
+    HIPCHECK_CTX(ctx, hipDeviceGetAttribute(&wall_rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
     for (int q = 0; q < nsys; q++) {
         cdft_result &r = results[order[q]];
         r.idid = hr[q].idid;
@@ -660,6 +663,7 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         r.tol = hr[q].tol;
         r.nevals_fp32 = hr[q].nevals32;
         r.reserved = 0;
+        r.device_ms = wall_rate_khz > 0 ? (double)hr[q].ticks / (double)wall_rate_khz : 0.0;
     }
     return CDFT_OK;
 }

@@ -55,6 +55,7 @@ struct BOpts {
 struct BResult {
     int idid, errcode, iters, nevals, nevals32, pad;
     double resnorm, tol;
+    unsigned long long ticks;   // wall_clock64 ticks spent in the workgroup
 };
 
 constexpr int BT = 256;        // threads per workgroup
@@ -401,6 +402,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
     __shared__ PhiConst pcs;
     constexpr int F = 7;
     const int tid = threadIdx.x;
+    const unsigned long long t_start = wall_clock64();
 
     // ---- descriptor -> registers / LDS, once
     const BSys *D = systems + blockIdx.x;
@@ -698,6 +700,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
         r.pad = 0;
         r.resnorm = resnorm;
         r.tol = tol;
+        r.ticks = wall_clock64() - t_start;
         results[blockIdx.x] = r;
     }
 }

@@ -44,7 +44,7 @@ end
 
 struct CdftResult
     idid::Cint; errcode::Cint; iters::Cint; nevals::Cint; nhist::Cint
-    resnorm::Cdouble; tol::Cdouble; nevals_fp32::Cint; reserved::Cint
+    resnorm::Cdouble; tol::Cdouble; nevals_fp32::Cint; reserved::Cint; device_ms::Cdouble
 end
 
 const CTX = Ref{Ptr{Cvoid}}(C_NULL)

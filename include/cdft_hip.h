@@ -93,6 +93,7 @@ typedef struct {
     double tol;            /* rtol*res0 + atol                                    */
     int nevals_fp32;       /* map evaluations done in FP32 (CDFT_MIXED)           */
     int reserved;
+    double device_ms;      /* batched path: this system's time in its workgroup   */
 } cdft_result;
 
 /* Context: device selection and one HIP stream.  (No counterpart: the
