@@ -93,7 +93,6 @@ void free_sys_buffers(cdft_sys *s)
     for (double **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (s->H64) { (void)hipFree(s->H64); s->H64 = nullptr; }
-    if (s->H32s) { (void)hipFree(s->H32s); s->H32s = nullptr; }
 }
 
 template <int NC>
@@ -155,10 +154,7 @@ int cdft_ctx_destroy(cdft_ctx *ctx)
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    for (auto &kv : ctx->twiddles) {
-        (void)hipFree(kv.second.first);
-        (void)hipFree(kv.second.second);
-    }
+    for (auto &kv : ctx->twiddles) (void)hipFree(kv.second);
     delete ctx;
     return CDFT_OK;
 }
@@ -502,30 +498,30 @@ static int fft_length(int N)
     return M;
 }
 
-static size_t batched_lds_bytes(const cdft_sys *s) { return batched_lds_layout(s->N, s->M, s->nc); }
+static size_t batched_lds_bytes(const cdft_sys *s, bool g_lds) { return batched_lds_layout(s->N, s->M, s->nc, g_lds); }
 
 static int batched_prepare(cdft_ctx *ctx, cdft_sys *s)
 {
     hipStream_t st = ctx->stream;
     if (s->M == 0) {
         int M = fft_length(s->N);
-        if (M > 4096) return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched path: grid too large (FFT length > 4096)");
-        size_t cnt = (size_t)s->nc * (s->F - 1) * M * 2;
+        if (M > 1024)
+            return fail(ctx, CDFT_ERR_UNSUPPORTED,
+                        "batched path: grid too large (FFT ring > 1024, i.e. N > 512 unless N is 1024); use cdft_converge");
+        const int G1 = (M >> 1) + 1;
+        size_t cnt = (size_t)s->nc * (s->F - 1) * G1;
         HIPCHECK_CTX(ctx, hipMalloc((void **)&s->H64, sizeof(double) * cnt));
-        HIPCHECK_CTX(ctx, hipMalloc((void **)&s->H32s, sizeof(float) * cnt));
-        hipLaunchKernelGGL(k_setup_spec, dim3(nblocks(M), s->nc * (s->F - 1)), dim3(256), 0, st, s->h, s->N, M, s->F,
-                           s->H64, s->H32s);
+        hipLaunchKernelGGL(k_setup_spec, dim3(nblocks(G1), s->nc * (s->F - 1)), dim3(256), 0, st, s->h, s->N, M, s->F,
+                           s->H64);
         HIPCHECK_CTX(ctx, hipGetLastError());
         s->M = M;
     }
     if (!ctx->twiddles.count(s->M)) {
         double *t64 = nullptr;
-        float *t32 = nullptr;
         HIPCHECK_CTX(ctx, hipMalloc((void **)&t64, sizeof(double) * 2 * s->M));
-        HIPCHECK_CTX(ctx, hipMalloc((void **)&t32, sizeof(float) * 2 * s->M));
-        hipLaunchKernelGGL(k_setup_tw, dim3(nblocks(s->M)), dim3(256), 0, st, s->M, t64, t32);
+        hipLaunchKernelGGL(k_setup_tw, dim3(nblocks(s->M)), dim3(256), 0, st, s->M, t64);
         HIPCHECK_CTX(ctx, hipGetLastError());
-        ctx->twiddles[s->M] = {t64, t32};
+        ctx->twiddles[s->M] = t64;
     }
     return CDFT_OK;
 }
@@ -553,17 +549,19 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
     hipStream_t st = ctx->stream;
     int maxlds = 0;
     HIPCHECK_CTX(ctx, hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
-    size_t lds = 0;
-    int maxM = 0;
+    size_t lds_g = 0, lds_nog = 0;
     for (int i = 0; i < nsys; i++) {
         int rc = batched_prepare(ctx, sys[i]);
         if (rc != CDFT_OK) return rc;
-        lds = std::max(lds, batched_lds_bytes(sys[i]));
-        maxM = std::max(maxM, sys[i]->M);
+        lds_g = std::max(lds_g, batched_lds_bytes(sys[i], true));
+        lds_nog = std::max(lds_nog, batched_lds_bytes(sys[i], false));
     }
-    size_t lds_limit = std::max<size_t>((size_t)maxlds, 160 * 1024);
-    if (lds + sizeof(AState) + sizeof(PhiConst) > lds_limit)
-        return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched path: system too large for one workgroup's LDS");
+    // spectra in LDS when the largest system leaves room for them, else read from HBM
+    const size_t lds_limit = std::max<size_t>((size_t)maxlds, 160 * 1024) - sizeof(AState) - sizeof(PhiConst);
+    const bool g_lds = lds_g <= lds_limit;
+    const size_t lds = g_lds ? lds_g : lds_nog;
+    if (lds > lds_limit)
+        return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched path: system too large for one workgroup's LDS; use cdft_converge");
     // largest systems first: the tail of the launch is then the short ones
     std::vector<int> order(nsys);
     for (int i = 0; i < nsys; i++) order[i] = i;
@@ -582,10 +580,8 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         b.NA = s->NA;
         for (int c = 0; c < s->nc; c++) b.cp[c] = s->cp[c];
         b.vext = s->vext;
-        b.H64 = s->H64;
-        b.H32 = s->H32s;
-        b.tw64 = ctx->twiddles[s->M].first;
-        b.tw32 = ctx->twiddles[s->M].second;
+        b.G = s->H64;
+        b.tw64 = ctx->twiddles[s->M];
         long n = (long)s->N * s->nc;
         if (device_ptrs) {
             b.rho = rho[order[q]];
@@ -618,6 +614,7 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
     bo.mixed = (prec == CDFT_MIXED);
     bo.switch_factor = 4.0;
     bo.stall_window = 40;
+    bo.g_lds = g_lds ? 1 : 0;
     auto launch = [&](auto kern) -> int {
         HIPCHECK_CTX(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev0, st));
@@ -627,16 +624,12 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         return CDFT_OK;
     };
     int rc;
-    const int mb = maxM <= 1024 ? 1 : (maxM <= 2048 ? 2 : 4);
-#define CDFT_LAUNCH_NC(NCV)                                  \
-    (mb == 1 ? launch(k_batched<NCV, 1>) : mb == 2 ? launch(k_batched<NCV, 2>) : launch(k_batched<NCV, 4>))
     switch (nc) {
-    case 1: rc = CDFT_LAUNCH_NC(1); break;
-    case 2: rc = CDFT_LAUNCH_NC(2); break;
-    case 3: rc = CDFT_LAUNCH_NC(3); break;
-    default: rc = CDFT_LAUNCH_NC(4); break;
+    case 1: rc = launch(k_batched<1>); break;
+    case 2: rc = launch(k_batched<2>); break;
+    case 3: rc = launch(k_batched<3>); break;
+    default: rc = launch(k_batched<4>); break;
     }
-#undef CDFT_LAUNCH_NC
     if (rc != CDFT_OK) return rc;
     std::vector<BResult> hr(nsys);
     HIPCHECK_CTX(ctx, hipMemcpyAsync(hr.data(), dr, sizeof(BResult) * nsys, hipMemcpyDeviceToHost, st));

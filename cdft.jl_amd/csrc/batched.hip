@@ -6,12 +6,18 @@
 // The algorithm per system is exactly cdft_converge's (converge!/aasol,
 // src/methods/converge.jl:214-254, src/utils/anderson.jl:301-443).
 //
-// Convolutions run in LDS as FFTs: the reference's spectral operator
+// Convolutions run in LDS as FFTs.  The reference's spectral operator
 // Re IFFT_N(FFT_N(u) * Omega_f) equals the circular convolution h_f (*) u with the
-// real kernel h_f = IFFT_N(Omega_herm_f).  With M = N (N a power of two) it is
-// applied directly as IFFT_M(FFT_M(u) * H_f), H_f = DFT_N(h_f); otherwise through a
-// zero-padded power-of-two length M >= 2N - 1 (linear convolution, then the period-N
-// fold out[i] = y[i] + y[i + N]), which is exact for every N.
+// real kernel h_f = IFFT_N(Omega_herm_f), which is even (scalar weights) or odd (the
+// vector weight).  With M = N (N a power of two) it is applied directly; otherwise
+// the N-periodic kernel is unrolled onto a power-of-two ring M >= 2N - 1 at lags
+// -(N-1) .. N-1 (h_M[t] = h[t], h_M[M - s] = h[N - s]), and the length-M circular
+// convolution of the zero-padded profile then equals the N-periodic one on 0..N-1.
+// Either way the kernel spectrum G_f on the ring is real (even kernels) or purely
+// imaginary (odd), so it is stored as one real half-spectrum per field, k <= M/2,
+// and kept in LDS.  Two real fields travel in one complex FFT (u1 + i u2), and the
+// three field pairs of a component share one set of barrier-separated FFT passes:
+// four FFT rounds per component and evaluation.
 //
 // Precision: CDFT_FP64 runs the whole map in FP64.  CDFT_MIXED runs the FFTs, the
 // weighted densities and Phi in FP32 (Phi's n3-singular factors in FP64) until the
@@ -35,10 +41,8 @@ struct BSys {
     double NA, beta_unused;
     double cp[CDFT_MAXC];
     const double *vext;        // [c*N + i] or null
-    const double *H64;         // spectra [(c*(F-1) + f-1)*M + k] as (re, im) pairs
-    const float *H32;
+    const double *G;           // real half-spectra [(c*(F-1) + f-1)*(M/2+1) + k], k <= M/2
     const double *tw64;        // W_M^t, t < M, (re, im) pairs
-    const float *tw32;
     double *rho;               // in: seed, out: converged profile  [c*N + i]
     double *sol, *gx, *df, *dg, *res, *Q, *DG;   // Anderson storage, n = N*nc each (Q, DG: CDFT_MAXM cols)
     PhiConst pc;
@@ -50,6 +54,7 @@ struct BOpts {
     int mixed;                 // 1: FP32 map until the noise floor, then FP64
     double switch_factor;      // FP32 -> FP64 when resnorm <= switch_factor * tol
     int stall_window;          // ... or when it has not halved in this many iterations
+    int g_lds;                 // 1: the spectra are copied into LDS at the start
 };
 
 struct BResult {
@@ -70,42 +75,44 @@ __device__ __attribute__((always_inline)) inline c2<T> cmul(c2<T> a, c2<T> b)
 
 #define CDFT_INLINE __device__ __attribute__((always_inline)) inline
 
-// In-place Stockham FFT of length M (power of two, 4 <= M <= 1024 * MAXB) in LDS:
-// radix-4 passes and a final radix-2 pass when log2 M is odd.  Each pass loads
-// all its butterfly inputs into registers, synchronises, then stores, so one
-// buffer suffices.  Twiddles W_M^t come from the LDS table tw.  Unnormalised;
-// sign -1 forward, +1 inverse.
-template <typename T, int MAXB>
-CDFT_INLINE void fft_lds(c2<T> *a, const c2<double> *tw, int M, int log2M, bool inverse)
+// In-place Stockham FFTs of NARR arrays of length M (power of two, 4 <= M <= 1024)
+// in LDS, array q at a + q * astride, all advancing through the same passes:
+// radix-4 passes and a final radix-2 pass when log2 M is odd.  Each pass loads its
+// butterfly inputs into registers, synchronises, then stores, so one buffer per
+// array suffices.  Twiddles W_M^t come from the LDS table tw.  Unnormalised; sign
+// -1 forward, +1 inverse.
+template <typename T, int NARR>
+CDFT_INLINE void fft_lds(c2<T> *a, int astride, const c2<double> *tw, int M, int log2M, bool inverse)
 {
     const int tid = threadIdx.x;
-    int Ns = 1;
-    const int nr4 = log2M >> 1;
     const T sg = inverse ? T(1) : T(-1);
+    const int nr4 = log2M >> 1;
+    const int nb = M >> 2, lnb = log2M - 2;        // radix-4 butterflies per array
+    int Ns = 1, lNs = 0;
     for (int pass = 0; pass < nr4; pass++) {
-        c2<T> v[MAXB][4];
-        const int nb = M >> 2;
-        const int step = M / (Ns * 4);
+        c2<T> v[NARR][4];
+        const int lstep = log2M - 2 - lNs;          // twiddle stride M / (4 Ns)
 #pragma unroll
-        for (int b = 0; b < MAXB; b++) {
-            int j = tid + b * BT;
-            if (j < nb) {
-                int k = j & (Ns - 1);
+        for (int b = 0; b < NARR; b++) {
+            const int g = tid + b * BT;
+            const int q = g >> lnb, j = g & (nb - 1);
+            if (q < NARR) {
+                const c2<T> *x = a + q * astride;
+                const int k = j & (Ns - 1);
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    c2<T> x = a[j + r * nb];
+                    c2<T> y = x[j + r * nb];
                     if (r) {
-                        c2<double> wd = tw[(r * k * step) & (M - 1)];
-                        c2<T> w = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
-                        x = cmul(x, w);
+                        const c2<double> wd = tw[(r * k) << lstep];
+                        y = cmul(y, c2<T>{T(wd.x), inverse ? T(-wd.y) : T(wd.y)});
                     }
-                    v[b][r] = x;
+                    v[b][r] = y;
                 }
-                c2<T> s02 = {v[b][0].x + v[b][2].x, v[b][0].y + v[b][2].y};
-                c2<T> d02 = {v[b][0].x - v[b][2].x, v[b][0].y - v[b][2].y};
-                c2<T> s13 = {v[b][1].x + v[b][3].x, v[b][1].y + v[b][3].y};
-                c2<T> d13 = {v[b][1].x - v[b][3].x, v[b][1].y - v[b][3].y};
-                c2<T> id13 = {-sg * d13.y, sg * d13.x};   // (sg i) * d13
+                const c2<T> s02 = {v[b][0].x + v[b][2].x, v[b][0].y + v[b][2].y};
+                const c2<T> d02 = {v[b][0].x - v[b][2].x, v[b][0].y - v[b][2].y};
+                const c2<T> s13 = {v[b][1].x + v[b][3].x, v[b][1].y + v[b][3].y};
+                const c2<T> d13 = {v[b][1].x - v[b][3].x, v[b][1].y - v[b][3].y};
+                const c2<T> id13 = {-sg * d13.y, sg * d13.x};   // (sg i) * d13
                 v[b][0] = {s02.x + s13.x, s02.y + s13.y};
                 v[b][1] = {d02.x + id13.x, d02.y + id13.y};
                 v[b][2] = {s02.x - s13.x, s02.y - s13.y};
@@ -114,42 +121,50 @@ CDFT_INLINE void fft_lds(c2<T> *a, const c2<double> *tw, int M, int log2M, bool 
         }
         __syncthreads();
 #pragma unroll
-        for (int b = 0; b < MAXB; b++) {
-            int j = tid + b * BT;
-            if (j < nb) {
-                int k = j & (Ns - 1);
-                int d = (j / Ns) * Ns * 4 + k;
+        for (int b = 0; b < NARR; b++) {
+            const int g = tid + b * BT;
+            const int q = g >> lnb, j = g & (nb - 1);
+            if (q < NARR) {
+                c2<T> *x = a + q * astride;
+                const int k = j & (Ns - 1);
+                const int d = ((j >> lNs) << (lNs + 2)) + k;
 #pragma unroll
-                for (int r = 0; r < 4; r++) a[d + r * Ns] = v[b][r];
+                for (int r = 0; r < 4; r++) x[d + r * Ns] = v[b][r];
             }
         }
         __syncthreads();
         Ns <<= 2;
+        lNs += 2;
     }
     if (log2M & 1) {
-        c2<T> v[2 * MAXB][2];
-        const int nb = M >> 1;
+        const int nb2 = M >> 1, lnb2 = log2M - 1;
+        const int lstep = log2M - 1 - lNs;          // M / (2 Ns)
+        c2<T> v[2 * NARR][2];
 #pragma unroll
-        for (int b = 0; b < 2 * MAXB; b++) {
-            int j = tid + b * BT;
-            if (j < nb) {
-                int k = j & (Ns - 1);
-                c2<double> wd = tw[(k * (M / (Ns * 2))) & (M - 1)];
-                c2<T> w = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
-                c2<T> x0 = a[j], x1 = cmul(a[j + nb], w);
+        for (int b = 0; b < 2 * NARR; b++) {
+            const int g = tid + b * BT;
+            const int q = g >> lnb2, j = g & (nb2 - 1);
+            if (q < NARR) {
+                const c2<T> *x = a + q * astride;
+                const int k = j & (Ns - 1);
+                const c2<double> wd = tw[k << lstep];
+                const c2<T> w = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
+                const c2<T> x0 = x[j], x1 = cmul(x[j + nb2], w);
                 v[b][0] = {x0.x + x1.x, x0.y + x1.y};
                 v[b][1] = {x0.x - x1.x, x0.y - x1.y};
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int b = 0; b < 2 * MAXB; b++) {
-            int j = tid + b * BT;
-            if (j < nb) {
-                int k = j & (Ns - 1);
-                int d = (j / Ns) * Ns * 2 + k;
-                a[d] = v[b][0];
-                a[d + Ns] = v[b][1];
+        for (int b = 0; b < 2 * NARR; b++) {
+            const int g = tid + b * BT;
+            const int q = g >> lnb2, j = g & (nb2 - 1);
+            if (q < NARR) {
+                c2<T> *x = a + q * astride;
+                const int k = j & (Ns - 1);
+                const int d = ((j >> lNs) << (lNs + 1)) + k;
+                x[d] = v[b][0];
+                x[d + Ns] = v[b][1];
             }
         }
         __syncthreads();
@@ -185,8 +200,6 @@ struct SysRegs {
     double NA;
     double cp[NC];
     const double *__restrict__ vext;
-    const double *__restrict__ H64;
-    const float *__restrict__ H32;
     double *__restrict__ rho;
     double *__restrict__ sol;
     double *__restrict__ gx;
@@ -197,37 +210,43 @@ struct SysRegs {
     double *__restrict__ DG;
 };
 
+// Spectral multipliers of the three field pairs of one component at ring index k:
+// Z_q = H_{1+2q} + i H_{2+2q} with H_f = G_f (even kernels) or i G_f (the vector
+// weight, slot 3), from the half-spectra Gc[(f-1)*G1 + k'] with k' = min(k, M-k)
+// and G_f(M-k) = +-G_f(k).  The transposed operator uses conj(Z_q): the vector
+// weight's transpose is its negative, the scalar weights are symmetric.
 template <typename T>
-struct HSel;
-template <>
-struct HSel<double> {
-    static CDFT_INLINE const double *get(const double *h64, const float *) { return h64; }
-};
-template <>
-struct HSel<float> {
-    static CDFT_INLINE const float *get(const double *, const float *h32) { return h32; }
-};
+CDFT_INLINE void pair_multipliers(const double *Gc, int G1, int M, int k, c2<T> (&Z)[3])
+{
+    const bool up = k > (M >> 1);
+    const int kk = up ? M - k : k;
+    const double g3 = up ? -Gc[2 * G1 + kk] : Gc[2 * G1 + kk];
+    Z[0] = {T(Gc[kk]), T(Gc[G1 + kk])};
+    Z[1] = {T(0), T(g3 + Gc[3 * G1 + kk])};
+    Z[2] = {T(Gc[4 * G1 + kk]), T(Gc[5 * G1 + kk])};
+}
 
 // One evaluation of the fixed-point map plus the Anderson book-keeping of mode
 // (1: first/Picard step, 2: aa_point!).  out = |res|^2, |df|^2, df.res.
-// LDS: tw (M, FP64), work A and accumulator P (M complex T each, at FP64
-// stride), fields fld (F*N*NC of T at FP64 stride), red (64 doubles).
-template <typename T, int NC, int MAXB>
-CDFT_INLINE void evalG(const SysRegs<NC> &S, const PhiConst &P, const c2<double> *tw, c2<T> *A, c2<T> *Pc, T *fld,
-                       double *red, int mode, double b, double (&out)[3])
+// LDS: tw (M, FP64), spectra G (6 NC (M/2+1) FP64, or global), three work arrays W
+// and the accumulator P (M complex T each, at FP64 stride), fields fld (F*N*NC of T
+// at FP64 stride), red (64 doubles).
+template <typename T, int NC>
+CDFT_INLINE void evalG(const SysRegs<NC> &S, const PhiConst &P, const c2<double> *tw, const double *G, c2<T> *W,
+                       c2<T> *Pc, T *fld, double *red, int mode, double b, double (&out)[3])
 {
     constexpr int F = 7;
     const int tid = threadIdx.x;
-    const int N = S.N, M = S.M, lg = S.lg;
-    const bool fold = (M != N);
-    const T *H = HSel<T>::get(S.H64, S.H32);
+    const int N = S.N, M = S.M, lg = S.lg, G1 = (S.M >> 1) + 1;
     const T invM = T(1) / T(M);
+    const T sc = T(S.NA) * invM;
 
     for (int c = 0; c < NC; c++) {
+        const double *Gc = G + (size_t)c * (F - 1) * G1;
         // rho = exp(sol): slot 0 of the fields and the zero-padded FFT input
         for (int k = tid; k < M; k += BT) {
             if (k < N) {
-                double r = exp(S.sol[c * N + k]);
+                const double r = exp(S.sol[c * N + k]);
                 fld[(c * F) * N + k] = T(S.NA * r);
                 Pc[k] = {T(r), T(0)};
             } else {
@@ -235,28 +254,25 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, const PhiConst &P, const c2<double>
             }
         }
         __syncthreads();
-        fft_lds<T, MAXB>(Pc, tw, M, lg, false);
-        for (int q = 0; q < 3; q++) {
-            const int f1 = 1 + 2 * q, f2 = 2 + 2 * q;
-            const T *H1 = H + (size_t)(c * (F - 1) + f1 - 1) * M * 2;
-            const T *H2 = H + (size_t)(c * (F - 1) + f2 - 1) * M * 2;
-            for (int k = tid; k < M; k += BT) {
-                c2<T> p = Pc[k];
-                c2<T> a1 = cmul(p, c2<T>{H1[2 * k], H1[2 * k + 1]});
-                c2<T> a2 = cmul(p, c2<T>{H2[2 * k], H2[2 * k + 1]});
-                A[k] = {a1.x - a2.y, a1.y + a2.x};          // P H1 + i P H2
-            }
-            __syncthreads();
-            fft_lds<T, MAXB>(A, tw, M, lg, true);
-            const T sc = T(S.NA) * invM;
-            for (int i = tid; i < N; i += BT) {
-                c2<T> y = A[i];
-                if (fold && i + N < M) { y.x += A[i + N].x; y.y += A[i + N].y; }
-                fld[(c * F + f1) * N + i] = sc * y.x;
-                fld[(c * F + f2) * N + i] = sc * y.y;
-            }
-            __syncthreads();
+        fft_lds<T, 1>(Pc, M, tw, M, lg, false);
+        for (int k = tid; k < M; k += BT) {
+            c2<T> Z[3];
+            pair_multipliers<T>(Gc, G1, M, k, Z);
+            const c2<T> p = Pc[k];
+#pragma unroll
+            for (int q = 0; q < 3; q++) W[q * M + k] = cmul(p, Z[q]);
         }
+        __syncthreads();
+        fft_lds<T, 3>(W, M, tw, M, lg, true);
+        for (int i = tid; i < N; i += BT) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const c2<T> y = W[q * M + i];
+                fld[(c * F + 1 + 2 * q) * N + i] = sc * y.x;
+                fld[(c * F + 2 + 2 * q) * N + i] = sc * y.y;
+            }
+        }
+        __syncthreads();
     }
     // Phi and dPhi/dn, in place over the field slots
     for (int i = tid; i < N; i += BT) {
@@ -274,53 +290,50 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, const PhiConst &P, const c2<double>
     __syncthreads();
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
     for (int c = 0; c < NC; c++) {
-        for (int k = tid; k < M; k += BT) Pc[k] = {T(0), T(0)};
-        for (int q = 0; q < 3; q++) {
-            const int f1 = 1 + 2 * q, f2 = 2 + 2 * q;
-            const T s1 = (f1 == 3) ? T(-1) : T(1), s2 = (f2 == 3) ? T(-1) : T(1);   // vector slot subtracts
-            const T *H1 = H + (size_t)(c * (F - 1) + f1 - 1) * M * 2;
-            const T *H2 = H + (size_t)(c * (F - 1) + f2 - 1) * M * 2;
-            for (int k = tid; k < M; k += BT)
-                A[k] = (k < N) ? c2<T>{fld[(c * F + f1) * N + k], fld[(c * F + f2) * N + k]} : c2<T>{T(0), T(0)};
-            __syncthreads();
-            fft_lds<T, MAXB>(A, tw, M, lg, false);
-            for (int k = tid; k < M; k += BT) {
-                int km = (M - k) & (M - 1);
-                c2<T> a = A[k], bb = A[km];
-                c2<T> u1 = {T(0.5) * (a.x + bb.x), T(0.5) * (a.y - bb.y)};     // (a + conj b)/2
-                c2<T> u2 = {T(0.5) * (a.y + bb.y), T(-0.5) * (a.x - bb.x)};    // (a - conj b)/(2i)
-                c2<T> t1 = cmul(u1, c2<T>{H1[2 * k], H1[2 * k + 1]});
-                c2<T> t2 = cmul(u2, c2<T>{H2[2 * k], H2[2 * k + 1]});
-                c2<T> acc = Pc[k];
-                acc.x += s1 * t1.x + s2 * t2.x;
-                acc.y += s1 * t1.y + s2 * t2.y;
-                Pc[k] = acc;
-            }
-            __syncthreads();
+        const double *Gc = G + (size_t)c * (F - 1) * G1;
+        // transposed convolutions: Re IFFT(sum_q FFT(u_{1+2q} + i u_{2+2q}) conj(Z_q))
+        for (int k = tid; k < M; k += BT) {
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                W[q * M + k] = (k < N) ? c2<T>{fld[(c * F + 1 + 2 * q) * N + k], fld[(c * F + 2 + 2 * q) * N + k]}
+                                       : c2<T>{T(0), T(0)};
         }
-        fft_lds<T, MAXB>(Pc, tw, M, lg, true);
+        __syncthreads();
+        fft_lds<T, 3>(W, M, tw, M, lg, false);
+        for (int k = tid; k < M; k += BT) {
+            c2<T> Z[3];
+            pair_multipliers<T>(Gc, G1, M, k, Z);
+            c2<T> acc = {T(0), T(0)};
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const c2<T> w = W[q * M + k];
+                acc.x += w.x * Z[q].x + w.y * Z[q].y;              // w * conj(Z)
+                acc.y += w.y * Z[q].x - w.x * Z[q].y;
+            }
+            Pc[k] = acc;
+        }
+        __syncthreads();
+        fft_lds<T, 1>(Pc, M, tw, M, lg, true);
         for (int i = tid; i < N; i += BT) {
-            T y = Pc[i].x;
-            if (fold && i + N < M) y += Pc[i + N].x;
-            double s = double(fld[(c * F) * N + i]) + double(y * invM);
+            double s = double(fld[(c * F) * N + i]) + double(Pc[i].x * invM);
             const int t = c * N + i;
             if (S.vext) s += S.vext[t];
             double lgv = S.cp[c] - s;
             lgv = fmin(fmax(lgv, -100.0), 100.0);          // clamp!(ln_Gx, -100, 100)
-            double x = S.sol[t];
-            double gnew = (b == 1.0) ? lgv : (1.0 - b) * x + b * lgv;
+            const double x = S.sol[t];
+            const double gnew = (b == 1.0) ? lgv : (1.0 - b) * x + b * lgv;
             if (mode == 1) {
                 S.gx[t] = gnew;
-                double r = gnew - x;
+                const double r = gnew - x;
                 S.res[t] = r;
                 v0 += r * r;
             } else {
-                double gold = S.gx[t], rold = S.res[t];
+                const double gold = S.gx[t], rold = S.res[t];
                 S.dg[t] = -gold + gnew;
                 S.gx[t] = gnew;
-                double r = gnew - x;
+                const double r = gnew - x;
                 S.res[t] = r;
-                double d = -rold + r;
+                const double d = -rold + r;
                 S.df[t] = d;
                 v0 += r * r;
                 v1 += d * d;
@@ -336,47 +349,80 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, const PhiConst &P, const c2<double>
     out[2] = v[2];
 }
 
-// Householder downdate of the Anderson R (same arithmetic as the host version).
+// Sum over the 8 lanes of one column group (lanes 8j .. 8j+7) of a wave.
+CDFT_INLINE double col8_sum(double x)
+{
+    x += __shfl_xor(x, 1, 64);
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 4, 64);
+    return x;
+}
+
+// Downdate of the Anderson QR when the oldest column leaves (anderson.jl, qr! of
+// R[:, 2:m]): Householder QR of the m x (m-1) matrix R[:, 1:m] with LAPACK geqrf
+// conventions (beta = -sign(alpha) |x|, tau = (beta - alpha) / beta, reflector
+// scaled to a unit leading entry), its explicit Q factor Qx (m x (m-1), column-major
+// ld m) and the new R (upper (m-1) x (m-1), last column zero).  Each wave holds
+// the matrix one element per lane, row i = lane & 7 and column j = lane >> 3, and
+// reduces over the 8 lanes of a column with xor shuffles.  Every wave computes the
+// same values; wave 0 writes them.  Called by all threads of the workgroup.
 CDFT_INLINE void dev_householder_downdate(double *R, int m, double *Qx)
 {
-    int nr = m, ncol = m - 1;
-    double A[CDFT_MAXM * CDFT_MAXM], tau[CDFT_MAXM], v[CDFT_MAXM];
-    for (int j = 0; j < ncol; j++)
-        for (int i = 0; i < nr; i++) A[j * CDFT_MAXM + i] = R[(j + 1) * m + i];
-    for (int j = 0; j < ncol; j++) {
-        double alpha = A[j * CDFT_MAXM + j], xn = 0.0;
-        for (int i = j + 1; i < nr; i++) xn += A[j * CDFT_MAXM + i] * A[j * CDFT_MAXM + i];
-        xn = sqrt(xn);
-        if (xn == 0.0) { tau[j] = 0.0; continue; }
-        double beta = -copysign(hypot(alpha, xn), alpha);
-        tau[j] = (beta - alpha) / beta;
-        double sc = 1.0 / (alpha - beta);
-        for (int i = j + 1; i < nr; i++) A[j * CDFT_MAXM + i] *= sc;
-        A[j * CDFT_MAXM + j] = beta;
-        for (int jj = j + 1; jj < ncol; jj++) {
-            double w = A[jj * CDFT_MAXM + j];
-            for (int i = j + 1; i < nr; i++) w += A[j * CDFT_MAXM + i] * A[jj * CDFT_MAXM + i];
-            w *= tau[j];
-            A[jj * CDFT_MAXM + j] -= w;
-            for (int i = j + 1; i < nr; i++) A[jj * CDFT_MAXM + i] -= w * A[j * CDFT_MAXM + i];
-        }
+    const int lane = threadIdx.x & 63, i = lane & 7, j = lane >> 3;
+    const int nr = m, ncol = m - 1;
+    double a = (i < nr && j < ncol) ? R[(j + 1) * m + i] : 0.0;
+    double tau[CDFT_MAXM];
+#pragma unroll
+    for (int h = 0; h < CDFT_MAXM; h++) {
+        tau[h] = 0.0;
+        if (h >= ncol) continue;
+        const double x = __shfl(a, h * 8 + i, 64);                 // A[i][h]
+        const double alpha = __shfl(a, h * 8 + h, 64);             // A[h][h]
+        const double xn = sqrt(col8_sum((i > h && i < nr) ? x * x : 0.0));
+        if (xn == 0.0) continue;                                    // H_h = I
+        const double beta = -copysign(hypot(alpha, xn), alpha);
+        const double th = (beta - alpha) / beta;
+        tau[h] = th;
+        const double sc = 1.0 / (alpha - beta);
+        const double v = (i == h) ? 1.0 : ((i > h && i < nr) ? x * sc : 0.0);
+        const double w = col8_sum(v * a) * th;                      // (v^T A[:, j]) tau
+        if (j > h && j < ncol) a -= w * v;
+        if (j == h && i == h) a = beta;
+        if (j == h && i > h && i < nr) a = v;                       // reflector below the diagonal
     }
-    for (int j = 0; j < ncol; j++)
-        for (int i = 0; i < nr; i++) Qx[j * nr + i] = (i == j) ? 1.0 : 0.0;
-    for (int h = ncol - 1; h >= 0; h--) {
-        if (tau[h] == 0.0) continue;
-        v[h] = 1.0;
-        for (int i = h + 1; i < nr; i++) v[i] = A[h * CDFT_MAXM + i];
-        for (int j = 0; j < ncol; j++) {
-            double w = 0.0;
-            for (int i = h; i < nr; i++) w += v[i] * Qx[j * nr + i];
-            w *= tau[h];
-            for (int i = h; i < nr; i++) Qx[j * nr + i] -= w * v[i];
-        }
+    // Qx = H_0 H_1 ... H_{ncol-1} I[:, 0:ncol]
+    double q = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+    for (int h = CDFT_MAXM - 1; h >= 0; h--) {
+        if (h >= ncol || tau[h] == 0.0) continue;
+        const double vh = __shfl(a, h * 8 + i, 64);
+        const double v = (i == h) ? 1.0 : ((i > h && i < nr) ? vh : 0.0);
+        const double w = col8_sum(v * q) * tau[h];
+        q -= w * v;
     }
-    for (int j = 0; j < ncol; j++)
-        for (int i = 0; i < ncol; i++) R[j * m + i] = (i <= j) ? A[j * CDFT_MAXM + i] : 0.0;
-    for (int i = 0; i < m; i++) R[(m - 1) * m + i] = 0.0;
+    __syncthreads();                                                // every wave has read R
+    if (threadIdx.x < 64) {
+        if (i < nr && j < ncol) Qx[j * nr + i] = q;
+        if (i < ncol && j < ncol) R[j * m + i] = (i <= j) ? a : 0.0;
+        if (j == 0 && i < m) R[(m - 1) * m + i] = 0.0;
+    }
+}
+
+// theta = R[0:mk, 0:mk] \ tres by column-oriented back substitution; lane l of every
+// wave ends holding theta_l (l < mk).  Called by all threads; R, tres in LDS.
+CDFT_INLINE double dev_triangular_solve(const double *R, const double *tres, int m, int mk)
+{
+    const int lane = threadIdx.x & 63;
+    double x = (lane < mk) ? tres[lane] : 0.0;
+    double th = 0.0;
+#pragma unroll
+    for (int j = CDFT_MAXM - 1; j >= 0; j--) {
+        if (j >= mk) continue;
+        const double tj = __shfl(x, j, 64) / R[j * m + j];
+        if (lane == j) th = tj;
+        if (lane < j) x -= R[j * m + lane] * tj;
+    }
+    return th;
 }
 
 // Shared scalar state of one system's solve (LDS).
@@ -388,13 +434,15 @@ struct AState {
     int ring[CDFT_MAXM];
 };
 
-// Dynamic LDS bytes of one system (FP64 layout; the FP32 map uses the same slots).
-__host__ __device__ inline size_t batched_lds_layout(int N, int M, int nc)
+// Dynamic LDS bytes of one system (FP64 layout; the FP32 map uses the same slots):
+// twiddles, three work arrays and the accumulator (M complex each), the spectra when
+// they live in LDS, the fields and the reduction slots.
+__host__ __device__ inline size_t batched_lds_layout(int N, int M, int nc, bool g_lds)
 {
-    return (size_t)3 * M * 16 + (size_t)7 * N * nc * 8 + 64 * 8;
+    return (size_t)5 * M * 16 + (g_lds ? (size_t)6 * nc * ((M >> 1) + 1) * 8 : 0) + (size_t)7 * N * nc * 8 + 64 * 8;
 }
 
-template <int NC, int MAXB>
+template <int NC>
 __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems, BOpts o, BResult *results)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -415,8 +463,6 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
 #pragma unroll
     for (int c = 0; c < NC; c++) S.cp[c] = D->cp[c];
     S.vext = D->vext;
-    S.H64 = D->H64;
-    S.H32 = D->H32;
     S.rho = D->rho;
     S.sol = D->sol;
     S.gx = D->gx;
@@ -432,15 +478,22 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
         for (int i = tid; i < (int)(sizeof(PhiConst) / sizeof(double)); i += BT) dst[i] = src[i];
     }
     const int N = S.N, M = S.M, n = S.n;
+    const int ng = (F - 1) * NC * ((M >> 1) + 1);
     c2<double> *tw = reinterpret_cast<c2<double> *>(smem);
-    c2<double> *A64 = tw + M;
-    c2<double> *P64 = A64 + M;
-    double *fld64 = reinterpret_cast<double *>(P64 + M);
+    c2<double> *W64 = tw + M;
+    c2<double> *P64 = W64 + 3 * M;
+    double *Gl = reinterpret_cast<double *>(P64 + M);
+    double *fld64 = Gl + (o.g_lds ? ng : 0);
     double *red = fld64 + (size_t)F * N * NC;
-    c2<float> *A32 = reinterpret_cast<c2<float> *>(A64);
+    c2<float> *W32 = reinterpret_cast<c2<float> *>(W64);
     c2<float> *P32 = reinterpret_cast<c2<float> *>(P64);
     float *fld32 = reinterpret_cast<float *>(fld64);
     for (int t = tid; t < M; t += BT) tw[t] = {twg[2 * t], twg[2 * t + 1]};
+    const double *G = D->G;
+    if (o.g_lds) {
+        for (int t = tid; t < ng; t += BT) Gl[t] = G[t];
+        G = Gl;
+    }
 
     const int m = o.m;
     const double beta = o.beta;
@@ -506,7 +559,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                     qcol = 0;
                 } else {
                     const int down = (kq > m - 1);
-                    if (down && tid == 0) dev_householder_downdate(st.R, m, st.Qx);
+                    if (down) dev_householder_downdate(st.R, m, st.Qx);
                     __syncthreads();
                     const int kqq = kq < m - 1 ? kq : m - 1;
                     // pass 1: optional downdate of Q (pointwise, in place); h1 = Q^T df, t = Q^T res
@@ -584,19 +637,14 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                     qcol = kqq;
                 }
                 const int mk = m < ka ? m : ka;
-                if (tid == 0) {
-                    for (int j = mk - 1; j >= 0; j--) {             // theta = RA \ tres
-                        double acc = st.tres[j];
-                        for (int l = j + 1; l < mk; l++) acc -= st.R[l * m + j] * st.theta[l];
-                        st.theta[j] = acc / st.R[j * m + j];
-                    }
-                }
-                __syncthreads();
+                __syncthreads();                                    // R, tres written by thread 0
+                const double thl = dev_triangular_solve(st.R, st.tres, m, mk);
                 double th[CDFT_MAXM];
                 int rg[CDFT_MAXM];
 #pragma unroll
                 for (int j = 0; j < CDFT_MAXM; j++) {
-                    th[j] = (j < mk) ? st.theta[j] : 0.0;
+                    th[j] = __shfl(thl, j, 64);
+                    if (j >= mk) th[j] = 0.0;
                     rg[j] = (j < mk) ? st.ring[j] : 0;
                 }
                 // new Q column; sol = gx - DG theta
@@ -617,10 +665,10 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
 
         // ---------------- the one map evaluation
         if (fp32) {
-            evalG<float, NC, MAXB>(S, pcs, tw, A32, P32, fld32, red, mode, bb, fo);
+            evalG<float, NC>(S, pcs, tw, G, W32, P32, fld32, red, mode, bb, fo);
             nevals32++;
         } else {
-            evalG<double, NC, MAXB>(S, pcs, tw, A64, P64, fld64, red, mode, bb, fo);
+            evalG<double, NC>(S, pcs, tw, G, W64, P64, fld64, red, mode, bb, fo);
         }
         nevals++;
         resnorm = sqrt(fo[0]);
@@ -706,39 +754,49 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
 }
 
 // ---------------------------------------------------------------- set-up kernels
-// H[(c*(F-1)+f-1)*M + k] = sum_{j<N} h_f[j] exp(-2 pi i j k / M)   (FP64, exact twiddles)
-__global__ void k_setup_spec(const double *h, int N, int M, int F, double *H64, float *H32)
+// Real half-spectra of the kernels on the FFT ring (k <= M/2), FP64, exact twiddles:
+//   M == N:  H(k) = sum_{t<N} h[t] e^{-2 pi i t k / N}
+//   M > N:   H(k) = h[0] + sum_{s=1}^{N-1} (h[s] e^{-2 pi i s k / M} + h[N-s] e^{+2 pi i s k / M})
+// G = Re H for the even kernels and Im H for the odd one (slot 3); the other part
+// vanishes by symmetry and is dropped.
+__global__ void k_setup_spec(const double *h, int N, int M, int F, double *G)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int cf = blockIdx.y;                     // c*(F-1) + f-1
-    if (k >= M) return;
-    int c = cf / (F - 1), f = 1 + cf % (F - 1);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cf = blockIdx.y;                       // c*(F-1) + f-1
+    const int G1 = (M >> 1) + 1;
+    if (k >= G1) return;
+    const int c = cf / (F - 1), f = 1 + cf % (F - 1);
     const double *hk = h + (size_t)(c * F + f) * N;
     double re = 0.0, im = 0.0;
-    for (int j = 0; j < N; j++) {
-        long r = ((long)j * k) % M;
-        double sn, cs;
-        sincospi(2.0 * (double)r / (double)M, &sn, &cs);
-        re += hk[j] * cs;
-        im -= hk[j] * sn;
+    if (M == N) {
+        for (int t = 0; t < N; t++) {
+            const long r = ((long)t * k) % M;
+            double sn, cs;
+            sincospi(2.0 * (double)r / (double)M, &sn, &cs);
+            re += hk[t] * cs;
+            im -= hk[t] * sn;
+        }
+    } else {
+        re = hk[0];
+        for (int t = 1; t < N; t++) {
+            const long r = ((long)t * k) % M;
+            double sn, cs;
+            sincospi(2.0 * (double)r / (double)M, &sn, &cs);
+            re += (hk[t] + hk[N - t]) * cs;
+            im -= (hk[t] - hk[N - t]) * sn;
+        }
     }
-    size_t o = ((size_t)cf * M + k) * 2;
-    H64[o] = re;
-    H64[o + 1] = im;
-    H32[o] = (float)re;
-    H32[o + 1] = (float)im;
+    G[(size_t)cf * G1 + k] = (f == 3) ? im : re;
 }
 
-__global__ void k_setup_tw(int M, double *tw64, float *tw32)
+__global__ void k_setup_tw(int M, double *tw64)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= M) return;
     double sn, cs;
     sincospi(2.0 * (double)t / (double)M, &sn, &cs);
     tw64[2 * t] = cs;
     tw64[2 * t + 1] = -sn;
-    tw32[2 * t] = (float)cs;
-    tw32[2 * t + 1] = (float)-sn;
 }
 
 }  // namespace cdft

@@ -19,7 +19,7 @@ struct cdft_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     double last_kernel_ms = 0.0;
-    std::map<int, std::pair<double *, float *>> twiddles;   // FFT length M -> W_M^t tables
+    std::map<int, double *> twiddles;   // FFT ring length M -> W_M^t table (re, im pairs)
 };
 
 // Per-system state.  Host copies of the descriptor plus device buffers.
@@ -49,10 +49,9 @@ struct cdft_sys {
     // Anderson storage (anderson.jl:665-703)
     double *sol = nullptr, *gx = nullptr, *df = nullptr, *dg = nullptr, *res = nullptr;
     double *Q = nullptr, *Qalt = nullptr, *DG = nullptr;   // m columns each
-    // batched path: kernel spectra at the padded FFT length M
+    // batched path: real half-spectra of the kernels on the FFT ring of length M
     int M = 0;
     double *H64 = nullptr;
-    float *H32s = nullptr;
     double *red = nullptr;     // block partials
     double *fin = nullptr;     // finalized reductions [CDFT_NQ]
     int nblk = 0;
