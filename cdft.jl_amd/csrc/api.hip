@@ -624,11 +624,14 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         return CDFT_OK;
     };
     int rc;
+    // Anderson state in registers when every system fits the register layout
+    bool reg = (nc == 1 && o->m <= kRegAM);
+    for (int i = 0; i < nsys && reg; i++) reg = sys[i]->N <= kRegNPT * BT;
     switch (nc) {
-    case 1: rc = launch(k_batched<1>); break;
-    case 2: rc = launch(k_batched<2>); break;
-    case 3: rc = launch(k_batched<3>); break;
-    default: rc = launch(k_batched<4>); break;
+    case 1: rc = reg ? launch(k_batched<1, true>) : launch(k_batched<1, false>); break;
+    case 2: rc = launch(k_batched<2, false>); break;
+    case 3: rc = launch(k_batched<3, false>); break;
+    default: rc = launch(k_batched<4, false>); break;
     }
     if (rc != CDFT_OK) return rc;
     std::vector<BResult> hr(nsys);

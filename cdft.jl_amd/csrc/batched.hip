@@ -201,14 +201,78 @@ struct SysRegs {
     double cp[NC];
     const double *__restrict__ vext;
     double *__restrict__ rho;
-    double *__restrict__ sol;
-    double *__restrict__ gx;
-    double *__restrict__ df;
-    double *__restrict__ dg;
-    double *__restrict__ res;
-    double *__restrict__ Q;
-    double *__restrict__ DG;
 };
+
+// Per-point Anderson state of one system (anderson.jl:665-703: sol, gx, res, df,
+// dg and the Q / Delta-G history columns).  REG: in registers of the thread that
+// owns the points i = tid + b BT (b < NPT, one component, at most AM history
+// columns), so an Anderson step touches no memory; otherwise in HBM ([i] and
+// [col n + i]).  Column indices that are not compile-time constants after
+// unrolling go through the set/get helpers, which select among the registers.
+constexpr int kRegNPT = 2;     // points per thread of the register-resident state
+constexpr int kRegAM = 5;      // history columns it holds (Anderson m <= 5)
+
+template <bool REG>
+struct AStore {
+    static constexpr int NPT = REG ? kRegNPT : 1, AM = REG ? kRegAM : 1;
+    double sol_[NPT], gx_[NPT], res_[NPT], df_[NPT], dg_[NPT], Q_[NPT][AM], DG_[NPT][AM];
+    double *gsol, *ggx, *gres, *gdf, *gdg, *gQ, *gDG;
+    long n;
+
+    CDFT_INLINE double &sol(int b, int i) { if constexpr (REG) return sol_[b]; else return gsol[i]; }
+    CDFT_INLINE double &gx(int b, int i) { if constexpr (REG) return gx_[b]; else return ggx[i]; }
+    CDFT_INLINE double &res(int b, int i) { if constexpr (REG) return res_[b]; else return gres[i]; }
+    CDFT_INLINE double &df(int b, int i) { if constexpr (REG) return df_[b]; else return gdf[i]; }
+    CDFT_INLINE double &dg(int b, int i) { if constexpr (REG) return dg_[b]; else return gdg[i]; }
+    // column j known at compile time after unrolling (j < AM for REG)
+    CDFT_INLINE double &Q(int b, int i, int j) { if constexpr (REG) return Q_[b][j]; else return gQ[j * n + i]; }
+    CDFT_INLINE double getDG(int b, int i, int col)
+    {
+        if constexpr (REG) {
+            double v = 0.0;
+#pragma unroll
+            for (int s = 0; s < AM; s++) v = (s == col) ? DG_[b][s] : v;
+            return v;
+        } else {
+            return gDG[col * n + i];
+        }
+    }
+    CDFT_INLINE void setDG(int b, int i, int col, double v)
+    {
+        if constexpr (REG) {
+#pragma unroll
+            for (int s = 0; s < AM; s++)
+                if (s == col) DG_[b][s] = v;
+        } else {
+            gDG[col * n + i] = v;
+        }
+    }
+    CDFT_INLINE void setQ(int b, int i, int col, double v)
+    {
+        if constexpr (REG) {
+#pragma unroll
+            for (int s = 0; s < AM; s++)
+                if (s == col) Q_[b][s] = v;
+        } else {
+            gQ[col * n + i] = v;
+        }
+    }
+};
+
+// Calls fn(b, i) for the points i < n of this thread: i = tid + b BT.
+template <bool REG, typename Fn>
+CDFT_INLINE void for_points(int n, Fn &&fn)
+{
+    if constexpr (REG) {
+#pragma unroll
+        for (int b = 0; b < kRegNPT; b++) {
+            const int i = (int)threadIdx.x + b * BT;
+            if (i < n) fn(b, i);
+        }
+    } else {
+        for (int i = threadIdx.x; i < n; i += BT) fn(0, i);
+    }
+}
 
 // Spectral multipliers of the three field pairs of one component at ring index k:
 // Z_q = H_{1+2q} + i H_{2+2q} with H_f = G_f (even kernels) or i G_f (the vector
@@ -231,9 +295,9 @@ CDFT_INLINE void pair_multipliers(const double *Gc, int G1, int M, int k, c2<T> 
 // LDS: tw (M, FP64), spectra G (6 NC (M/2+1) FP64, or global), three work arrays W
 // and the accumulator P (M complex T each, at FP64 stride), fields fld (F*N*NC of T
 // at FP64 stride), red (64 doubles).
-template <typename T, int NC>
-CDFT_INLINE void evalG(const SysRegs<NC> &S, const PhiConst &P, const c2<double> *tw, const double *G, c2<T> *W,
-                       c2<T> *Pc, T *fld, double *red, int mode, double b, double (&out)[3])
+template <typename T, int NC, bool REG>
+CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, const c2<double> *tw, const double *G,
+                       c2<T> *W, c2<T> *Pc, T *fld, double *red, int mode, double b, double (&out)[3])
 {
     constexpr int F = 7;
     const int tid = threadIdx.x;
@@ -244,13 +308,18 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, const PhiConst &P, const c2<double>
     for (int c = 0; c < NC; c++) {
         const double *Gc = G + (size_t)c * (F - 1) * G1;
         // rho = exp(sol): slot 0 of the fields and the zero-padded FFT input
-        for (int k = tid; k < M; k += BT) {
-            if (k < N) {
-                const double r = exp(S.sol[c * N + k]);
-                fld[(c * F) * N + k] = T(S.NA * r);
-                Pc[k] = {T(r), T(0)};
-            } else {
-                Pc[k] = {T(0), T(0)};
+#pragma unroll
+        for (int j = 0; j < 1024 / BT; j++) {
+            const int k = tid + j * BT;
+            if (k < M) {
+                if (k < N) {
+                    const int jb = j < AStore<REG>::NPT ? j : AStore<REG>::NPT - 1;
+                    const double r = exp(A.sol(jb, c * N + k));
+                    fld[(c * F) * N + k] = T(S.NA * r);
+                    Pc[k] = {T(r), T(0)};
+                } else {
+                    Pc[k] = {T(0), T(0)};
+                }
             }
         }
         __syncthreads();
@@ -314,27 +383,31 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, const PhiConst &P, const c2<double>
         }
         __syncthreads();
         fft_lds<T, 1>(Pc, M, tw, M, lg, true);
-        for (int i = tid; i < N; i += BT) {
+#pragma unroll
+        for (int j = 0; j < 1024 / BT; j++) {
+            const int i = tid + j * BT;
+            if (i >= N) continue;
+            const int jb = j < AStore<REG>::NPT ? j : AStore<REG>::NPT - 1;
             double s = double(fld[(c * F) * N + i]) + double(Pc[i].x * invM);
             const int t = c * N + i;
             if (S.vext) s += S.vext[t];
             double lgv = S.cp[c] - s;
             lgv = fmin(fmax(lgv, -100.0), 100.0);          // clamp!(ln_Gx, -100, 100)
-            const double x = S.sol[t];
+            const double x = A.sol(jb, t);
             const double gnew = (b == 1.0) ? lgv : (1.0 - b) * x + b * lgv;
             if (mode == 1) {
-                S.gx[t] = gnew;
+                A.gx(jb, t) = gnew;
                 const double r = gnew - x;
-                S.res[t] = r;
+                A.res(jb, t) = r;
                 v0 += r * r;
             } else {
-                const double gold = S.gx[t], rold = S.res[t];
-                S.dg[t] = -gold + gnew;
-                S.gx[t] = gnew;
+                const double gold = A.gx(jb, t), rold = A.res(jb, t);
+                A.dg(jb, t) = -gold + gnew;
+                A.gx(jb, t) = gnew;
                 const double r = gnew - x;
-                S.res[t] = r;
+                A.res(jb, t) = r;
                 const double d = -rold + r;
-                S.df[t] = d;
+                A.df(jb, t) = d;
                 v0 += r * r;
                 v1 += d * d;
                 v2 += d * r;
@@ -442,13 +515,14 @@ __host__ __device__ inline size_t batched_lds_layout(int N, int M, int nc, bool 
     return (size_t)5 * M * 16 + (g_lds ? (size_t)6 * nc * ((M >> 1) + 1) * 8 : 0) + (size_t)7 * N * nc * 8 + 64 * 8;
 }
 
-template <int NC>
+template <int NC, bool REG>
 __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems, BOpts o, BResult *results)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ AState st;
     __shared__ PhiConst pcs;
     constexpr int F = 7;
+    constexpr int AM = REG ? kRegAM : CDFT_MAXM;     // history columns addressed
     const int tid = threadIdx.x;
     const unsigned long long t_start = wall_clock64();
 
@@ -464,13 +538,15 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
     for (int c = 0; c < NC; c++) S.cp[c] = D->cp[c];
     S.vext = D->vext;
     S.rho = D->rho;
-    S.sol = D->sol;
-    S.gx = D->gx;
-    S.df = D->df;
-    S.dg = D->dg;
-    S.res = D->res;
-    S.Q = D->Q;
-    S.DG = D->DG;
+    AStore<REG> A;
+    A.gsol = D->sol;
+    A.ggx = D->gx;
+    A.gdf = D->df;
+    A.gdg = D->dg;
+    A.gres = D->res;
+    A.gQ = D->Q;
+    A.gDG = D->DG;
+    A.n = (long)S.N * NC;
     const double *__restrict__ twg = D->tw64;
     {
         const double *src = reinterpret_cast<const double *>(&D->pc);
@@ -504,7 +580,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
     int nevals = 0, nevals32 = 0;
     double fo[3];
 
-    for (int i = tid; i < n; i += BT) S.sol[i] = log(S.rho[i]);   // x0 = log(rho)
+    for_points<REG>(n, [&](int b, int i) { A.sol(b, i) = log(S.rho[i]); });   // x0 = log(rho)
     __syncthreads();
 
     // aasol (anderson.jl:301-443) as a phase machine with a single map-evaluation
@@ -526,7 +602,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                 for (int j = 0; j < CDFT_MAXM * CDFT_MAXM; j++) st.R[j] = 0.0;
                 for (int j = 0; j < CDFT_MAXM; j++) st.ring[j] = j;
             }
-            for (int i = tid; i < n; i += BT) S.sol[i] = S.gx[i];   // copy!(sol, gx)
+            for_points<REG>(n, [&](int b, int i) { A.sol(b, i) = A.gx(b, i); });   // copy!(sol, gx)
             __syncthreads();
             k += 1;
             ka = 1;
@@ -535,7 +611,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
             fp32 = false;                                           // FP32 noise floor: FP64 map from here
         } else if (phase == PH_STEP) {
             if (m == 0) {
-                for (int i = tid; i < n; i += BT) S.sol[i] = S.gx[i];
+                for_points<REG>(n, [&](int b, int i) { A.sol(b, i) = A.gx(b, i); });
                 __syncthreads();
             } else {
                 const int kk = ka + 1;
@@ -547,7 +623,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                 }
                 __syncthreads();
                 const int slot = (m == 1) ? st.ring[0] : (kk > m + 1 ? st.ring[m - 1] : st.ring[kk - 2]);
-                for (int i = tid; i < n; i += BT) S.DG[(size_t)slot * n + i] = S.dg[i];
+                for_points<REG>(n, [&](int b, int i) { A.setDG(b, i, slot, A.dg(b, i)); });
                 double nrm;
                 int qcol;
                 if (kq == 0) {                                      // aa_qr_update! first column
@@ -566,66 +642,66 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                     double v1[2 * CDFT_MAXM];
 #pragma unroll
                     for (int j = 0; j < 2 * CDFT_MAXM; j++) v1[j] = 0.0;
-                    for (int i = tid; i < n; i += BT) {
-                        double qv[CDFT_MAXM];
+                    for_points<REG>(n, [&](int b, int i) {
+                        double qv[AM];
                         if (down) {
-                            double qo[CDFT_MAXM];
+                            double qo[AM];
 #pragma unroll
-                            for (int l = 0; l < CDFT_MAXM; l++) qo[l] = (l < m) ? S.Q[(size_t)l * n + i] : 0.0;
+                            for (int l = 0; l < AM; l++) qo[l] = (l < m) ? A.Q(b, i, l) : 0.0;
 #pragma unroll
-                            for (int j = 0; j < CDFT_MAXM - 1; j++) {
+                            for (int j = 0; j < AM - 1; j++) {
                                 double a = 0.0;
 #pragma unroll
-                                for (int l = 0; l < CDFT_MAXM; l++)
+                                for (int l = 0; l < AM; l++)
                                     if (l < m) a += qo[l] * st.Qx[j * m + l];
                                 qv[j] = a;
-                                if (j < m - 1) S.Q[(size_t)j * n + i] = a;
+                                if (j < m - 1) A.Q(b, i, j) = a;
                             }
-                            S.Q[(size_t)(m - 1) * n + i] = 0.0;
+                            A.setQ(b, i, m - 1, 0.0);
                         } else {
 #pragma unroll
-                            for (int j = 0; j < CDFT_MAXM - 1; j++) qv[j] = (j < kqq) ? S.Q[(size_t)j * n + i] : 0.0;
+                            for (int j = 0; j < AM - 1; j++) qv[j] = (j < kqq) ? A.Q(b, i, j) : 0.0;
                         }
-                        const double dv = S.df[i], rv = S.res[i];
+                        const double dv = A.df(b, i), rv = A.res(b, i);
 #pragma unroll
-                        for (int j = 0; j < CDFT_MAXM - 1; j++)
+                        for (int j = 0; j < AM - 1; j++)
                             if (j < kqq) {
                                 v1[j] += qv[j] * dv;
                                 v1[CDFT_MAXM + j] += qv[j] * rv;
                             }
-                    }
+                    });
                     wg_sum<2 * CDFT_MAXM>(v1, red);
                     // pass 2: df -= Q h1; h2 = Q^T df
                     double v2[CDFT_MAXM];
 #pragma unroll
                     for (int j = 0; j < CDFT_MAXM; j++) v2[j] = 0.0;
-                    for (int i = tid; i < n; i += BT) {
-                        double qv[CDFT_MAXM], a = 0.0;
+                    for_points<REG>(n, [&](int b, int i) {
+                        double qv[AM], a = 0.0;
 #pragma unroll
-                        for (int j = 0; j < CDFT_MAXM - 1; j++)
+                        for (int j = 0; j < AM - 1; j++)
                             if (j < kqq) {
-                                qv[j] = S.Q[(size_t)j * n + i];
+                                qv[j] = A.Q(b, i, j);
                                 a += qv[j] * v1[j];
                             }
-                        const double d = S.df[i] - a;
-                        S.df[i] = d;
+                        const double d = A.df(b, i) - a;
+                        A.df(b, i) = d;
 #pragma unroll
-                        for (int j = 0; j < CDFT_MAXM - 1; j++)
+                        for (int j = 0; j < AM - 1; j++)
                             if (j < kqq) v2[j] += qv[j] * d;
-                    }
+                    });
                     wg_sum<CDFT_MAXM>(v2, red);
                     // pass 3: df -= Q h2; |df|^2, df.res
                     double v3[2] = {0.0, 0.0};
-                    for (int i = tid; i < n; i += BT) {
+                    for_points<REG>(n, [&](int b, int i) {
                         double a = 0.0;
 #pragma unroll
-                        for (int j = 0; j < CDFT_MAXM - 1; j++)
-                            if (j < kqq) a += S.Q[(size_t)j * n + i] * v2[j];
-                        const double d = S.df[i] - a;
-                        S.df[i] = d;
+                        for (int j = 0; j < AM - 1; j++)
+                            if (j < kqq) a += A.Q(b, i, j) * v2[j];
+                        const double d = A.df(b, i) - a;
+                        A.df(b, i) = d;
                         v3[0] += d * d;
-                        v3[1] += d * S.res[i];
-                    }
+                        v3[1] += d * A.res(b, i);
+                    });
                     wg_sum<2>(v3, red);
                     nrm = sqrt(v3[0]);
                     if (tid == 0) {
@@ -639,23 +715,35 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                 const int mk = m < ka ? m : ka;
                 __syncthreads();                                    // R, tres written by thread 0
                 const double thl = dev_triangular_solve(st.R, st.tres, m, mk);
-                double th[CDFT_MAXM];
-                int rg[CDFT_MAXM];
+                // coefficient of each physical Delta-G column: theta of its ring position
+                double cf[AM];
+#pragma unroll
+                for (int s2 = 0; s2 < AM; s2++) cf[s2] = 0.0;
 #pragma unroll
                 for (int j = 0; j < CDFT_MAXM; j++) {
-                    th[j] = __shfl(thl, j, 64);
-                    if (j >= mk) th[j] = 0.0;
-                    rg[j] = (j < mk) ? st.ring[j] : 0;
+                    const double thj = __shfl(thl, j, 64);
+                    if (j < mk) {
+                        const int col = st.ring[j];
+#pragma unroll
+                        for (int s2 = 0; s2 < AM; s2++)
+                            if (s2 == col) cf[s2] = thj;
+                    }
                 }
                 // new Q column; sol = gx - DG theta
-                for (int i = tid; i < n; i += BT) {
-                    S.Q[(size_t)qcol * n + i] = S.df[i] / nrm;
+                for_points<REG>(n, [&](int b, int i) {
+                    const double dv = A.df(b, i);
+                    A.setQ(b, i, qcol, dv / nrm);
                     double a = 0.0;
+                    if constexpr (REG) {
 #pragma unroll
-                    for (int j = 0; j < CDFT_MAXM; j++)
-                        if (j < mk) a += S.DG[(size_t)rg[j] * n + i] * th[j];
-                    S.sol[i] = S.gx[i] - a;
-                }
+                        for (int s2 = 0; s2 < AM; s2++) a += A.DG_[b][s2] * cf[s2];
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < CDFT_MAXM; j++)
+                            if (j < mk) a += A.getDG(b, i, st.ring[j]) * __shfl(thl, j, 64);
+                    }
+                    A.sol(b, i) = A.gx(b, i) - a;
+                });
                 __syncthreads();
             }
             k += 1;
@@ -665,10 +753,10 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
 
         // ---------------- the one map evaluation
         if (fp32) {
-            evalG<float, NC>(S, pcs, tw, G, W32, P32, fld32, red, mode, bb, fo);
+            evalG<float, NC, REG>(S, A, pcs, tw, G, W32, P32, fld32, red, mode, bb, fo);
             nevals32++;
         } else {
-            evalG<double, NC>(S, pcs, tw, G, W64, P64, fld64, red, mode, bb, fo);
+            evalG<double, NC, REG>(S, A, pcs, tw, G, W64, P64, fld64, red, mode, bb, fo);
         }
         nevals++;
         resnorm = sqrt(fo[0]);
@@ -683,7 +771,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
             if (toosoon) {
                 phase = PH_DONE;
             } else if (picard_maxit > 0) {
-                for (int i = tid; i < n; i += BT) S.sol[i] = S.gx[i];
+                for_points<REG>(n, [&](int b, int i) { A.sol(b, i) = A.gx(b, i); });
                 __syncthreads();
                 pit = 0;
                 phase = (picard_maxit - 1 > 0) ? PH_PICARD : PH_RESTART;
@@ -694,7 +782,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
             pit++;
             const bool brk = (resnorm <= picard_tol || resnorm >= up_bd);
             if (!brk) {
-                for (int i = tid; i < n; i += BT) S.sol[i] = S.gx[i];
+                for_points<REG>(n, [&](int b, int i) { A.sol(b, i) = A.gx(b, i); });
                 __syncthreads();
             }
             if (brk || pit >= picard_maxit - 1) {
@@ -737,7 +825,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
     else if (resnorm >= up_bd) { idid = 0; errcode = -2; }
     else if (resnorm <= tol && !fp32) { idid = 1; errcode = 0; }
     else { idid = 0; errcode = 10; }
-    for (int i = tid; i < n; i += BT) S.rho[i] = exp(S.sol[i]);
+    for_points<REG>(n, [&](int b, int i) { S.rho[i] = exp(A.sol(b, i)); });
     if (tid == 0) {
         BResult r;
         r.idid = idid;
