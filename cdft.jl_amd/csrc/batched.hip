@@ -171,18 +171,48 @@ CDFT_INLINE void fft_lds(c2<T> *a, int astride, const c2<double> *tw, int M, int
     }
 }
 
+// Cross-lane moves on FP64 values, as two 32-bit halves: DPP within a row of 16
+// lanes (quad_perm xor 1 = 0xB1, xor 2 = 0x4E, row_half_mirror = 0x141,
+// row_mirror = 0x140) and reads of one lane (uniform lane index).
+template <int CTRL>
+CDFT_INLINE double dpp_f64(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+CDFT_INLINE double readlane_f64(double x, int lane)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 8 lanes of a half-row; pairwise commutative steps, so every lane
+// of the group ends with the same bits.
+CDFT_INLINE double sum8(double x)
+{
+    x += dpp_f64<0xB1>(x);
+    x += dpp_f64<0x4E>(x);
+    x += dpp_f64<0x141>(x);
+    return x;
+}
+
 // Sum of NQ values over the workgroup; every thread receives the same totals
-// (fixed summation order, so the result is reproducible and uniform).
+// (fixed summation order, so the result is reproducible and uniform): rows of 16
+// lanes by DPP, the four rows of a wave by lane reads, the waves through LDS.
 template <int NQ>
 CDFT_INLINE void wg_sum(double (&v)[NQ], double *red)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-        double x = v[q];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-        v[q] = x;
+        double x = sum8(v[q]);
+        x += dpp_f64<0x140>(x);
+        v[q] = (readlane_f64(x, 0) + readlane_f64(x, 16)) + (readlane_f64(x, 32) + readlane_f64(x, 48));
     }
     if (lane == 0)
 #pragma unroll
@@ -241,8 +271,7 @@ struct AStore {
     {
         if constexpr (REG) {
 #pragma unroll
-            for (int s = 0; s < AM; s++)
-                if (s == col) DG_[b][s] = v;
+            for (int s = 0; s < AM; s++) DG_[b][s] = (s == col) ? v : DG_[b][s];   // no dynamic register index
         } else {
             gDG[col * n + i] = v;
         }
@@ -251,8 +280,7 @@ struct AStore {
     {
         if constexpr (REG) {
 #pragma unroll
-            for (int s = 0; s < AM; s++)
-                if (s == col) Q_[b][s] = v;
+            for (int s = 0; s < AM; s++) Q_[b][s] = (s == col) ? v : Q_[b][s];
         } else {
             gQ[col * n + i] = v;
         }
@@ -422,15 +450,6 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
     out[2] = v[2];
 }
 
-// Sum over the 8 lanes of one column group (lanes 8j .. 8j+7) of a wave.
-CDFT_INLINE double col8_sum(double x)
-{
-    x += __shfl_xor(x, 1, 64);
-    x += __shfl_xor(x, 2, 64);
-    x += __shfl_xor(x, 4, 64);
-    return x;
-}
-
 // Downdate of the Anderson QR when the oldest column leaves (anderson.jl, qr! of
 // R[:, 2:m]): Householder QR of the m x (m-1) matrix R[:, 1:m] with LAPACK geqrf
 // conventions (beta = -sign(alpha) |x|, tau = (beta - alpha) / beta, reflector
@@ -450,15 +469,15 @@ CDFT_INLINE void dev_householder_downdate(double *R, int m, double *Qx)
         tau[h] = 0.0;
         if (h >= ncol) continue;
         const double x = __shfl(a, h * 8 + i, 64);                 // A[i][h]
-        const double alpha = __shfl(a, h * 8 + h, 64);             // A[h][h]
-        const double xn = sqrt(col8_sum((i > h && i < nr) ? x * x : 0.0));
+        const double alpha = readlane_f64(a, h * 9);               // A[h][h]
+        const double xn = sqrt(sum8((i > h && i < nr) ? x * x : 0.0));
         if (xn == 0.0) continue;                                    // H_h = I
         const double beta = -copysign(hypot(alpha, xn), alpha);
         const double th = (beta - alpha) / beta;
         tau[h] = th;
         const double sc = 1.0 / (alpha - beta);
         const double v = (i == h) ? 1.0 : ((i > h && i < nr) ? x * sc : 0.0);
-        const double w = col8_sum(v * a) * th;                      // (v^T A[:, j]) tau
+        const double w = sum8(v * a) * th;                      // (v^T A[:, j]) tau
         if (j > h && j < ncol) a -= w * v;
         if (j == h && i == h) a = beta;
         if (j == h && i > h && i < nr) a = v;                       // reflector below the diagonal
@@ -470,7 +489,7 @@ CDFT_INLINE void dev_householder_downdate(double *R, int m, double *Qx)
         if (h >= ncol || tau[h] == 0.0) continue;
         const double vh = __shfl(a, h * 8 + i, 64);
         const double v = (i == h) ? 1.0 : ((i > h && i < nr) ? vh : 0.0);
-        const double w = col8_sum(v * q) * tau[h];
+        const double w = sum8(v * q) * tau[h];
         q -= w * v;
     }
     __syncthreads();                                                // every wave has read R
@@ -491,7 +510,7 @@ CDFT_INLINE double dev_triangular_solve(const double *R, const double *tres, int
 #pragma unroll
     for (int j = CDFT_MAXM - 1; j >= 0; j--) {
         if (j >= mk) continue;
-        const double tj = __shfl(x, j, 64) / R[j * m + j];
+        const double tj = readlane_f64(x, j) / R[j * m + j];
         if (lane == j) th = tj;
         if (lane < j) x -= R[j * m + lane] * tj;
     }
@@ -580,7 +599,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
     int nevals = 0, nevals32 = 0;
     double fo[3];
 
-    for_points<REG>(n, [&](int b, int i) { A.sol(b, i) = log(S.rho[i]); });   // x0 = log(rho)
+    for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = log(S.rho[i]); });   // x0 = log(rho)
     __syncthreads();
 
     // aasol (anderson.jl:301-443) as a phase machine with a single map-evaluation
@@ -602,7 +621,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                 for (int j = 0; j < CDFT_MAXM * CDFT_MAXM; j++) st.R[j] = 0.0;
                 for (int j = 0; j < CDFT_MAXM; j++) st.ring[j] = j;
             }
-            for_points<REG>(n, [&](int b, int i) { A.sol(b, i) = A.gx(b, i); });   // copy!(sol, gx)
+            for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });   // copy!(sol, gx)
             __syncthreads();
             k += 1;
             ka = 1;
@@ -611,7 +630,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
             fp32 = false;                                           // FP32 noise floor: FP64 map from here
         } else if (phase == PH_STEP) {
             if (m == 0) {
-                for_points<REG>(n, [&](int b, int i) { A.sol(b, i) = A.gx(b, i); });
+                for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });
                 __syncthreads();
             } else {
                 const int kk = ka + 1;
@@ -623,7 +642,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                 }
                 __syncthreads();
                 const int slot = (m == 1) ? st.ring[0] : (kk > m + 1 ? st.ring[m - 1] : st.ring[kk - 2]);
-                for_points<REG>(n, [&](int b, int i) { A.setDG(b, i, slot, A.dg(b, i)); });
+                for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.setDG(b, i, slot, A.dg(b, i)); });
                 double nrm;
                 int qcol;
                 if (kq == 0) {                                      // aa_qr_update! first column
@@ -639,10 +658,10 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                     __syncthreads();
                     const int kqq = kq < m - 1 ? kq : m - 1;
                     // pass 1: optional downdate of Q (pointwise, in place); h1 = Q^T df, t = Q^T res
-                    double v1[2 * CDFT_MAXM];
+                    double v1[2 * (AM - 1)];
 #pragma unroll
-                    for (int j = 0; j < 2 * CDFT_MAXM; j++) v1[j] = 0.0;
-                    for_points<REG>(n, [&](int b, int i) {
+                    for (int j = 0; j < 2 * (AM - 1); j++) v1[j] = 0.0;
+                    for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) {
                         double qv[AM];
                         if (down) {
                             double qo[AM];
@@ -667,15 +686,15 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                         for (int j = 0; j < AM - 1; j++)
                             if (j < kqq) {
                                 v1[j] += qv[j] * dv;
-                                v1[CDFT_MAXM + j] += qv[j] * rv;
+                                v1[(AM - 1) + j] += qv[j] * rv;
                             }
                     });
-                    wg_sum<2 * CDFT_MAXM>(v1, red);
+                    wg_sum<2 * (AM - 1)>(v1, red);
                     // pass 2: df -= Q h1; h2 = Q^T df
-                    double v2[CDFT_MAXM];
+                    double v2[AM - 1];
 #pragma unroll
-                    for (int j = 0; j < CDFT_MAXM; j++) v2[j] = 0.0;
-                    for_points<REG>(n, [&](int b, int i) {
+                    for (int j = 0; j < AM - 1; j++) v2[j] = 0.0;
+                    for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) {
                         double qv[AM], a = 0.0;
 #pragma unroll
                         for (int j = 0; j < AM - 1; j++)
@@ -689,10 +708,10 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                         for (int j = 0; j < AM - 1; j++)
                             if (j < kqq) v2[j] += qv[j] * d;
                     });
-                    wg_sum<CDFT_MAXM>(v2, red);
+                    wg_sum<AM - 1>(v2, red);
                     // pass 3: df -= Q h2; |df|^2, df.res
                     double v3[2] = {0.0, 0.0};
-                    for_points<REG>(n, [&](int b, int i) {
+                    for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) {
                         double a = 0.0;
 #pragma unroll
                         for (int j = 0; j < AM - 1; j++)
@@ -705,9 +724,13 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                     wg_sum<2>(v3, red);
                     nrm = sqrt(v3[0]);
                     if (tid == 0) {
-                        for (int j = 0; j < kqq; j++) st.R[kqq * m + j] = v1[j] + v2[j];
+#pragma unroll
+                        for (int j = 0; j < AM - 1; j++)
+                            if (j < kqq) {
+                                st.R[kqq * m + j] = v1[j] + v2[j];
+                                st.tres[j] = v1[(AM - 1) + j];
+                            }
                         st.R[kqq * m + kqq] = nrm;
-                        for (int j = 0; j < kqq; j++) st.tres[j] = v1[CDFT_MAXM + j];
                         st.tres[kqq] = v3[1] / nrm;
                     }
                     qcol = kqq;
@@ -716,21 +739,19 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                 __syncthreads();                                    // R, tres written by thread 0
                 const double thl = dev_triangular_solve(st.R, st.tres, m, mk);
                 // coefficient of each physical Delta-G column: theta of its ring position
-                double cf[AM];
+                double cf[AM], th[CDFT_MAXM];
 #pragma unroll
                 for (int s2 = 0; s2 < AM; s2++) cf[s2] = 0.0;
 #pragma unroll
                 for (int j = 0; j < CDFT_MAXM; j++) {
-                    const double thj = __shfl(thl, j, 64);
-                    if (j < mk) {
-                        const int col = st.ring[j];
+                    const double thj = readlane_f64(thl, j);
+                    th[j] = thj;
+                    const int col = (j < mk) ? st.ring[j] : -1;
 #pragma unroll
-                        for (int s2 = 0; s2 < AM; s2++)
-                            if (s2 == col) cf[s2] = thj;
-                    }
+                    for (int s2 = 0; s2 < AM; s2++) cf[s2] = (s2 == col) ? thj : cf[s2];
                 }
                 // new Q column; sol = gx - DG theta
-                for_points<REG>(n, [&](int b, int i) {
+                for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) {
                     const double dv = A.df(b, i);
                     A.setQ(b, i, qcol, dv / nrm);
                     double a = 0.0;
@@ -740,7 +761,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                     } else {
 #pragma unroll
                         for (int j = 0; j < CDFT_MAXM; j++)
-                            if (j < mk) a += A.getDG(b, i, st.ring[j]) * __shfl(thl, j, 64);
+                            if (j < mk) a += A.getDG(b, i, st.ring[j]) * th[j];
                     }
                     A.sol(b, i) = A.gx(b, i) - a;
                 });
@@ -771,7 +792,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
             if (toosoon) {
                 phase = PH_DONE;
             } else if (picard_maxit > 0) {
-                for_points<REG>(n, [&](int b, int i) { A.sol(b, i) = A.gx(b, i); });
+                for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });
                 __syncthreads();
                 pit = 0;
                 phase = (picard_maxit - 1 > 0) ? PH_PICARD : PH_RESTART;
@@ -782,7 +803,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
             pit++;
             const bool brk = (resnorm <= picard_tol || resnorm >= up_bd);
             if (!brk) {
-                for_points<REG>(n, [&](int b, int i) { A.sol(b, i) = A.gx(b, i); });
+                for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });
                 __syncthreads();
             }
             if (brk || pit >= picard_maxit - 1) {
@@ -825,7 +846,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
     else if (resnorm >= up_bd) { idid = 0; errcode = -2; }
     else if (resnorm <= tol && !fp32) { idid = 1; errcode = 0; }
     else { idid = 0; errcode = 10; }
-    for_points<REG>(n, [&](int b, int i) { S.rho[i] = exp(A.sol(b, i)); });
+    for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { S.rho[i] = exp(A.sol(b, i)); });
     if (tid == 0) {
         BResult r;
         r.idid = idid;
