@@ -505,6 +505,7 @@ static int batched_prepare(cdft_ctx *ctx, cdft_sys *s)
     hipStream_t st = ctx->stream;
     if (s->M == 0) {
         int M = fft_length(s->N);
+        if (M < 8) return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched path: grid too small (N < 5); use cdft_converge");
         if (M > 1024)
             return fail(ctx, CDFT_ERR_UNSUPPORTED,
                         "batched path: grid too large (FFT ring > 1024, i.e. N > 512 unless N is 1024); use cdft_converge");

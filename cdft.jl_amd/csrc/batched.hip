@@ -75,96 +75,112 @@ __device__ __attribute__((always_inline)) inline c2<T> cmul(c2<T> a, c2<T> b)
 
 #define CDFT_INLINE __device__ __attribute__((always_inline)) inline
 
-// In-place Stockham FFTs of NARR arrays of length M (power of two, 4 <= M <= 1024)
-// in LDS, array q at a + q * astride, all advancing through the same passes:
-// radix-4 passes and a final radix-2 pass when log2 M is odd.  Each pass loads its
-// butterfly inputs into registers, synchronises, then stores, so one buffer per
-// array suffices.  Twiddles W_M^t come from the LDS table tw.  Unnormalised; sign
-// -1 forward, +1 inverse.
-template <typename T, int NARR>
-CDFT_INLINE void fft_lds(c2<T> *a, int astride, const c2<double> *tw, int M, int log2M, bool inverse)
+// Radix-4 butterfly (unnormalised DFT of length 4 with sign sg) in registers.
+template <typename T>
+CDFT_INLINE void radix4(c2<T> (&v)[4], T sg)
 {
-    const int tid = threadIdx.x;
+    const c2<T> s02 = {v[0].x + v[2].x, v[0].y + v[2].y};
+    const c2<T> d02 = {v[0].x - v[2].x, v[0].y - v[2].y};
+    const c2<T> s13 = {v[1].x + v[3].x, v[1].y + v[3].y};
+    const c2<T> d13 = {v[1].x - v[3].x, v[1].y - v[3].y};
+    const c2<T> id13 = {-sg * d13.y, sg * d13.x};   // (sg i) * d13
+    v[0] = {s02.x + s13.x, s02.y + s13.y};
+    v[1] = {d02.x + id13.x, d02.y + id13.y};
+    v[2] = {s02.x - s13.x, s02.y - s13.y};
+    v[3] = {d02.x - id13.x, d02.y - id13.y};
+}
+
+// Stockham FFTs of NARR arrays of length M (power of two, 8 <= M <= 1024) that
+// advance together through radix-4 passes and a final radix-2 pass when log2 M is
+// odd.  Thread j handles butterfly j of every array (at most M/4 resp. M/2 <= 256
+// butterflies per pass), so the twiddles W^{rk} of a pass are one table read and
+// two products, shared by the arrays.  The first pass reads its inputs through
+// load0(idx, vals[NARR]) and the last pass hands its outputs to
+// storeL(idx, vals[NARR]); the passes between work in place on the LDS arrays
+// W + q M.  Unnormalised; sign -1 forward, +1 inverse.  Ends with a barrier.
+template <typename T, int NARR, class Load0, class StoreL>
+CDFT_INLINE void fft_fused(c2<T> *W, const c2<double> *tw, int M, int lg, bool inverse, Load0 &&load0,
+                           StoreL &&storeL)
+{
+    const int j = threadIdx.x;
     const T sg = inverse ? T(1) : T(-1);
-    const int nr4 = log2M >> 1;
-    const int nb = M >> 2, lnb = log2M - 2;        // radix-4 butterflies per array
+    const int nr4 = lg >> 1;
+    const bool odd = (lg & 1) != 0;
     int Ns = 1, lNs = 0;
     for (int pass = 0; pass < nr4; pass++) {
+        const bool first = (pass == 0), last = (!odd && pass == nr4 - 1);
+        const int nb = M >> 2;
+        const int k = j & (Ns - 1);
         c2<T> v[NARR][4];
-        const int lstep = log2M - 2 - lNs;          // twiddle stride M / (4 Ns)
-#pragma unroll
-        for (int b = 0; b < NARR; b++) {
-            const int g = tid + b * BT;
-            const int q = g >> lnb, j = g & (nb - 1);
-            if (q < NARR) {
-                const c2<T> *x = a + q * astride;
-                const int k = j & (Ns - 1);
+        if (j < nb) {
+            if (first) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    c2<T> y = x[j + r * nb];
-                    if (r) {
-                        const c2<double> wd = tw[(r * k) << lstep];
-                        y = cmul(y, c2<T>{T(wd.x), inverse ? T(-wd.y) : T(wd.y)});
-                    }
-                    v[b][r] = y;
+                    c2<T> in[NARR];
+                    load0(j + r * nb, in);
+#pragma unroll
+                    for (int q = 0; q < NARR; q++) v[q][r] = in[q];
                 }
-                const c2<T> s02 = {v[b][0].x + v[b][2].x, v[b][0].y + v[b][2].y};
-                const c2<T> d02 = {v[b][0].x - v[b][2].x, v[b][0].y - v[b][2].y};
-                const c2<T> s13 = {v[b][1].x + v[b][3].x, v[b][1].y + v[b][3].y};
-                const c2<T> d13 = {v[b][1].x - v[b][3].x, v[b][1].y - v[b][3].y};
-                const c2<T> id13 = {-sg * d13.y, sg * d13.x};   // (sg i) * d13
-                v[b][0] = {s02.x + s13.x, s02.y + s13.y};
-                v[b][1] = {d02.x + id13.x, d02.y + id13.y};
-                v[b][2] = {s02.x - s13.x, s02.y - s13.y};
-                v[b][3] = {d02.x - id13.x, d02.y - id13.y};
+            } else {
+#pragma unroll
+                for (int q = 0; q < NARR; q++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) v[q][r] = W[q * M + j + r * nb];
+                const c2<double> wd = tw[k << (lg - 2 - lNs)];
+                const c2<T> w1 = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
+                const c2<T> w2 = cmul(w1, w1), w3 = cmul(w1, w2);
+#pragma unroll
+                for (int q = 0; q < NARR; q++) {
+                    v[q][1] = cmul(v[q][1], w1);
+                    v[q][2] = cmul(v[q][2], w2);
+                    v[q][3] = cmul(v[q][3], w3);
+                }
             }
+#pragma unroll
+            for (int q = 0; q < NARR; q++) radix4<T>(v[q], sg);
         }
-        __syncthreads();
+        if (!first && !last) __syncthreads();     // in place: every read before any write
+        if (j < nb) {
+            const int d = ((j >> lNs) << (lNs + 2)) + k;
+            if (last) {
 #pragma unroll
-        for (int b = 0; b < NARR; b++) {
-            const int g = tid + b * BT;
-            const int q = g >> lnb, j = g & (nb - 1);
-            if (q < NARR) {
-                c2<T> *x = a + q * astride;
-                const int k = j & (Ns - 1);
-                const int d = ((j >> lNs) << (lNs + 2)) + k;
+                for (int r = 0; r < 4; r++) {
+                    c2<T> out[NARR];
 #pragma unroll
-                for (int r = 0; r < 4; r++) x[d + r * Ns] = v[b][r];
+                    for (int q = 0; q < NARR; q++) out[q] = v[q][r];
+                    storeL(d + r * Ns, out);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < NARR; q++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) W[q * M + d + r * Ns] = v[q][r];
             }
         }
         __syncthreads();
         Ns <<= 2;
         lNs += 2;
     }
-    if (log2M & 1) {
-        const int nb2 = M >> 1, lnb2 = log2M - 1;
-        const int lstep = log2M - 1 - lNs;          // M / (2 Ns)
-        c2<T> v[2 * NARR][2];
+    if (odd) {
+        const int nb = M >> 1;
+        const int k = j & (Ns - 1);
+        c2<T> v[NARR][2];
+        if (j < nb) {
+            const c2<double> wd = tw[k << (lg - 1 - lNs)];
+            const c2<T> w = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
 #pragma unroll
-        for (int b = 0; b < 2 * NARR; b++) {
-            const int g = tid + b * BT;
-            const int q = g >> lnb2, j = g & (nb2 - 1);
-            if (q < NARR) {
-                const c2<T> *x = a + q * astride;
-                const int k = j & (Ns - 1);
-                const c2<double> wd = tw[k << lstep];
-                const c2<T> w = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
-                const c2<T> x0 = x[j], x1 = cmul(x[j + nb2], w);
-                v[b][0] = {x0.x + x1.x, x0.y + x1.y};
-                v[b][1] = {x0.x - x1.x, x0.y - x1.y};
+            for (int q = 0; q < NARR; q++) {
+                const c2<T> x0 = W[q * M + j], x1 = cmul(W[q * M + j + nb], w);
+                v[q][0] = {x0.x + x1.x, x0.y + x1.y};
+                v[q][1] = {x0.x - x1.x, x0.y - x1.y};
             }
-        }
-        __syncthreads();
+            const int d = ((j >> lNs) << (lNs + 1)) + k;
 #pragma unroll
-        for (int b = 0; b < 2 * NARR; b++) {
-            const int g = tid + b * BT;
-            const int q = g >> lnb2, j = g & (nb2 - 1);
-            if (q < NARR) {
-                c2<T> *x = a + q * astride;
-                const int k = j & (Ns - 1);
-                const int d = ((j >> lNs) << (lNs + 1)) + k;
-                x[d] = v[b][0];
-                x[d + Ns] = v[b][1];
+            for (int r = 0; r < 2; r++) {
+                c2<T> out[NARR];
+#pragma unroll
+                for (int q = 0; q < NARR; q++) out[q] = v[q][r];
+                storeL(d + r * Ns, out);
             }
         }
         __syncthreads();
@@ -351,25 +367,31 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
             }
         }
         __syncthreads();
-        fft_lds<T, 1>(Pc, M, tw, M, lg, false);
-        for (int k = tid; k < M; k += BT) {
-            c2<T> Z[3];
-            pair_multipliers<T>(Gc, G1, M, k, Z);
-            const c2<T> p = Pc[k];
+        // P = FFT(rho)
+        fft_fused<T, 1>(
+            W, tw, M, lg, false,
+            [&](int idx, c2<T>(&v)[1]) __attribute__((always_inline)) { v[0] = Pc[idx]; },
+            [&](int idx, const c2<T>(&v)[1]) __attribute__((always_inline)) { Pc[idx] = v[0]; });
+        // n_{1+2q} + i n_{2+2q} = IFFT(P Z_q): the multiply feeds the first pass, the
+        // last pass writes the weighted densities
+        fft_fused<T, 3>(
+            W, tw, M, lg, true,
+            [&](int idx, c2<T>(&v)[3]) __attribute__((always_inline)) {
+                c2<T> Z[3];
+                pair_multipliers<T>(Gc, G1, M, idx, Z);
+                const c2<T> p = Pc[idx];
 #pragma unroll
-            for (int q = 0; q < 3; q++) W[q * M + k] = cmul(p, Z[q]);
-        }
-        __syncthreads();
-        fft_lds<T, 3>(W, M, tw, M, lg, true);
-        for (int i = tid; i < N; i += BT) {
+                for (int q = 0; q < 3; q++) v[q] = cmul(p, Z[q]);
+            },
+            [&](int idx, const c2<T>(&v)[3]) __attribute__((always_inline)) {
+                if (idx < N) {
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                const c2<T> y = W[q * M + i];
-                fld[(c * F + 1 + 2 * q) * N + i] = sc * y.x;
-                fld[(c * F + 2 + 2 * q) * N + i] = sc * y.y;
-            }
-        }
-        __syncthreads();
+                    for (int q = 0; q < 3; q++) {
+                        fld[(c * F + 1 + 2 * q) * N + idx] = sc * v[q].x;
+                        fld[(c * F + 2 + 2 * q) * N + idx] = sc * v[q].y;
+                    }
+                }
+            });
     }
     // Phi and dPhi/dn, in place over the field slots
     for (int i = tid; i < N; i += BT) {
@@ -388,29 +410,31 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
     for (int c = 0; c < NC; c++) {
         const double *Gc = G + (size_t)c * (F - 1) * G1;
-        // transposed convolutions: Re IFFT(sum_q FFT(u_{1+2q} + i u_{2+2q}) conj(Z_q))
-        for (int k = tid; k < M; k += BT) {
+        // transposed convolutions: Re IFFT(sum_q FFT(u_{1+2q} + i u_{2+2q}) conj(Z_q)); the
+        // first pass reads the gradient pairs, the last one accumulates the spectrum
+        fft_fused<T, 3>(
+            W, tw, M, lg, false,
+            [&](int idx, c2<T>(&v)[3]) __attribute__((always_inline)) {
 #pragma unroll
-            for (int q = 0; q < 3; q++)
-                W[q * M + k] = (k < N) ? c2<T>{fld[(c * F + 1 + 2 * q) * N + k], fld[(c * F + 2 + 2 * q) * N + k]}
-                                       : c2<T>{T(0), T(0)};
-        }
-        __syncthreads();
-        fft_lds<T, 3>(W, M, tw, M, lg, false);
-        for (int k = tid; k < M; k += BT) {
-            c2<T> Z[3];
-            pair_multipliers<T>(Gc, G1, M, k, Z);
-            c2<T> acc = {T(0), T(0)};
+                for (int q = 0; q < 3; q++)
+                    v[q] = (idx < N) ? c2<T>{fld[(c * F + 1 + 2 * q) * N + idx], fld[(c * F + 2 + 2 * q) * N + idx]}
+                                     : c2<T>{T(0), T(0)};
+            },
+            [&](int idx, const c2<T>(&v)[3]) __attribute__((always_inline)) {
+                c2<T> Z[3];
+                pair_multipliers<T>(Gc, G1, M, idx, Z);
+                c2<T> acc = {T(0), T(0)};
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                const c2<T> w = W[q * M + k];
-                acc.x += w.x * Z[q].x + w.y * Z[q].y;              // w * conj(Z)
-                acc.y += w.y * Z[q].x - w.x * Z[q].y;
-            }
-            Pc[k] = acc;
-        }
-        __syncthreads();
-        fft_lds<T, 1>(Pc, M, tw, M, lg, true);
+                for (int q = 0; q < 3; q++) {
+                    acc.x += v[q].x * Z[q].x + v[q].y * Z[q].y;          // v * conj(Z)
+                    acc.y += v[q].y * Z[q].x - v[q].x * Z[q].y;
+                }
+                Pc[idx] = acc;
+            });
+        fft_fused<T, 1>(
+            W, tw, M, lg, true,
+            [&](int idx, c2<T>(&v)[1]) __attribute__((always_inline)) { v[0] = Pc[idx]; },
+            [&](int idx, const c2<T>(&v)[1]) __attribute__((always_inline)) { Pc[idx] = v[0]; });
 #pragma unroll
         for (int j = 0; j < 1024 / BT; j++) {
             const int i = tid + j * BT;
