@@ -23,6 +23,7 @@ EXPORTS = [
     "cdft_ctx_create", "cdft_ctx_destroy", "cdft_last_error", "cdft_system_create",
     "cdft_system_destroy", "cdft_system_npoints", "cdft_functional_eval", "cdft_converge",
     "cdft_converge_batch", "cdft_converge_batch_device", "cdft_last_kernel_ms",
+    "cdft_surface_tension", "cdft_adsorption", "cdft_phi_vjp", "cdft_hvp", "cdft_adjoint_solve",
 ]
 
 
@@ -84,6 +85,11 @@ def load():
                                              C.POINTER(SolverOpts), C.POINTER(Result)]
     L.cdft_last_kernel_ms.argtypes = [vp]
     L.cdft_last_kernel_ms.restype = C.c_double
+    L.cdft_surface_tension.argtypes = [vp, _dp, _dp, C.c_double, _dp]
+    L.cdft_adsorption.argtypes = [vp, _dp, _dp]
+    L.cdft_phi_vjp.argtypes = [vp, _dp, _dp, _dp]
+    L.cdft_hvp.argtypes = [vp, _dp, _dp, _dp]
+    L.cdft_adjoint_solve.argtypes = [vp, _dp, _dp, _dp, C.POINTER(SolverOpts), C.POINTER(Result)]
     _lib = L
     return L
 
@@ -189,6 +195,45 @@ class DeviceSystem:
         if hist_cap:
             out["history"] = hist[:min(r.nhist, hist_cap)].copy()
         return out
+
+    def _vec(self, a, n=None):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        assert a.size == (self.N * self.nc if n is None else n)
+        return a
+
+    def surface_tension(self, rho, mu, p):
+        rho = self._vec(rho)
+        mu = np.ascontiguousarray(mu, dtype=np.float64).reshape(-1)
+        out = np.zeros(1)
+        check(load().cdft_surface_tension(self.handle, ptr(rho), ptr(mu), float(p), ptr(out)), self.ctx.handle)
+        return float(out[0])
+
+    def adsorption(self, rho):
+        rho = self._vec(rho)
+        out = np.zeros(self.nc)
+        check(load().cdft_adsorption(self.handle, ptr(rho), ptr(out)), self.ctx.handle)
+        return out
+
+    def phi_vjp(self, rho, cot):
+        rho, cot = self._vec(rho), self._vec(cot, self.N)
+        out = np.zeros(self.N * self.nc)
+        check(load().cdft_phi_vjp(self.handle, ptr(rho), ptr(cot), ptr(out)), self.ctx.handle)
+        return out
+
+    def hvp(self, rho, v):
+        rho, v = self._vec(rho), self._vec(v)
+        out = np.zeros(self.N * self.nc)
+        check(load().cdft_hvp(self.handle, ptr(rho), ptr(v), ptr(out)), self.ctx.handle)
+        return out
+
+    def adjoint_solve(self, rho, g, lam0=None, *, maxit=10000, beta=1e-2, tol=1e-10, anderson_start=1e-1, m=5):
+        rho, g = self._vec(rho), self._vec(g)
+        lam = np.zeros(self.N * self.nc) if lam0 is None else self._vec(lam0).copy()
+        o = SolverOpts(int(maxit), float(beta), float(tol), float(anderson_start), int(m), 0, None)
+        r = Result()
+        check(load().cdft_adjoint_solve(self.handle, ptr(rho), ptr(g), ptr(lam), C.byref(o), C.byref(r)),
+              self.ctx.handle)
+        return lam, r.as_dict()
 
 
 def converge_batch(ctx, systems, rhos, *, maxit=10000, beta=1e-2, tol=1e-4, anderson_start=1e-1,

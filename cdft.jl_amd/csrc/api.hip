@@ -102,24 +102,65 @@ void launch_phi(cdft_sys *s, hipStream_t st)
                        s->NA, s->pc);
 }
 
-// Weighted densities, Phi + gradient and transposed convolutions into s->dphi / s->part.
-void launch_functional(cdft_sys *s)
+void launch_phi_any(cdft_sys *s)
 {
     hipStream_t st = s->ctx->stream;
-    int N = s->N, F = s->F, nc = s->nc;
-    dim3 g(nblocks(N), nc * (F - 1));
-    // forward: src = rho_c, dst = n[(c*F+f)*N]
-    hipLaunchKernelGGL(k_circ1d<double>, g, dim3(256), 0, st, s->rho, (long)N, 0L, s->h, s->n, (long)F * N,
-                       (long)N, 0, N, F, s->NA, 0);
-    switch (nc) {
+    switch (s->nc) {
     case 1: launch_phi<1>(s, st); break;
     case 2: launch_phi<2>(s, st); break;
     case 3: launch_phi<3>(s, st); break;
     default: launch_phi<4>(s, st); break;
     }
-    // transposed: src = dphi[(c*F+f)*N], dst = part[(c*(F-1)+f-1)*N], minus on the vector slot
-    hipLaunchKernelGGL(k_circ1d<double>, g, dim3(256), 0, st, s->dphi, (long)F * N, (long)N, s->h, s->part,
-                       (long)(F - 1) * N, (long)N, 1, N, F, 1.0, 1);
+}
+
+// Transposed convolutions: src = dphi[(c*F+f)*N], dst = part[(c*(F-1)+f-1)*N], minus
+// on the vector slot.
+void launch_transposed(cdft_sys *s)
+{
+    int N = s->N, F = s->F;
+    hipLaunchKernelGGL(k_circ1d<double>, dim3(nblocks(N), s->nc * (F - 1)), dim3(256), 0, s->ctx->stream, s->dphi,
+                       (long)F * N, (long)N, s->h, s->part, (long)(F - 1) * N, (long)N, 1, N, F, 1.0, 1);
+}
+
+// Weighted densities of rho (s->rho) into s->n.
+void launch_weighted(cdft_sys *s)
+{
+    int N = s->N, F = s->F;
+    hipLaunchKernelGGL(k_circ1d<double>, dim3(nblocks(N), s->nc * (F - 1)), dim3(256), 0, s->ctx->stream, s->rho,
+                       (long)N, 0L, s->h, s->n, (long)F * N, (long)N, 0, N, F, s->NA, 0);
+}
+
+// Weighted densities, Phi + gradient and transposed convolutions into s->dphi / s->part.
+void launch_functional(cdft_sys *s)
+{
+    launch_weighted(s);
+    launch_phi_any(s);
+    launch_transposed(s);
+}
+
+template <int NC>
+void launch_phi_hvp(cdft_sys *s, const double *v, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_phi_hvp<NC, 1>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, v, s->n, s->dphi, s->N,
+                       s->NA, s->pc);
+}
+
+// Hessian-vector product of F_res at s->rho (weighted densities already in s->n)
+// along v: dphi <- H_Phi (N_A v, W v), part <- transposed convolutions of it.
+void launch_hvp(cdft_sys *s, const double *v)
+{
+    hipStream_t st = s->ctx->stream;
+    int N = s->N, F = s->F, nc = s->nc;
+    dim3 g(nblocks(N), nc * (F - 1));
+    hipLaunchKernelGGL(k_circ1d<double>, g, dim3(256), 0, st, v, (long)N, 0L, s->h, s->dphi, (long)F * N, (long)N, 0,
+                       N, F, s->NA, 0);
+    switch (nc) {
+    case 1: launch_phi_hvp<1>(s, v, st); break;
+    case 2: launch_phi_hvp<2>(s, v, st); break;
+    case 3: launch_phi_hvp<3>(s, v, st); break;
+    default: launch_phi_hvp<4>(s, v, st); break;
+    }
+    launch_transposed(s);
 }
 
 }  // namespace
@@ -294,7 +335,8 @@ int cdft_functional_eval(cdft_sys *s, const double *rho, double *dfdrho, double 
     launch_functional(s);
     hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, s->dphi, s->part, (const double *)nullptr,
                        s->N, s->nc, s->F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
-                       (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0);
+                       (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
+                       (const double *)nullptr, (const double *)nullptr);
     HIPCHECK_CTX(ctx, hipGetLastError());
     HIPCHECK_CTX(ctx, hipMemcpyAsync(dfdrho, s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     std::vector<double> fv;
@@ -311,10 +353,19 @@ int cdft_functional_eval(cdft_sys *s, const double *rho, double *dfdrho, double 
     return CDFT_OK;
 }
 
-// converge!(prob, ::AASol, rho) with aasol (src/utils/anderson.jl:301-443): the
-// fixed-point map and every N-length vector operation run on the device; the
-// m x m QR downdate and the triangular solve run on the host, as in the reference.
-int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdft_result *r)
+// The linear map of the fixed-point adjoint, lambda -> g - rho* (H (mask lambda)),
+// H the Hessian of F_res at rho* (device buffers; rho* in s->rho, W rho* in s->n).
+struct AdjMap {
+    const double *g;
+    const double *mask;
+    double *vbuf;              // mask lambda, the direction of the next product
+};
+
+// aasol (src/utils/anderson.jl:301-443) on the device: the fixed-point map and
+// every N-length vector operation run on the device; the m x m QR downdate and
+// the triangular solve run on the host, as in the reference.  adj == nullptr: the
+// Euler-Lagrange map of converge! on x = log rho; otherwise the adjoint map on x.
+static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdft_result *r, const AdjMap *adj)
 {
     if (!s || !rho_inout || !o || !r) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_converge: NULL argument");
     cdft_ctx *ctx = s->ctx;
@@ -338,10 +389,14 @@ int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdf
     auto hist = [&](double v) { if (o->hist && nh < o->hist_cap) o->hist[nh] = v; nh++; };
 
     // G evaluation + post-processing; returns the finalized reductions in fin_h[0..2]
+    double *ptgt = adj ? adj->vbuf : s->rho;              // where a new point is expanded
+    const double *pmask = adj ? adj->mask : nullptr;
     auto evalG = [&](int mode, double b) -> int {
-        launch_functional(s);
+        if (adj) launch_hvp(s, adj->vbuf);
+        else launch_functional(s);
         hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, s->dphi, s->part, s->vext, s->N, s->nc, s->F,
-                           s->cpd, s->out, s->sol, s->gx, s->res, s->df, s->dg, b, mode, s->red, nb);
+                           s->cpd, s->out, s->sol, s->gx, s->res, s->df, s->dg, b, mode, s->red, nb,
+                           adj ? adj->g : nullptr, adj ? (const double *)s->rho : nullptr);
         hipLaunchKernelGGL(k_finalize, dim3(3), dim3(256), 0, st, s->red, nb, s->fin);
         HIPCHECK_CTX(ctx, hipGetLastError());
         HIPCHECK_CTX(ctx, hipMemcpyAsync(fin_h, s->fin, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
@@ -351,8 +406,13 @@ int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdf
     };
 #define EVAL(mode, b) do { int rc_ = evalG((mode), (b)); if (rc_ != CDFT_OK) return rc_; } while (0)
 
-    HIPCHECK_CTX(ctx, hipMemcpyAsync(s->out, rho_inout, sizeof(double) * n, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_init_x, dim3(nb), dim3(256), 0, st, s->out, s->sol, s->rho, (int)n);
+    if (adj) {
+        HIPCHECK_CTX(ctx, hipMemcpyAsync(s->sol, rho_inout, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->sol, s->sol, ptgt, (int)n, pmask);
+    } else {
+        HIPCHECK_CTX(ctx, hipMemcpyAsync(s->out, rho_inout, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_init_x, dim3(nb), dim3(256), 0, st, s->out, s->sol, s->rho, (int)n);
+    }
 
     int k = 0;
     EVAL(1, beta);
@@ -364,13 +424,13 @@ int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdf
     bool toosoon = (resnorm <= tol);
 
     if (!toosoon && picard_maxit > 0) {
-        hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, s->rho, (int)n);
+        hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, ptgt, (int)n, pmask);
         for (int it = 0; it < picard_maxit - 1; it++) {
             EVAL(1, picard_beta);
             resnorm = std::sqrt(fin_h[0]);
             hist(resnorm);
             if (resnorm <= picard_tol || resnorm >= up_bd) break;
-            hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, s->rho, (int)n);
+            hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, ptgt, (int)n, pmask);
         }
         toosoon = (resnorm <= tol);
     }
@@ -380,7 +440,7 @@ int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdf
     for (int j = 0; j < CDFT_MAXM; j++) ring[j] = j;
     double dfn2 = 0.0, dfres = 0.0;
     if (!toosoon) {
-        hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, s->rho, (int)n);
+        hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, ptgt, (int)n, pmask);
         k += 1;
         EVAL(2, beta);
         resnorm = std::sqrt(fin_h[0]);
@@ -392,7 +452,7 @@ int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdf
     double *Qc = s->Q, *Qa = s->Qalt;
     while (k < maxit + picard_maxit && resnorm > tol && !toosoon && resnorm < up_bd) {
         if (m == 0) {
-            hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, s->rho, (int)n);
+            hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, ptgt, (int)n, pmask);
         } else {
             // BuildDG!(DG, m, k+1, dg) with a ring of physical columns
             int kk = k + 1, slot;
@@ -453,7 +513,7 @@ int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdf
             }
             for (int j = 0; j < mk; j++) th.col[j] = ring[j];
             hipLaunchKernelGGL(k_aa_newpoint, dim3(nb), dim3(256), 0, st, (int)n, s->df, nrm,
-                               Qc + (size_t)qcol * n, s->sol, s->gx, s->DG, th, mk, s->rho);
+                               Qc + (size_t)qcol * n, s->sol, s->gx, s->DG, th, mk, ptgt, pmask);
         }
         k += 1;
         EVAL(2, beta);
@@ -470,9 +530,9 @@ int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdf
     else if (resnorm <= tol) { idid = 1; errcode = 0; }
     else { idid = 0; errcode = 10; }
 
-    hipLaunchKernelGGL(k_exp, dim3(nb), dim3(256), 0, st, s->sol, s->out, (int)n);
+    if (!adj) hipLaunchKernelGGL(k_exp, dim3(nb), dim3(256), 0, st, s->sol, s->out, (int)n);
     HIPCHECK_CTX(ctx, hipGetLastError());
-    HIPCHECK_CTX(ctx, hipMemcpyAsync(rho_inout, s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(rho_inout, adj ? s->sol : s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
     r->idid = idid;
     r->errcode = errcode;
@@ -484,6 +544,137 @@ int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdf
     r->nevals_fp32 = 0;
     r->reserved = 0;
     r->device_ms = 0.0;
+    return CDFT_OK;
+}
+
+int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdft_result *r)
+{
+    return aasol_run(s, rho_inout, o, r, nullptr);
+}
+
+// Device scratch of one call, freed on every return path.
+struct DevBufs {
+    std::vector<double *> p;
+    ~DevBufs()
+    {
+        for (double *q : p) (void)hipFree(q);
+    }
+    hipError_t get(double **out, size_t count)
+    {
+        *out = nullptr;
+        hipError_t e = hipMalloc((void **)out, sizeof(double) * (count ? count : 1));
+        if (e == hipSuccess) p.push_back(*out);
+        return e;
+    }
+};
+
+int cdft_phi_vjp(cdft_sys *s, const double *rho, const double *cot, double *out)
+{
+    if (!s || !rho || !cot || !out) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_phi_vjp: NULL argument");
+    cdft_ctx *ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    const long n = (long)s->N * s->nc;
+    HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
+    DevBufs tmp;
+    double *cotd = nullptr;
+    HIPCHECK_CTX(ctx, tmp.get(&cotd, s->N));
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(s->rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(cotd, cot, sizeof(double) * s->N, hipMemcpyHostToDevice, st));
+    launch_weighted(s);
+    launch_phi_any(s);
+    hipLaunchKernelGGL(k_scale_slots, dim3(nblocks(s->N)), dim3(256), 0, st, s->dphi, cotd, s->N, s->nc * s->F);
+    launch_transposed(s);
+    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, s->dphi, s->part, (const double *)nullptr,
+                       s->N, s->nc, s->F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
+                       (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
+                       (const double *)nullptr, (const double *)nullptr);
+    HIPCHECK_CTX(ctx, hipGetLastError());
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(out, s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+    return CDFT_OK;
+}
+
+int cdft_hvp(cdft_sys *s, const double *rho, const double *v, double *out)
+{
+    if (!s || !rho || !v || !out) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_hvp: NULL argument");
+    cdft_ctx *ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    const long n = (long)s->N * s->nc;
+    HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
+    DevBufs tmp;
+    double *vd = nullptr;
+    HIPCHECK_CTX(ctx, tmp.get(&vd, n));
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(s->rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(vd, v, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    launch_weighted(s);
+    launch_hvp(s, vd);
+    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, s->dphi, s->part, (const double *)nullptr,
+                       s->N, s->nc, s->F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
+                       (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
+                       (const double *)nullptr, (const double *)nullptr);
+    HIPCHECK_CTX(ctx, hipGetLastError());
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(out, s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+    return CDFT_OK;
+}
+
+int cdft_adjoint_solve(cdft_sys *s, const double *rho, const double *g, double *lambda_inout,
+                       const cdft_solver_opts *o, cdft_result *r)
+{
+    if (!s || !rho || !g || !lambda_inout || !o || !r)
+        return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_adjoint_solve: NULL argument");
+    cdft_ctx *ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    const long n = (long)s->N * s->nc;
+    HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
+    DevBufs tmp;
+    double *gd = nullptr, *maskd = nullptr, *vbuf = nullptr;
+    HIPCHECK_CTX(ctx, tmp.get(&gd, n));
+    HIPCHECK_CTX(ctx, tmp.get(&maskd, n));
+    HIPCHECK_CTX(ctx, tmp.get(&vbuf, n));
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(s->rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(gd, g, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    // dF_res/drho at rho*, then the mask of points where the exponent clamp is inactive
+    launch_functional(s);
+    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, s->dphi, s->part, (const double *)nullptr,
+                       s->N, s->nc, s->F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
+                       (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
+                       (const double *)nullptr, (const double *)nullptr);
+    hipLaunchKernelGGL(k_clamp_mask, dim3(nblocks(n)), dim3(256), 0, st, s->out, s->vext, s->cpd, s->N, s->nc, maskd);
+    HIPCHECK_CTX(ctx, hipGetLastError());
+    AdjMap am{gd, maskd, vbuf};
+    return aasol_run(s, lambda_inout, o, r, &am);     // s->n keeps W rho* for every product
+}
+
+int cdft_surface_tension(cdft_sys *s, const double *rho, const double *mu, double p, double *out)
+{
+    if (!s || !rho || !mu || !out) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_surface_tension: NULL argument");
+    const long n = (long)s->N * s->nc;
+    std::vector<double> dfd(n), fv(s->N);
+    double fres = 0.0;
+    int rc = cdft_functional_eval(s, rho, dfd.data(), &fres, nullptr);
+    if (rc != CDFT_OK) return rc;
+    const double dz = (s->ub[0] - s->lb[0]) / s->N;
+    std::vector<double> phi(s->N, 0.0), one(s->N, 1.0);
+    const double NAv = 6.02214076e23, kB = 1.380649e-23;
+    for (int c = 0; c < s->nc; c++)
+        for (int i = 0; i < s->N; i++) {
+            const double r = rho[(size_t)c * s->N + i];
+            phi[i] += NAv * r * (std::log(r) - 1.5 * std::log(s->T) - 1.0);   // f_ideal, BasicIdeal.jl:10-39
+        }
+    double chem = 0.0;
+    for (int c = 0; c < s->nc; c++) chem += mu[c] * simpson1d(rho + (size_t)c * s->N, s->N, dz);
+    const double F = simpson1d(phi.data(), s->N, dz) + fres;
+    *out = F * kB * s->T - chem + p * simpson1d(one.data(), s->N, dz);   // surface_tension.jl:31-53
+    return CDFT_OK;
+}
+
+int cdft_adsorption(cdft_sys *s, const double *rho, double *out)
+{
+    if (!s || !rho || !out) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_adsorption: NULL argument");
+    const double V = s->ub[0] - s->lb[0];
+    const double dz = V / s->N;
+    for (int c = 0; c < s->nc; c++) out[c] = simpson1d(rho + (size_t)c * s->N, s->N, dz) / V;   // adsorption.jl:22-32
     return CDFT_OK;
 }
 

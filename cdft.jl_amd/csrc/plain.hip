@@ -11,6 +11,7 @@
 // Re IFFT(FFT(.) * Omega_f) (src/utils/integrals.jl:88-96), so the result equals
 // the reference's FFT convolution up to rounding.
 #include "common.h"
+#include "dual.h"
 #include "kernels.h"
 
 namespace cdft {
@@ -140,6 +141,53 @@ __global__ __launch_bounds__(256) void k_phi(const double *__restrict__ rho, con
     if (fval) fval[i] = v;
 }
 
+// Hessian-vector product of Phi at every point (no reference counterpart: the
+// second derivative behind the fixed-point adjoint).  Slot 0 moves along N_A v,
+// the other slots along their weighted densities ndot = W v (in ndio on entry);
+// on exit ndio holds H_Phi(n) (N_A v, ndot) for every slot, the forward-mode
+// derivative of phi_grad taken with dual numbers.
+template <int NC, int ND>
+__global__ __launch_bounds__(256) void k_phi_hvp(const double *__restrict__ rho, const double *__restrict__ v,
+                                                 const double *__restrict__ n, double *__restrict__ ndio, int N,
+                                                 double NA, PhiConst P)
+{
+    constexpr int F = 6 + ND;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    dual nl[NC][F], g[NC][F];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        nl[c][0] = dual(rho[(size_t)c * N + i] * NA, v[(size_t)c * N + i] * NA);
+#pragma unroll
+        for (int f = 1; f < F; f++) nl[c][f] = dual(n[(size_t)(c * F + f) * N + i], ndio[(size_t)(c * F + f) * N + i]);
+    }
+    phi_grad<dual, dual, NC, ND>(P, nl, g);
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int f = 0; f < F; f++) ndio[(size_t)(c * F + f) * N + i] = g[c][f].d;
+}
+
+// mask = 1 where get_new_profile!'s clamp of the exponent to [-100, 100] is inactive
+// (converge.jl:150): the exponent's derivative vanishes where it is clamped.
+__global__ void k_clamp_mask(const double *dfdrho, const double *vext, const double *cp, int N, int nc, double *mask)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * nc) return;
+    const int c = t / N;
+    const double lg = cp[c] - dfdrho[t] - (vext ? vext[t] : 0.0);
+    mask[t] = (lg > -100.0 && lg < 100.0) ? 1.0 : 0.0;
+}
+
+// dphi[(c F + f) N + i] *= cot[i]: the pullback of sum_i cot_i Phi_i (rrule seed).
+__global__ void k_scale_slots(double *dphi, const double *cot, int N, int nslots)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double w = cot[i];
+    for (int q = 0; q < nslots; q++) dphi[(size_t)q * N + i] *= w;
+}
+
 // ---------------------------------------------------------------------------
 // Reductions: each block writes one partial per quantity; k_finalize sums the
 // partials of each quantity in a fixed order (bitwise reproducible).
@@ -190,14 +238,14 @@ __global__ void k_init_x(const double *rho_in, double *sol, double *rho, int n)
     rho[i] = exp(x);
 }
 
-// sol = src; rho = exp(sol)
-__global__ void k_set_point(const double *src, double *sol, double *rho, int n)
+// sol = src; rho = exp(sol), or rho = mask sol for the linear adjoint map
+__global__ void k_set_point(const double *src, double *sol, double *rho, int n, const double *mask)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double x = src[i];
     sol[i] = x;
-    rho[i] = exp(x);
+    rho[i] = mask ? mask[i] * x : exp(x);
 }
 
 __global__ void k_exp(const double *x, double *rho, int n)
@@ -211,13 +259,16 @@ __global__ void k_exp(const double *x, double *rho, int n)
 //  mode 0: out = dF_res/drho (no field, no clamp)                functional only
 //  mode 1: gx = betafix(lnG); res = gx - sol; partial |res|^2     first/Picard step
 //  mode 2: aa_point! (anderson.jl:470-481); partials |res|^2, |df|^2, df.res
+// With adj_g set, the map value is the adjoint's g - rho* (dphi + parts) instead of
+// the clamped exponent (dphi, part then hold a Hessian-vector product).
 __global__ __launch_bounds__(256) void k_update(const double *__restrict__ dphi, const double *__restrict__ part,
                                                 const double *__restrict__ vext, int N, int nc, int F,
                                                 const double *__restrict__ cp, double *__restrict__ out,
                                                 const double *__restrict__ sol, double *__restrict__ gx,
                                                 double *__restrict__ res, double *__restrict__ df,
                                                 double *__restrict__ dg, double beta, int mode, double *red,
-                                                int nblk)
+                                                int nblk, const double *__restrict__ adj_g,
+                                                const double *__restrict__ adj_rho)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
@@ -228,9 +279,14 @@ __global__ __launch_bounds__(256) void k_update(const double *__restrict__ dphi,
         if (mode == 0) {
             out[t] = s;
         } else {
-            if (vext) s += vext[t];
-            double lg = cp[c] - s;
-            lg = fmin(fmax(lg, -100.0), 100.0);
+            double lg;
+            if (adj_g) {                       // adjoint map: g - rho* (H (mask lambda))
+                lg = adj_g[t] - adj_rho[t] * s;
+            } else {
+                if (vext) s += vext[t];
+                lg = cp[c] - s;
+                lg = fmin(fmax(lg, -100.0), 100.0);
+            }
             double x = sol[t];
             double g = (beta == 1.0) ? lg : (1.0 - beta) * x + beta * lg;
             if (mode == 1) {
@@ -321,7 +377,8 @@ __global__ __launch_bounds__(256) void k_aa_proj2(int n, const double *__restric
 __global__ __launch_bounds__(256) void k_aa_newpoint(int n, const double *__restrict__ df, double nrm,
                                                      double *__restrict__ qcol, double *__restrict__ sol,
                                                      const double *__restrict__ gx, const double *__restrict__ DG,
-                                                     AAMat th, int mk, double *__restrict__ rho)
+                                                     AAMat th, int mk, double *__restrict__ rho,
+                                                     const double *__restrict__ mask)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -330,7 +387,7 @@ __global__ __launch_bounds__(256) void k_aa_newpoint(int n, const double *__rest
     for (int j = 0; j < mk; j++) a += DG[(size_t)th.col[j] * n + i] * th.a[j];
     double x = gx[i] - a;
     sol[i] = x;
-    rho[i] = exp(x);
+    rho[i] = mask ? mask[i] * x : exp(x);
 }
 
 }  // namespace cdft

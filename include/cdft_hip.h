@@ -138,6 +138,34 @@ int cdft_converge_batch_device(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, do
  * events on the context's stream. */
 double cdft_last_kernel_ms(const cdft_ctx *ctx);
 
+/* Post-processing of a converged profile.
+ * surface_tension: out = F k_B T - sum_i mu_i int rho_i + p int 1 with F = F_ideal +
+ * F_res (Simpson, dz = (ub - lb) / N); mu[nc] in J/mol, p in Pa.  Replaces
+ * surface_tension(system, rho) (src/methods/surface_tension.jl:31-53); the planar
+ * interface's tension is half of it (surface_tension.jl:28).
+ * adsorption: out[i] = int rho_i / (ub - lb).  Replaces adsorption(system, rho)
+ * (src/methods/adsorption.jl:22-32). */
+int cdft_surface_tension(cdft_sys *sys, const double *rho, const double *mu, double p, double *out);
+int cdft_adsorption(cdft_sys *sys, const double *rho, double *out);
+
+/* Differentiation.  The reference's only derivative is dPhi/dn, taken with
+ * Enzyme inside dFdrho_res (src/models/models.jl:355-411); these have no
+ * reference counterpart.
+ * phi_vjp: pullback of sum_i cot_i Phi(n_i) to rho, divided by N_A L^3:
+ *   out = cot * dPhi/dn_rho + sum_f W_f^T (cot * dPhi/dn_f)   (cot[N]);
+ * with cot = 1 it equals cdft_functional_eval's dfdrho (the rrule of F_res).
+ * hvp: out = d(dfdrho)/drho . v at rho, the Hessian of F_res applied to v.
+ * adjoint_solve: lambda with (I - J^T) lambda = g at the fixed point rho, J the
+ * Jacobian of the undamped map x -> clamp(mu_res + ln rho_b - dfdrho(e^x) - V)
+ * (converge.jl:108-153) at x = log rho.  Solved by the same damped Picard /
+ * Anderson iteration as cdft_converge (opts, result as there); lambda_inout holds
+ * the initial guess.  dJ/dtheta = dJ/dtheta|_rho + lambda . dlnG/dtheta|_rho for an
+ * objective J(rho, theta) with g = rho * dJ/drho. */
+int cdft_phi_vjp(cdft_sys *sys, const double *rho, const double *cot, double *out);
+int cdft_hvp(cdft_sys *sys, const double *rho, const double *v, double *out);
+int cdft_adjoint_solve(cdft_sys *sys, const double *rho, const double *g, double *lambda_inout,
+                       const cdft_solver_opts *opts, cdft_result *result);
+
 #ifdef __cplusplus
 }
 #endif
