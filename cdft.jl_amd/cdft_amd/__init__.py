@@ -3,11 +3,12 @@ Euler-Lagrange Picard/Anderson fixed point) behind cDFT.jl's DFTSystem /
 initialize_profiles / converge! API.  Compute runs in libcdft_hip.so (HIP, gfx950);
 there is no CPU fallback."""
 from .bulk import KB, NA, RGAS, PCSAFT
-from .structure import Steele, Structure, TwoPhase1DCart, Uniform1DCart
+from .structure import HardSphereMatrix, Steele, Structure, TwoPhase1DCart, Uniform1DCart, Uniform3DCart
 from .system import (DFTOptions, DFTSystem, F_ideal, F_res, converge_, dFdrho_res, free_energy,
-                     initialize_profiles, simpson)
+                     initialize_profiles, integrate, integration_weights, simpson)
 from .methods import adsorption, surface_tension
 
-__all__ = ["PCSAFT", "Steele", "Structure", "Uniform1DCart", "TwoPhase1DCart", "DFTOptions", "DFTSystem",
-           "initialize_profiles", "converge_", "dFdrho_res", "F_res", "F_ideal", "free_energy", "simpson",
+__all__ = ["PCSAFT", "Steele", "HardSphereMatrix", "Structure", "Uniform1DCart", "Uniform3DCart", "TwoPhase1DCart",
+           "DFTOptions", "DFTSystem", "initialize_profiles", "converge_", "dFdrho_res", "F_res", "F_ideal",
+           "free_energy", "simpson", "integrate", "integration_weights",
            "surface_tension", "adsorption", "NA", "KB", "RGAS"]

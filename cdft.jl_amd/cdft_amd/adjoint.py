@@ -22,13 +22,13 @@ import numpy as np
 from .bulk import KB, NA, PCSAFT
 from .methods import adsorption_system, surface_tension_system
 from .structure import Structure
-from .system import DFTSystem
+from .system import DFTSystem, integration_weights
 
 PARAMS = ("sigma", "epsilon", "segment")
 
 
 def simpson_weights(n, dz):
-    """Weights of _int (src/utils/integrals.jl:28-40): 1, 4, 2, ..., (4|2), 1 times dz / 3."""
+    """Weights of _int along one axis (src/utils/integrals.jl:28-40): 1, 4, 2, ..., (4|2), 1 times dz / 3."""
     k = np.arange(1, n + 1)
     c = np.where(k % 2 == 0, 4.0, 2.0)
     c[0] = 1.0
@@ -60,11 +60,12 @@ def perturbed_system(system, comp, param, delta):
     x = st.rho_bulk / st.rho_bulk.sum()
     if st.topology == "two_phase":
         p, vl, vv, y = new.bubble_pressure(T, x)
-        nst = Structure((p, T), x / vl, [st.bounds[0]], st.ngrid, topology="two_phase", rho_bulk2=y / vv)
+        nst = Structure((p, T), x / vl, list(st.bounds), st.ngrid, topology="two_phase", rho_bulk2=y / vv,
+                        ndim=st.ndim)
     else:
         phase = "liquid" if st.rho_bulk.sum() * model.volume(p, T, x, phase="liquid") > 0.5 else "vapour"
         v = new.volume(p, T, x, phase=phase)
-        nst = Structure((p, T), x / v, [st.bounds[0]], st.ngrid, topology=st.topology)
+        nst = Structure((p, T), x / v, list(st.bounds), st.ngrid, topology=st.topology, ndim=st.ndim)
     return DFTSystem(new, nst, system.external_field or None, system.options)
 
 
@@ -81,11 +82,10 @@ def objective_grad_rho(system, rho, kind, comp=0):
     st = system.structure
     N, nc = system.N, system.nc
     rho = np.asarray(rho, dtype=np.float64).reshape(nc, N)
-    w = simpson_weights(N, st.dz())
+    w = integration_weights(st)
     out = np.zeros((nc, N))
     if kind == "adsorption":
-        lb, ub = st.bounds[0]
-        out[comp] = w / (ub - lb)
+        out[comp] = w / st.volume()
         return out.reshape(-1)
     if kind == "gamma":
         p, T = st.conditions

@@ -10,11 +10,14 @@
   slit_pores      512 methane slit pores between two graphite Steele walls at 150 K,
                   widths W ~ U[2 sigma, 12 sigma], p ~ logU[1e3, 1e7] Pa from seed 0,
                   stable bulk root, N = round(40 W / sigma) points over [0.5 L, W - 0.5 L]
+  matrix_3d       methane / ethane / propane (x = 0.6, 0.3, 0.1) at 200 K and 1e7 Pa (liquid
+                  root) in a periodic cube of 30 sigma on a 256^3 grid, with a matrix of 64
+                  non-overlapping hard spheres, radii ~ U[sigma, 2.5 sigma], from seed 0
 """
 import numpy as np
 
 from .bulk import PCSAFT
-from .structure import Steele, TwoPhase1DCart, Uniform1DCart
+from .structure import HardSphereMatrix, Steele, TwoPhase1DCart, Uniform1DCart, Uniform3DCart
 from .system import DFTOptions, DFTSystem, initialize_profiles
 
 
@@ -61,3 +64,16 @@ def slit_pores(n=512, seed=0, options=None, T=150.0):
         system = DFTSystem(model, structure, Steele(["graphite"], width), options)
         out.append((system, initialize_profiles(system)))
     return out
+
+
+def matrix_3d(options=None, n=256, box=30.0, nspheres=64, seed=0, T=200.0, p=1e7, x=(0.6, 0.3, 0.1),
+              components=("methane", "ethane", "propane"), rmin=1.0, rmax=2.5):
+    model = PCSAFT(list(components))
+    x = np.asarray(x, dtype=np.float64)
+    v = model.volume(p, T, x, phase="liquid")
+    L = model.length_scale()
+    side = box * L
+    structure = Uniform3DCart((p, T), x / v, [[0.0, side]] * 3, n)
+    field = HardSphereMatrix.random(nspheres, rmin * L, rmax * L, side, seed=seed) if nspheres else None
+    system = DFTSystem(model, structure, field, options)
+    return system, initialize_profiles(system)

@@ -9,7 +9,7 @@ import numpy as np
 
 from .bulk import KB
 from .structure import TwoPhase1DCart, Uniform1DCart
-from .system import DFTSystem, converge_, free_energy, initialize_profiles, simpson
+from .system import DFTSystem, converge_, free_energy, initialize_profiles, integrate, simpson
 
 
 def surface_tension_system(system, rho):
@@ -46,9 +46,8 @@ def surface_tension(model_or_system, T_or_rho, x=None, *, ngrid=201, box=10.0, o
 
 def adsorption_system(system, rho):
     rho = np.asarray(rho, dtype=np.float64).reshape(system.nc, system.N)
-    lb, ub = system.structure.bounds[0]
-    V = ub - lb
-    return np.array([simpson(rho[i], system.structure.dz()) / V for i in range(system.nc)])
+    V = system.structure.volume()
+    return np.array([integrate(system.structure, rho[i]) / V for i in range(system.nc)])
 
 
 def adsorption(model_or_system, surface_or_rho, p=None, T=None, n=(1.0,), *, options=None, **kw):

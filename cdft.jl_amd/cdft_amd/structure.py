@@ -1,50 +1,79 @@
 """Structures (grids + state conditions) and external fields, host side.
 
 Mirrors the reference's constructors:
-  Uniform1DCart   src/base/structure.jl:211-213
-  TwoPhase1DCart  src/base/structure.jl (TwoPhaseSystem, ``TwoPhase1DCart``)
+  Uniform1DCart   src/base/structure.jl:310
+  Uniform3DCart   src/base/structure.jl:335-352
+  TwoPhase1DCart  src/base/structure.jl:491 (``TwoPhase1DCart``)
   Steele          src/structure/external_fields/steele.jl:31-46 (database/Steele/steele_like.csv)
+and adds a field the reference does not have:
+  HardSphereMatrix  a periodic matrix of impenetrable spheres on a 3D grid
 """
 import numpy as np
 
 
 class Structure:
-    """A 1D Cartesian grid with (p, T) conditions and bulk densities."""
+    """A 1D or 3D Cartesian grid with (p, T) conditions and bulk densities."""
 
-    def __init__(self, conditions, rho_bulk, bounds, ngrid, topology="uniform", rho_bulk2=None):
+    def __init__(self, conditions, rho_bulk, bounds, ngrid, topology="uniform", rho_bulk2=None, ndim=1):
         p, T = conditions
         self.conditions = (float(p), float(T))
         self.rho_bulk = np.asarray(rho_bulk, dtype=np.float64).reshape(-1)
         b = np.asarray(bounds, dtype=np.float64).reshape(-1)
-        if b.size != 2:
-            raise ValueError("1D bounds must have exactly 2 numbers")   # DimensionMismatch in the reference
-        if not b[1] > b[0]:
+        if b.size != 2 * ndim:
+            raise ValueError(f"{ndim}D bounds must have exactly {2 * ndim} numbers")   # DimensionMismatch
+        b = b.reshape(ndim, 2)
+        if not np.all(b[:, 1] > b[:, 0]):
             raise ValueError("bounds must satisfy ub > lb")
-        self.bounds = ((float(b[0]), float(b[1])),)
-        n = ngrid[0] if isinstance(ngrid, (tuple, list)) else ngrid
-        self.ngrid = (int(n),)
+        self.bounds = tuple((float(x[0]), float(x[1])) for x in b)
+        g = list(ngrid) if isinstance(ngrid, (tuple, list)) else [ngrid]
+        if len(g) == 1 and ndim > 1:
+            g = g * ndim                                             # one count for every axis
+        if len(g) != ndim:
+            raise ValueError(f"ngrid must have {ndim} entries")
+        self.ngrid = tuple(int(n) for n in g)
         self.topology = topology
         self.rho_bulk2 = None if rho_bulk2 is None else np.asarray(rho_bulk2, dtype=np.float64).reshape(-1)
 
     @property
     def ndim(self):
-        return 1
+        return len(self.ngrid)
 
-    def coords(self):
-        """LinRange(lb, ub, N) (get_coords, src/structure/structure.jl:138-151)."""
-        lb, ub = self.bounds[0]
-        N = self.ngrid[0]
+    @property
+    def npoints(self):
+        return int(np.prod(self.ngrid))
+
+    def axis_coords(self, a):
+        """LinRange(lb, ub, N) of axis a (get_coords, src/structure/structure.jl:138-151)."""
+        lb, ub = self.bounds[a]
+        N = self.ngrid[a]
         t = np.arange(N, dtype=np.float64) / (N - 1)
         return (1.0 - t) * lb + t * ub
 
+    def coords(self):
+        """1D: the coordinate vector; 3D: the list of per-axis coordinate vectors."""
+        if self.ndim == 1:
+            return self.axis_coords(0)
+        return [self.axis_coords(a) for a in range(self.ndim)]
+
     def dz(self):
-        """(ub - lb)/N (structure_dz, src/structure/structure.jl:187-195)."""
-        lb, ub = self.bounds[0]
-        return (ub - lb) / self.ngrid[0]
+        """(ub - lb)/N of the first axis (structure_dz, src/structure/structure.jl:187-195)."""
+        return self.dzs()[0]
+
+    def dzs(self):
+        return tuple((ub - lb) / n for (lb, ub), n in zip(self.bounds, self.ngrid))
+
+    def volume(self):
+        """Product of the bound lengths (_domain_volume, src/methods/adsorption.jl:15)."""
+        return float(np.prod([ub - lb for lb, ub in self.bounds]))
 
 
 def Uniform1DCart(conditions, rho_bulk, bounds, ngrid):
     return Structure(conditions, rho_bulk, bounds, ngrid, "uniform")
+
+
+def Uniform3DCart(conditions, rho_bulk, bounds, ngrid):
+    """Uniform3DCart(conditions, rho_bulk, bounds (3x2), ngrid) (base/structure.jl:335-352)."""
+    return Structure(conditions, rho_bulk, bounds, ngrid, "uniform", ndim=3)
 
 
 def TwoPhase1DCart(conditions, rho_bulk, rho_bulk2, bounds, ngrid):
@@ -90,3 +119,66 @@ class Steele:
                     V[i] += pre * (2.0 / 5.0 * (ssi[i] / zz) ** 10 - (ssi[i] / zz) ** 4
                                    - ssi[i] ** 4 / (3.0 * D * (zz + 0.61 * D) ** 3))
         return (V * (1.0 / T)).reshape(-1)
+
+
+class HardSphereMatrix:
+    """Impenetrable spheres of a porous matrix on a periodic 3D grid (no counterpart
+    in the reference): beta V_ext = cap at grid points whose distance (minimum image)
+    to a sphere centre is below its radius, 0 elsewhere; the same for every
+    component.  Any cap above about 100 gives the same fixed point, because the
+    exponent is clamped to [-100, 100] (converge.jl:150)."""
+
+    def __init__(self, centres, radii, cap=500.0):
+        self.centres = np.asarray(centres, dtype=np.float64).reshape(-1, 3)
+        self.radii = np.asarray(radii, dtype=np.float64).reshape(-1)
+        self.cap = float(cap)
+
+    @classmethod
+    def random(cls, n, rmin, rmax, box, seed=0, cap=500.0, max_tries=200000):
+        """n non-overlapping spheres with radii ~ U[rmin, rmax] in the periodic cube
+        [0, box)^3, by random sequential addition from `seed`."""
+        rng = np.random.default_rng(seed)
+        cs, rs = [], []
+        tries = 0
+        while len(cs) < n:
+            tries += 1
+            if tries > max_tries:
+                raise RuntimeError("could not place the spheres without overlap")
+            r = rng.uniform(rmin, rmax)
+            c = rng.uniform(0.0, box, size=3)
+            ok = True
+            for c2, r2 in zip(cs, rs):
+                d = c - c2
+                d -= box * np.round(d / box)
+                if np.dot(d, d) < (r + r2) ** 2:
+                    ok = False
+                    break
+            if ok:
+                cs.append(c)
+                rs.append(r)
+        return cls(np.array(cs), np.array(rs), cap)
+
+    def mask(self, structure):
+        """Boolean (N,) mask of the points inside a sphere, column-major (x fastest)."""
+        xs = [structure.axis_coords(a) for a in range(3)]
+        lens = [ub - lb for lb, ub in structure.bounds]
+        inside = np.zeros(structure.ngrid[::-1], dtype=bool)             # [z, y, x]
+        for c, r in zip(self.centres, self.radii):
+            d2, idx = [], []
+            for a in range(3):                                           # per-axis window of the sphere
+                d = xs[a] - c[a]
+                d -= lens[a] * np.round(d / lens[a])
+                sel = np.nonzero(d * d < r * r)[0]
+                idx.append(sel)
+                d2.append(d[sel] ** 2)
+            if min(len(i) for i in idx) == 0:
+                continue
+            blk = d2[2][:, None, None] + d2[1][None, :, None] + d2[0][None, None, :] < r * r
+            sub = np.ix_(idx[2], idx[1], idx[0])
+            inside[sub] |= blk
+        return inside.reshape(-1)
+
+    def vext(self, structure, model):
+        m = self.mask(structure)
+        nc = len(model)
+        return np.tile(np.where(m, self.cap, 0.0), nc)

@@ -70,7 +70,7 @@ class DFTSystem:
         self.external_field = fields
         self.species = get_species(model, structure)
         self.nc = len(model)
-        self.N = structure.ngrid[0]
+        self.N = structure.npoints
         self.L = model.length_scale()
         self.vext = None
         if fields:
@@ -86,9 +86,10 @@ class DFTSystem:
         if self._dev is None:
             ctx = _lib.default_context(self.options.device)
             p, T = self.structure.conditions
-            lb, ub = self.structure.bounds[0]
+            st = self.structure
             self._dev = _lib.DeviceSystem(
-                ctx, nc=self.nc, ndim=1, ngrid=self.structure.ngrid, lb=[lb], ub=[ub], T=T, p=p, L=self.L,
+                ctx, nc=self.nc, ndim=st.ndim, ngrid=st.ngrid, lb=[b[0] for b in st.bounds],
+                ub=[b[1] for b in st.bounds], T=T, p=p, L=self.L,
                 psi=PSI, m=self.model.segment, d=self.species.size, sigma=self.model.sigma,
                 eps=self.model.epsilon, rho_bulk=self.species.bulk_density, mu_res=self.species.chempot_res,
                 vext=self.vext, precision=self.options.precision_code)
@@ -97,8 +98,9 @@ class DFTSystem:
     def descriptor(self):
         """Plain-value description of the system (the inputs every path receives)."""
         p, T = self.structure.conditions
-        lb, ub = self.structure.bounds[0]
-        return dict(nc=self.nc, nd=1, ngrid=[self.N], lb=[lb], ub=[ub], T=T, L=self.L,
+        st = self.structure
+        return dict(nc=self.nc, nd=st.ndim, ngrid=list(st.ngrid), lb=[b[0] for b in st.bounds],
+                    ub=[b[1] for b in st.bounds], T=T, L=self.L,
                     m=self.model.segment.copy(), d=self.species.size.copy(), sigma=self.model.sigma.copy(),
                     eps=self.model.epsilon.copy(), mu_res=self.species.chempot_res.copy(),
                     rho_bulk=self.species.bulk_density.copy(), vext=None if self.vext is None else self.vext.copy())
@@ -197,11 +199,40 @@ def simpson(f, dz):
     return float(np.sum(c * f) * (dz / 3.0))
 
 
+def integration_weights(structure):
+    """Weights w with _int(f) = w . f over the whole grid (column-major, x fastest)."""
+    w = None
+    for n, dz in zip(structure.ngrid, structure.dzs()):
+        k = np.arange(1, n + 1)
+        c = np.where(k % 2 == 0, 4.0, 2.0)
+        c[0] = 1.0
+        c[-1] = 1.0
+        c = c * (dz / 3.0)
+        w = c if w is None else np.outer(c, w).reshape(-1)
+    return w
+
+
+def integrate(structure, f):
+    """_int over every axis of the structure (src/utils/integrals.jl:28-40), f column-major."""
+    if structure.ndim == 1:
+        return simpson(f, structure.dz())
+    f = np.asarray(f, dtype=np.float64).reshape(structure.ngrid[::-1])      # [z, y, x]
+    out = f
+    for a, dz in enumerate(structure.dzs()):                                # x first
+        n = structure.ngrid[a]
+        k = np.arange(1, n + 1)
+        c = np.where(k % 2 == 0, 4.0, 2.0)
+        c[0] = 1.0
+        c[-1] = 1.0
+        out = np.tensordot(out, c * (dz / 3.0), axes=([out.ndim - 1], [0]))
+    return float(out)
+
+
 def F_ideal(system, rho):
     T = system.structure.conditions[1]
     rho = np.asarray(rho, dtype=np.float64).reshape(system.nc, system.N)
     phi = np.sum(NA * rho * (np.log(rho) - 1.5 * math.log(T) - 1.0), axis=0)
-    return simpson(phi, system.structure.dz())
+    return integrate(system.structure, phi)
 
 
 def free_energy(system, rho):

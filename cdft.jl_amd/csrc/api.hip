@@ -14,6 +14,7 @@
 #include "kernels.h"
 #include "plain.hip"
 #include "batched.hip"
+#include "fft3d.hip"
 
 using namespace cdft;
 
@@ -93,13 +94,158 @@ void free_sys_buffers(cdft_sys *s)
     for (double **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (s->H64) { (void)hipFree(s->H64); s->H64 = nullptr; }
+    double **b3[] = {&s->G3, &s->A3, &s->B3, &s->S3};
+    for (double **b : b3)
+        if (*b) { (void)hipFree(*b); *b = nullptr; }
+}
+
+// Composite Simpson of _int over nd axes (integrals.jl:28-40): per-axis weights
+// 1, 4, 2, ..., 1 times dz_a / 3, f column-major over ngrid.
+double simpson_nd(const double *f, int nd, const int *ngrid, const double *dz)
+{
+    if (nd == 1) return simpson1d(f, ngrid[0], dz[0]);
+    auto wgt = [](int k, int n) { return (k == 0 || k == n - 1) ? 1.0 : ((k % 2 == 1) ? 4.0 : 2.0); };
+    double s = 0.0;
+    for (int i2 = 0; i2 < ngrid[2]; i2++)
+        for (int i1 = 0; i1 < ngrid[1]; i1++) {
+            const double w12 = wgt(i1, ngrid[1]) * wgt(i2, ngrid[2]);
+            const double *row = f + ((size_t)i2 * ngrid[1] + i1) * ngrid[0];
+            double r = 0.0;
+            for (int i0 = 0; i0 < ngrid[0]; i0++) r += wgt(i0, ngrid[0]) * row[i0];
+            s += w12 * r;
+        }
+    return s * (dz[0] / 3.0) * (dz[1] / 3.0) * (dz[2] / 3.0);
+}
+
+// Arguments of k_update: per-field parts (1D) or the finished transposed sum (3D).
+struct UpdSrc {
+    const double *dphi, *part;
+    int F;
+};
+UpdSrc upd_src(const cdft_sys *s)
+{
+    if (s->nd == 3) return {s->part, nullptr, 1};
+    return {s->dphi, s->part, s->F};
+}
+
+// ---- 3D passes: one line FFT along axis AX over tiles of TB3 lines
+template <typename T, int AX, int IN, int OUT, int EPT>
+void fft3_launch(const cdft_sys *s, const Pass3 &p)
+{
+    const Grid3 &G = s->g3;
+    const int L = G.n[AX];
+    const size_t lds = (size_t)L * TB3 * sizeof(cx<T>);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_fft3<T, AX, IN, OUT, EPT>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    long nblk;
+    if (AX == 0) nblk = (G.N / G.n[0]) / TB3;
+    else if (AX == 1) nblk = (long)(G.n[0] / TB3) * G.n[2];
+    else nblk = (long)(G.n[0] / TB3) * G.n[1];
+    hipLaunchKernelGGL((k_fft3<T, AX, IN, OUT, EPT>), dim3((unsigned)nblk), dim3(256), lds, s->ctx->stream, G, p);
+}
+
+template <typename T, int AX, int IN, int OUT>
+void fft3_pass(const cdft_sys *s, Pass3 p, bool inverse)
+{
+    p.inverse = inverse ? 1 : 0;
+    p.tw = reinterpret_cast<const cx<double> *>(s->ctx->twiddles.at(s->g3.n[AX]));
+    switch (s->g3.n[AX]) {
+    case 512: fft3_launch<T, AX, IN, OUT, 8>(s, p); break;
+    case 256: fft3_launch<T, AX, IN, OUT, 4>(s, p); break;
+    case 128: fft3_launch<T, AX, IN, OUT, 2>(s, p); break;
+    default: fft3_launch<T, AX, IN, OUT, 1>(s, p); break;
+    }
+}
+
+// Field pairs of the 3D slots (PCSAFT.jl:55-61): (S, V), (v_x, v_y), (v_z, zeta), (lambda, disp).
+constexpr int kPair3[4][2] = {{1, 2}, {3, 4}, {5, 6}, {7, 8}};
+inline bool is_vec3(int f) { return f >= 3 && f <= 5; }
+
+Pass3 pair_pass(const cdft_sys *s, int c, int q)
+{
+    Pass3 p;
+    std::memset(&p, 0, sizeof(p));
+    const int f1 = kPair3[q][0], f2 = kPair3[q][1];
+    p.g1 = s->G3 + (size_t)(c * 8 + f1 - 1) * s->N;
+    p.g2 = s->G3 + (size_t)(c * 8 + f2 - 1) * s->N;
+    p.v1 = is_vec3(f1);
+    p.v2 = is_vec3(f2);
+    return p;
+}
+
+// Weighted densities of src (layout [c N + k]) into nout[(c F + f) N + k], f >= 1.
+void launch_weighted3d(cdft_sys *s, const double *src, double *nout)
+{
+    using C = cx<double>;
+    const int N = s->N, F = s->F;
+    for (int c = 0; c < s->nc; c++) {
+        Pass3 p;
+        std::memset(&p, 0, sizeof(p));
+        p.in = src + (size_t)c * N;
+        p.out = s->A3;
+        fft3_pass<double, 0, IN_RHO, OUT_CPLX>(s, p, false);
+        p.in = s->A3;
+        fft3_pass<double, 1, IN_CPLX, OUT_CPLX>(s, p, false);
+        fft3_pass<double, 2, IN_CPLX, OUT_CPLX>(s, p, false);
+        for (int q = 0; q < 4; q++) {
+            Pass3 pq = pair_pass(s, c, q);
+            pq.in = s->A3;
+            pq.out = s->B3;
+            fft3_pass<double, 2, IN_SPEC, OUT_CPLX>(s, pq, true);
+            pq.in = s->B3;
+            fft3_pass<double, 1, IN_CPLX, OUT_CPLX>(s, pq, true);
+            pq.out = nout + (size_t)(c * F + kPair3[q][0]) * N;
+            pq.out2 = nout + (size_t)(c * F + kPair3[q][1]) * N;
+            pq.scale = s->NA / (double)N;
+            fft3_pass<double, 0, IN_CPLX, OUT_FIELDS>(s, pq, true);
+        }
+    }
+    (void)sizeof(C);
+}
+
+// dtot[c N + k] = dphi[(c F) N + k] + sum_f (transposed convolution of dphi_f).
+void launch_transposed3d(cdft_sys *s, const double *dphi, double *dtot)
+{
+    const int N = s->N, F = s->F;
+    for (int c = 0; c < s->nc; c++) {
+        for (int q = 0; q < 4; q++) {
+            Pass3 pq = pair_pass(s, c, q);
+            pq.in = dphi + (size_t)(c * F + kPair3[q][0]) * N;
+            pq.in2 = dphi + (size_t)(c * F + kPair3[q][1]) * N;
+            pq.out = s->B3;
+            fft3_pass<double, 0, IN_PAIR, OUT_CPLX>(s, pq, false);
+            pq.in = s->B3;
+            fft3_pass<double, 1, IN_CPLX, OUT_CPLX>(s, pq, false);
+            pq.out = s->S3;
+            pq.first = (q == 0);
+            fft3_pass<double, 2, IN_CPLX, OUT_ACCUM>(s, pq, false);
+        }
+        Pass3 p;
+        std::memset(&p, 0, sizeof(p));
+        p.in = s->S3;
+        p.out = s->S3;
+        fft3_pass<double, 2, IN_CPLX, OUT_CPLX>(s, p, true);
+        fft3_pass<double, 1, IN_CPLX, OUT_CPLX>(s, p, true);
+        p.out = dtot + (size_t)c * N;
+        p.base = dphi + (size_t)(c * F) * N;
+        p.scale = 1.0 / (double)N;
+        fft3_pass<double, 0, IN_CPLX, OUT_REAL>(s, p, true);
+    }
 }
 
 template <int NC>
 void launch_phi(cdft_sys *s, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_phi<NC, 1>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, s->n, s->dphi, s->fval, s->N,
-                       s->NA, s->pc);
+    if (s->nd == 3)
+        hipLaunchKernelGGL((k_phi<NC, 3>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, s->n, s->dphi, s->fval,
+                           s->N, s->NA, s->pc);
+    else
+        hipLaunchKernelGGL((k_phi<NC, 1>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, s->n, s->dphi, s->fval,
+                           s->N, s->NA, s->pc);
 }
 
 void launch_phi_any(cdft_sys *s)
@@ -117,6 +263,10 @@ void launch_phi_any(cdft_sys *s)
 // on the vector slot.
 void launch_transposed(cdft_sys *s)
 {
+    if (s->nd == 3) {
+        launch_transposed3d(s, s->dphi, s->part);
+        return;
+    }
     int N = s->N, F = s->F;
     hipLaunchKernelGGL(k_circ1d<double>, dim3(nblocks(N), s->nc * (F - 1)), dim3(256), 0, s->ctx->stream, s->dphi,
                        (long)F * N, (long)N, s->h, s->part, (long)(F - 1) * N, (long)N, 1, N, F, 1.0, 1);
@@ -125,6 +275,10 @@ void launch_transposed(cdft_sys *s)
 // Weighted densities of rho (s->rho) into s->n.
 void launch_weighted(cdft_sys *s)
 {
+    if (s->nd == 3) {
+        launch_weighted3d(s, s->rho, s->n);
+        return;
+    }
     int N = s->N, F = s->F;
     hipLaunchKernelGGL(k_circ1d<double>, dim3(nblocks(N), s->nc * (F - 1)), dim3(256), 0, s->ctx->stream, s->rho,
                        (long)N, 0L, s->h, s->n, (long)F * N, (long)N, 0, N, F, s->NA, 0);
@@ -141,8 +295,12 @@ void launch_functional(cdft_sys *s)
 template <int NC>
 void launch_phi_hvp(cdft_sys *s, const double *v, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_phi_hvp<NC, 1>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, v, s->n, s->dphi, s->N,
-                       s->NA, s->pc);
+    if (s->nd == 3)
+        hipLaunchKernelGGL((k_phi_hvp<NC, 3>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, v, s->n, s->dphi,
+                           s->N, s->NA, s->pc);
+    else
+        hipLaunchKernelGGL((k_phi_hvp<NC, 1>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, v, s->n, s->dphi,
+                           s->N, s->NA, s->pc);
 }
 
 // Hessian-vector product of F_res at s->rho (weighted densities already in s->n)
@@ -151,9 +309,13 @@ void launch_hvp(cdft_sys *s, const double *v)
 {
     hipStream_t st = s->ctx->stream;
     int N = s->N, F = s->F, nc = s->nc;
-    dim3 g(nblocks(N), nc * (F - 1));
-    hipLaunchKernelGGL(k_circ1d<double>, g, dim3(256), 0, st, v, (long)N, 0L, s->h, s->dphi, (long)F * N, (long)N, 0,
-                       N, F, s->NA, 0);
+    if (s->nd == 3) {
+        launch_weighted3d(s, v, s->dphi);
+    } else {
+        dim3 g(nblocks(N), nc * (F - 1));
+        hipLaunchKernelGGL(k_circ1d<double>, g, dim3(256), 0, st, v, (long)N, 0L, s->h, s->dphi, (long)F * N,
+                           (long)N, 0, N, F, s->NA, 0);
+    }
     switch (nc) {
     case 1: launch_phi_hvp<1>(s, v, st); break;
     case 2: launch_phi_hvp<2>(s, v, st); break;
@@ -209,9 +371,20 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
     if (!ctx || !d || !out) return fail(ctx, CDFT_ERR_ARG, "cdft_system_create: NULL argument");
     *out = nullptr;
     if (d->nc < 1 || d->nc > 4) return fail(ctx, CDFT_ERR_UNSUPPORTED, "nc must be in 1..4");
-    if (d->ndim != 1) return fail(ctx, CDFT_ERR_UNSUPPORTED, "only 1D Cartesian grids are supported");
-    if (d->ngrid[0] < 3) return fail(ctx, CDFT_ERR_ARG, "ngrid must be >= 3");
-    if (!(d->ub[0] > d->lb[0])) return fail(ctx, CDFT_ERR_ARG, "bounds must satisfy ub > lb");
+    if (d->ndim != 1 && d->ndim != 3) return fail(ctx, CDFT_ERR_UNSUPPORTED, "only 1D and 3D Cartesian grids are supported");
+    if (d->ndim == 1 && d->ngrid[0] < 3) return fail(ctx, CDFT_ERR_ARG, "ngrid must be >= 3");
+    if (d->ndim == 3) {
+        long N3 = 1;
+        for (int a = 0; a < 3; a++) {
+            const int g = d->ngrid[a];
+            if (g < 16 || g > 512 || (g & (g - 1)))
+                return fail(ctx, CDFT_ERR_UNSUPPORTED, "3D grids need power-of-two axes of 16..512 points");
+            N3 *= g;
+        }
+        if (N3 > (1L << 27)) return fail(ctx, CDFT_ERR_UNSUPPORTED, "3D grid larger than 2^27 points");
+    }
+    for (int a = 0; a < d->ndim; a++)
+        if (!(d->ub[a] > d->lb[a])) return fail(ctx, CDFT_ERR_ARG, "bounds must satisfy ub > lb");
     if (!(d->T > 0) || !(d->L > 0)) return fail(ctx, CDFT_ERR_ARG, "T and L must be positive");
     if (!d->m || !d->d || !d->sigma || !d->eps || !d->rho_bulk || !d->mu_res)
         return fail(ctx, CDFT_ERR_ARG, "parameter array is NULL");
@@ -225,11 +398,14 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
     s->ctx = ctx;
     s->nc = d->nc;
     s->nd = d->ndim;
-    s->N = d->ngrid[0];
+    s->N = 1;
+    for (int a = 0; a < d->ndim; a++) {
+        s->ngrid[a] = d->ngrid[a];
+        s->lb[a] = d->lb[a];
+        s->ub[a] = d->ub[a];
+        s->N *= d->ngrid[a];
+    }
     s->F = 6 + d->ndim;
-    s->ngrid[0] = d->ngrid[0];
-    s->lb[0] = d->lb[0];
-    s->ub[0] = d->ub[0];
     s->T = d->T;
     s->p = d->p;
     s->L = d->L;
@@ -264,9 +440,13 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
     auto alloc = [&](double **p, size_t count) { return hipMalloc((void **)p, sizeof(double) * count); };
     hipError_t e = hipSuccess;
     double *ddev = nullptr;
-    if ((e = alloc(&s->h, (size_t)nc * F * N)) != hipSuccess || (e = alloc(&s->rho, n)) != hipSuccess ||
+    const bool d3 = (s->nd == 3);
+    if ((!d3 && (e = alloc(&s->h, (size_t)nc * F * N)) != hipSuccess) || (e = alloc(&s->rho, n)) != hipSuccess ||
         (e = alloc(&s->n, (size_t)nc * F * N)) != hipSuccess || (e = alloc(&s->dphi, (size_t)nc * F * N)) != hipSuccess ||
-        (e = alloc(&s->part, (size_t)nc * (F - 1) * N)) != hipSuccess || (e = alloc(&s->fval, N)) != hipSuccess ||
+        (e = alloc(&s->part, d3 ? (size_t)n : (size_t)nc * (F - 1) * N)) != hipSuccess ||
+        (e = alloc(&s->fval, N)) != hipSuccess ||
+        (d3 && ((e = alloc(&s->G3, (size_t)8 * nc * N)) != hipSuccess || (e = alloc(&s->A3, (size_t)2 * N)) != hipSuccess ||
+                (e = alloc(&s->B3, (size_t)2 * N)) != hipSuccess || (e = alloc(&s->S3, (size_t)2 * N)) != hipSuccess)) ||
         (e = alloc(&s->out, n)) != hipSuccess || (e = alloc(&s->sol, n)) != hipSuccess ||
         (e = alloc(&s->gx, n)) != hipSuccess || (e = alloc(&s->df, n)) != hipSuccess ||
         (e = alloc(&s->dg, n)) != hipSuccess || (e = alloc(&s->res, n)) != hipSuccess ||
@@ -293,16 +473,45 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
             return fail(ctx, CDFT_ERR_HIP, std::string("vext upload failed: ") + hipGetErrorString(e));
         }
     }
-    // real-space kernels from the analytic weights
+    // real-space kernels (1D) or tabulated spectra (3D) from the analytic weights
     double dd[CDFT_MAXC];
     for (int i = 0; i < nc; i++) dd[i] = P.dd[i];
-    double mult = ((double)N / (s->ub[0] - s->lb[0])) / (double)N;   // fftfreq spacing
-    e = hipMemcpy(ddev, dd, sizeof(double) * nc, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemsetAsync(s->h, 0, sizeof(double) * nc * F * N, ctx->stream);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_setup_h1d, dim3(nblocks(N), nc * F), dim3(256), 0, ctx->stream, s->h, N, nc, ddev,
-                           s->psi, mult, s->L);
-        e = hipGetLastError();
+    if (d3) {
+        Grid3 &G = s->g3;
+        std::memset(&G, 0, sizeof(G));
+        G.N = N;
+        for (int a = 0; a < 3; a++) {
+            G.n[a] = s->ngrid[a];
+            while ((1 << G.lg[a]) < G.n[a]) G.lg[a]++;
+            G.mult[a] = s->L / (s->ub[a] - s->lb[a]);             // fftfreq(n, n/(ub-lb)) * L
+            if (!ctx->twiddles.count(G.n[a])) {
+                double *t64 = nullptr;
+                if ((e = alloc(&t64, (size_t)2 * G.n[a])) == hipSuccess) {
+                    hipLaunchKernelGGL(k_setup_tw, dim3(nblocks(G.n[a])), dim3(256), 0, ctx->stream, G.n[a], t64);
+                    ctx->twiddles[G.n[a]] = t64;
+                }
+            }
+        }
+        const double twopi = 2.0 * kPi;
+        for (int c = 0; c < nc; c++) {
+            G.R[c][0] = twopi * 0.5 * dd[c];
+            G.R[c][1] = twopi * dd[c];
+            G.R[c][2] = twopi * dd[c] * s->psi;
+        }
+        for (int c = 0; c < nc && e == hipSuccess; c++)
+            for (int f = 1; f <= 8; f++)
+                hipLaunchKernelGGL(k_setup_omega3, dim3(nblocks(N)), dim3(256), 0, ctx->stream, G, c, f,
+                                   s->G3 + (size_t)(c * 8 + f - 1) * N);
+        if (e == hipSuccess) e = hipGetLastError();
+    } else {
+        double mult = ((double)N / (s->ub[0] - s->lb[0])) / (double)N;   // fftfreq spacing
+        e = hipMemcpy(ddev, dd, sizeof(double) * nc, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemsetAsync(s->h, 0, sizeof(double) * nc * F * N, ctx->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_setup_h1d, dim3(nblocks(N), nc * F), dim3(256), 0, ctx->stream, s->h, N, nc, ddev,
+                               s->psi, mult, s->L);
+            e = hipGetLastError();
+        }
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(ddev);
@@ -333,8 +542,8 @@ int cdft_functional_eval(cdft_sys *s, const double *rho, double *dfdrho, double 
     HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
     HIPCHECK_CTX(ctx, hipMemcpyAsync(s->rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, st));
     launch_functional(s);
-    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, s->dphi, s->part, (const double *)nullptr,
-                       s->N, s->nc, s->F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
+    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part,
+                       (const double *)nullptr, s->N, s->nc, upd_src(s).F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
                        (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
                        (const double *)nullptr, (const double *)nullptr);
     HIPCHECK_CTX(ctx, hipGetLastError());
@@ -347,8 +556,9 @@ int cdft_functional_eval(cdft_sys *s, const double *rho, double *dfdrho, double 
     HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
     if (fval) std::memcpy(fval, fv.data(), sizeof(double) * s->N);
     if (f_res) {
-        double dz = (s->ub[0] - s->lb[0]) / s->N;                     // structure_dz
-        *f_res = simpson1d(fv.data(), s->N, dz) / (s->L * s->L * s->L);  // F_res, models.jl:86-91
+        double dz[3];
+        for (int a = 0; a < s->nd; a++) dz[a] = (s->ub[a] - s->lb[a]) / s->ngrid[a];   // structure_dz
+        *f_res = simpson_nd(fv.data(), s->nd, s->ngrid, dz) / (s->L * s->L * s->L);    // F_res, models.jl:86-91
     }
     return CDFT_OK;
 }
@@ -394,7 +604,8 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     auto evalG = [&](int mode, double b) -> int {
         if (adj) launch_hvp(s, adj->vbuf);
         else launch_functional(s);
-        hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, s->dphi, s->part, s->vext, s->N, s->nc, s->F,
+        hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part, s->vext, s->N, s->nc,
+                           upd_src(s).F,
                            s->cpd, s->out, s->sol, s->gx, s->res, s->df, s->dg, b, mode, s->red, nb,
                            adj ? adj->g : nullptr, adj ? (const double *)s->rho : nullptr);
         hipLaunchKernelGGL(k_finalize, dim3(3), dim3(256), 0, st, s->red, nb, s->fin);
@@ -584,8 +795,8 @@ int cdft_phi_vjp(cdft_sys *s, const double *rho, const double *cot, double *out)
     launch_phi_any(s);
     hipLaunchKernelGGL(k_scale_slots, dim3(nblocks(s->N)), dim3(256), 0, st, s->dphi, cotd, s->N, s->nc * s->F);
     launch_transposed(s);
-    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, s->dphi, s->part, (const double *)nullptr,
-                       s->N, s->nc, s->F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
+    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part,
+                       (const double *)nullptr, s->N, s->nc, upd_src(s).F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
                        (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
                        (const double *)nullptr, (const double *)nullptr);
     HIPCHECK_CTX(ctx, hipGetLastError());
@@ -608,8 +819,8 @@ int cdft_hvp(cdft_sys *s, const double *rho, const double *v, double *out)
     HIPCHECK_CTX(ctx, hipMemcpyAsync(vd, v, sizeof(double) * n, hipMemcpyHostToDevice, st));
     launch_weighted(s);
     launch_hvp(s, vd);
-    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, s->dphi, s->part, (const double *)nullptr,
-                       s->N, s->nc, s->F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
+    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part,
+                       (const double *)nullptr, s->N, s->nc, upd_src(s).F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
                        (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
                        (const double *)nullptr, (const double *)nullptr);
     HIPCHECK_CTX(ctx, hipGetLastError());
@@ -636,8 +847,8 @@ int cdft_adjoint_solve(cdft_sys *s, const double *rho, const double *g, double *
     HIPCHECK_CTX(ctx, hipMemcpyAsync(gd, g, sizeof(double) * n, hipMemcpyHostToDevice, st));
     // dF_res/drho at rho*, then the mask of points where the exponent clamp is inactive
     launch_functional(s);
-    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, s->dphi, s->part, (const double *)nullptr,
-                       s->N, s->nc, s->F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
+    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part,
+                       (const double *)nullptr, s->N, s->nc, upd_src(s).F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
                        (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
                        (const double *)nullptr, (const double *)nullptr);
     hipLaunchKernelGGL(k_clamp_mask, dim3(nblocks(n)), dim3(256), 0, st, s->out, s->vext, s->cpd, s->N, s->nc, maskd);
@@ -654,7 +865,9 @@ int cdft_surface_tension(cdft_sys *s, const double *rho, const double *mu, doubl
     double fres = 0.0;
     int rc = cdft_functional_eval(s, rho, dfd.data(), &fres, nullptr);
     if (rc != CDFT_OK) return rc;
-    const double dz = (s->ub[0] - s->lb[0]) / s->N;
+    double dz[3];
+    for (int a = 0; a < s->nd; a++) dz[a] = (s->ub[a] - s->lb[a]) / s->ngrid[a];
+    auto integ = [&](const double *f) { return simpson_nd(f, s->nd, s->ngrid, dz); };
     std::vector<double> phi(s->N, 0.0), one(s->N, 1.0);
     const double NAv = 6.02214076e23, kB = 1.380649e-23;
     for (int c = 0; c < s->nc; c++)
@@ -663,18 +876,21 @@ int cdft_surface_tension(cdft_sys *s, const double *rho, const double *mu, doubl
             phi[i] += NAv * r * (std::log(r) - 1.5 * std::log(s->T) - 1.0);   // f_ideal, BasicIdeal.jl:10-39
         }
     double chem = 0.0;
-    for (int c = 0; c < s->nc; c++) chem += mu[c] * simpson1d(rho + (size_t)c * s->N, s->N, dz);
-    const double F = simpson1d(phi.data(), s->N, dz) + fres;
-    *out = F * kB * s->T - chem + p * simpson1d(one.data(), s->N, dz);   // surface_tension.jl:31-53
+    for (int c = 0; c < s->nc; c++) chem += mu[c] * integ(rho + (size_t)c * s->N);
+    const double F = integ(phi.data()) + fres;
+    *out = F * kB * s->T - chem + p * integ(one.data());   // surface_tension.jl:31-53
     return CDFT_OK;
 }
 
 int cdft_adsorption(cdft_sys *s, const double *rho, double *out)
 {
     if (!s || !rho || !out) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_adsorption: NULL argument");
-    const double V = s->ub[0] - s->lb[0];
-    const double dz = V / s->N;
-    for (int c = 0; c < s->nc; c++) out[c] = simpson1d(rho + (size_t)c * s->N, s->N, dz) / V;   // adsorption.jl:22-32
+    double V = 1.0, dz[3];
+    for (int a = 0; a < s->nd; a++) {
+        V *= s->ub[a] - s->lb[a];                                   // _domain_volume, adsorption.jl:15
+        dz[a] = (s->ub[a] - s->lb[a]) / s->ngrid[a];
+    }
+    for (int c = 0; c < s->nc; c++) out[c] = simpson_nd(rho + (size_t)c * s->N, s->nd, s->ngrid, dz) / V;   // adsorption.jl:22-32
     return CDFT_OK;
 }
 
@@ -734,6 +950,7 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         if (sys[i]->ctx != ctx) return fail(ctx, CDFT_ERR_ARG, "all systems must belong to the context");
         if (sys[i]->nc != nc) return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched systems must share nc");
         if (sys[i]->precision != prec) return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched systems must share precision");
+        if (sys[i]->nd != 1) return fail(ctx, CDFT_ERR_UNSUPPORTED, "the batched path takes 1D systems");
         for (int j = 0; j < i; j++)
             if (sys[j] == sys[i]) return fail(ctx, CDFT_ERR_ARG, "a system appears twice in the batch");
     }

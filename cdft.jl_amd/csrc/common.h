@@ -13,6 +13,17 @@
 #define CDFT_MAXM 8          // largest Anderson depth supported
 #define CDFT_NQ 16           // reduction slots per vector pass
 
+namespace cdft {
+// Per-system constants of the 3D weights (fft3d.hip).
+struct Grid3 {
+    int n[3];                 // points per axis (powers of two)
+    int lg[3];                // log2 of n
+    long N;                   // n0 * n1 * n2
+    double mult[3];           // fftfreq spacing times L: omega_a = k_signed * mult[a]
+    double R[CDFT_MAXC][3];   // 2 pi * width for widths d/2, d, psi d (reduced)
+};
+}  // namespace cdft
+
 struct cdft_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -52,6 +63,9 @@ struct cdft_sys {
     // batched path: real half-spectra of the kernels on the FFT ring of length M
     int M = 0;
     double *H64 = nullptr;
+    // 3D grids: tabulated spectra G [(c*8 + f-1)*N + k] and three complex work arrays
+    cdft::Grid3 g3;
+    double *G3 = nullptr, *A3 = nullptr, *B3 = nullptr, *S3 = nullptr;
     double *red = nullptr;     // block partials
     double *fin = nullptr;     // finalized reductions [CDFT_NQ]
     int nblk = 0;

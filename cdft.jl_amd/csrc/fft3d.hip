@@ -1,0 +1,302 @@
+// fft3d.hip -- weighted densities and transposed convolutions on 3D Cartesian
+// grids (power-of-two axes) with line FFTs in LDS.
+//
+// The reference convolves every (field, component) with its analytic kernel by an
+// FFT, a multiply and an inverse FFT (src/utils/integrals.jl:88-96,
+// src/fields/weighted_densities.jl:118-163,345-383).  Here, per component c:
+//   P   = FFT3(rho_c)                                         3 passes
+//   for each field pair (f1, f2):
+//     n_f1 + i n_f2 = IFFT3(P Z) N_A L^3 / N                  3 passes; the multiply is
+//                                                             fused into the first, the
+//                                                             field split into the last
+//   for each pair of gradients (u1, u2) = dPhi/dn_(f1, f2):
+//     S += conj(Z) FFT3(u1 + i u2)                            3 passes; accumulate fused
+//   dF/drho_c = dPhi/dn_rho + Re IFFT3(S) / N                 3 passes; sum fused
+// with Z = H_f1 + i H_f2 and H_f the Hermitian part of the reference's weight (the
+// part real() keeps): H = G (scalar weights) or i G (vector weights), G real and
+// tabulated once per system.  For real kernels h_f,
+// Re IFFT(conj(Z) FFT(u1 + i u2)) = h1^T u1 + h2^T u2, so no conjugate-symmetric
+// separation is needed; conj(H) is the transposed operator (the vector weights'
+// transpose is their negative, weighted_densities.jl:381).
+#pragma once
+#include "common.h"
+
+namespace cdft {
+
+template <typename T>
+struct cx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline cx<T> cxmul(cx<T> a, cx<T> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+
+enum { IN_CPLX = 0, IN_RHO = 1, IN_PAIR = 2, IN_SPEC = 3 };
+enum { OUT_CPLX = 0, OUT_FIELDS = 1, OUT_ACCUM = 2, OUT_REAL = 3 };
+
+struct Pass3 {
+    const void *in;           // IN_CPLX / IN_SPEC: complex T; IN_RHO: double rho; IN_PAIR: T fields
+    const void *in2;          // IN_PAIR: second field
+    void *out;                // OUT_CPLX / OUT_ACCUM: complex T; OUT_FIELDS: T field 1; OUT_REAL: double
+    void *out2;               // OUT_FIELDS: T field 2
+    const double *base;       // OUT_REAL: added to the result (dPhi/dn_rho), or null
+    const cx<double> *tw;     // W_L^t for this axis, t < L
+    const double *g1, *g2;    // IN_SPEC / OUT_ACCUM: real spectra G of the pair's fields
+    int v1, v2;               // 1 where the field is a vector weight (H = i G)
+    int first;                // OUT_ACCUM: overwrite instead of accumulate
+    int inverse;
+    double scale;             // OUT_FIELDS / OUT_REAL
+};
+
+// Z = H1 + i H2 at one frequency from the tabulated G (H = G or i G).
+template <typename T>
+__device__ __attribute__((always_inline)) inline cx<T> pair_z(const Pass3 &p, long g)
+{
+    const double a = p.g1[g], b = p.g2[g];
+    const cx<double> h1 = p.v1 ? cx<double>{0.0, a} : cx<double>{a, 0.0};
+    const cx<double> h2 = p.v2 ? cx<double>{0.0, b} : cx<double>{b, 0.0};
+    return {T(h1.x - h2.y), T(h1.y + h2.x)};
+}
+
+// Field table of PC-SAFT in 3D (PCSAFT.jl:55-61): kind 1 int rho dz, 2 int rho z^2 dz,
+// 3+j vector component j; width index 0: d/2, 1: d, 2: psi d.
+__device__ __attribute__((always_inline)) inline void field3(int f, int &kind, int &w)
+{
+    switch (f) {
+    case 1: kind = 1; w = 0; break;
+    case 2: kind = 2; w = 0; break;
+    case 3: kind = 3; w = 0; break;
+    case 4: kind = 4; w = 0; break;
+    case 5: kind = 5; w = 0; break;
+    case 6: kind = 2; w = 1; break;
+    case 7: kind = 1; w = 1; break;
+    default: kind = 2; w = 2; break;
+    }
+}
+
+// Hermitian part of Omega_f at grid index (i0, i1, i2), reduced units
+// (weighted_densities.jl:48-69, :210-227; frequencies structure.jl:153-185).
+template <typename T>
+__device__ inline cx<T> omega3(const Grid3 &G, int c, int f, int i0, int i1, int i2)
+{
+    int kind, wi;
+    field3(f, kind, wi);
+    const int ii[3] = {i0, i1, i2};
+    double w[3], wb2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int nn = G.n[a];
+        int ks = (ii[a] < (nn + 1) / 2) ? ii[a] : ii[a] - nn;
+        w[a] = (double)ks * G.mult[a];
+        wb2 += w[a] * w[a];
+    }
+    const double wb = sqrt(wb2);
+    const double R = G.R[c][wi];
+    const double twopi = 2.0 * kPi, fourpi = 4.0 * kPi, tp3 = twopi * twopi * twopi;
+    if (kind == 1) {
+        double om = (wb == 0.0 ? 2.0 * R : 2.0 * sin(wb * R) / wb) / twopi;
+        return {T(om), T(0)};
+    }
+    if (kind == 2) {
+        double om = (wb == 0.0 ? fourpi * (R * R * R) / 3.0
+                               : fourpi / (wb * wb * wb) * (sin(wb * R) - R * wb * cos(wb * R))) / tp3;
+        return {T(om), T(0)};
+    }
+    // vector component j: i W; no Hermitian part at k = 0 or on the Nyquist plane of axis j
+    const int j = kind - 3;
+    if (wb == 0.0 || 2 * ii[j] == G.n[j]) return {T(0), T(0)};
+    double W = fourpi * w[j] / (wb * wb * wb) * (sin(wb * R) - R * wb * cos(wb * R)) / tp3;
+    return {T(0), T(W)};
+}
+
+// One FFT pass along axis AX over a tile of TB lines held in LDS.  Lines along x
+// are contiguous (LDS [b][j], j fastest); along y and z a tile is TB consecutive x
+// values (LDS [j][b], b fastest), so global accesses are coalesced and LDS accesses
+// of consecutive lanes are consecutive.  EPT = radix-4 butterflies per thread.
+constexpr int TB3 = 16;
+
+template <typename T, int AX, int IN, int OUT, int EPT>
+__global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
+{
+    constexpr int TB = TB3;
+    extern __shared__ __align__(16) unsigned char smem3[];
+    cx<T> *a = reinterpret_cast<cx<T> *>(smem3);
+    const int tid = threadIdx.x;
+    const int L = G.n[AX], lg = G.lg[AX];
+    const int n0 = G.n[0], n1 = G.n[1];
+    long base, lstride, bstride;
+    int x0 = 0, y0 = 0, z0 = 0;
+    long line0 = 0;
+    if (AX == 0) {
+        line0 = (long)blockIdx.x * TB;
+        base = line0 * n0;
+        lstride = 1;
+        bstride = n0;
+    } else if (AX == 1) {
+        const int tiles_x = n0 / TB;
+        x0 = (blockIdx.x % tiles_x) * TB;
+        z0 = blockIdx.x / tiles_x;
+        base = x0 + (long)n0 * n1 * z0;
+        lstride = n0;
+        bstride = 1;
+    } else {
+        const int tiles_x = n0 / TB;
+        x0 = (blockIdx.x % tiles_x) * TB;
+        y0 = blockIdx.x / tiles_x;
+        base = x0 + (long)n0 * y0;
+        lstride = (long)n0 * n1;
+        bstride = 1;
+    }
+    auto lds = [&](int j, int b) -> int { return (AX == 0) ? b * L + j : j * TB + b; };
+    auto split = [&](int e, int len, int &j, int &b) {     // element e of a len x TB tile
+        if (AX == 0) { j = e % len; b = e / len; } else { b = e % TB; j = e / TB; }
+    };
+    const int total = L * TB;
+    // ---- load
+    for (int e = tid; e < total; e += 256) {
+        int j, b;
+        split(e, L, j, b);
+        const long g = base + (long)j * lstride + (long)b * bstride;
+        cx<T> v;
+        if (IN == IN_CPLX) {
+            v = reinterpret_cast<const cx<T> *>(p.in)[g];
+        } else if (IN == IN_RHO) {
+            v = {T(reinterpret_cast<const double *>(p.in)[g]), T(0)};
+        } else if (IN == IN_PAIR) {
+            v = {reinterpret_cast<const T *>(p.in)[g], reinterpret_cast<const T *>(p.in2)[g]};
+        } else {  // IN_SPEC: P Z
+            const cx<T> s = reinterpret_cast<const cx<T> *>(p.in)[g];
+            v = cxmul(s, pair_z<T>(p, g));
+        }
+        a[lds(j, b)] = v;
+    }
+    __syncthreads();
+    // ---- Stockham radix-4 (+ final radix-2) along j for the TB lines
+    const T sg = p.inverse ? T(1) : T(-1);
+    int Ns = 1;
+    const int nr4 = lg >> 1;
+    for (int pass = 0; pass < nr4; pass++) {
+        const int nb = L >> 2;
+        const int step = L / (Ns * 4);
+        cx<T> v[EPT][4];
+#pragma unroll
+        for (int q = 0; q < EPT; q++) {
+            const int e = tid + q * 256;
+            if (e < nb * TB) {
+                int jj, b;
+                split(e, nb, jj, b);
+                const int k = jj & (Ns - 1);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    cx<T> x = a[lds(jj + r * nb, b)];
+                    if (r) {
+                        const cx<double> wd = p.tw[(r * k * step) & (L - 1)];
+                        x = cxmul(x, cx<T>{T(wd.x), p.inverse ? T(-wd.y) : T(wd.y)});
+                    }
+                    v[q][r] = x;
+                }
+                const cx<T> s02 = {v[q][0].x + v[q][2].x, v[q][0].y + v[q][2].y};
+                const cx<T> d02 = {v[q][0].x - v[q][2].x, v[q][0].y - v[q][2].y};
+                const cx<T> s13 = {v[q][1].x + v[q][3].x, v[q][1].y + v[q][3].y};
+                const cx<T> d13 = {v[q][1].x - v[q][3].x, v[q][1].y - v[q][3].y};
+                const cx<T> id13 = {-sg * d13.y, sg * d13.x};
+                v[q][0] = {s02.x + s13.x, s02.y + s13.y};
+                v[q][1] = {d02.x + id13.x, d02.y + id13.y};
+                v[q][2] = {s02.x - s13.x, s02.y - s13.y};
+                v[q][3] = {d02.x - id13.x, d02.y - id13.y};
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < EPT; q++) {
+            const int e = tid + q * 256;
+            if (e < nb * TB) {
+                int jj, b;
+                split(e, nb, jj, b);
+                const int k = jj & (Ns - 1);
+                const int d = (jj / Ns) * Ns * 4 + k;
+#pragma unroll
+                for (int r = 0; r < 4; r++) a[lds(d + r * Ns, b)] = v[q][r];
+            }
+        }
+        __syncthreads();
+        Ns <<= 2;
+    }
+    if (lg & 1) {
+        const int nb = L >> 1;
+        cx<T> v[2 * EPT][2];
+#pragma unroll
+        for (int q = 0; q < 2 * EPT; q++) {
+            const int e = tid + q * 256;
+            if (e < nb * TB) {
+                int jj, b;
+                split(e, nb, jj, b);
+                const int k = jj & (Ns - 1);
+                const cx<double> wd = p.tw[(k * (L / (Ns * 2))) & (L - 1)];
+                const cx<T> w = {T(wd.x), p.inverse ? T(-wd.y) : T(wd.y)};
+                const cx<T> xa = a[lds(jj, b)], xb = cxmul(a[lds(jj + nb, b)], w);
+                v[q][0] = {xa.x + xb.x, xa.y + xb.y};
+                v[q][1] = {xa.x - xb.x, xa.y - xb.y};
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 2 * EPT; q++) {
+            const int e = tid + q * 256;
+            if (e < nb * TB) {
+                int jj, b;
+                split(e, nb, jj, b);
+                const int k = jj & (Ns - 1);
+                const int d = (jj / Ns) * Ns * 2 + k;
+                a[lds(d, b)] = v[q][0];
+                a[lds(d + Ns, b)] = v[q][1];
+            }
+        }
+        __syncthreads();
+    }
+    // ---- store
+    for (int e = tid; e < total; e += 256) {
+        int j, b;
+        split(e, L, j, b);
+        const long g = base + (long)j * lstride + (long)b * bstride;
+        const cx<T> v = a[lds(j, b)];
+        if (OUT == OUT_CPLX) {
+            reinterpret_cast<cx<T> *>(p.out)[g] = v;
+        } else if (OUT == OUT_FIELDS) {
+            reinterpret_cast<T *>(p.out)[g] = T(p.scale) * v.x;
+            reinterpret_cast<T *>(p.out2)[g] = T(p.scale) * v.y;
+        } else if (OUT == OUT_REAL) {
+            reinterpret_cast<double *>(p.out)[g] = p.scale * double(v.x) + (p.base ? p.base[g] : 0.0);
+        } else {  // OUT_ACCUM: S += conj(Z) A
+            const cx<T> z = pair_z<T>(p, g);
+            const cx<T> t = {v.x * z.x + v.y * z.y, v.y * z.x - v.x * z.y};
+            cx<T> *S = reinterpret_cast<cx<T> *>(p.out);
+            if (p.first) {
+                S[g] = t;
+            } else {
+                const cx<T> sv = S[g];
+                S[g] = {sv.x + t.x, sv.y + t.y};
+            }
+        }
+    }
+}
+
+// Tabulated real spectra G of field f of component c (Omega_herm = G or i G).
+__global__ void k_setup_omega3(Grid3 G, int c, int f, double *out)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G.N) return;
+    const int i0 = (int)(g % G.n[0]);
+    const int i1 = (int)((g / G.n[0]) % G.n[1]);
+    const int i2 = (int)(g / ((long)G.n[0] * G.n[1]));
+    const cx<double> o = omega3<double>(G, c, f, i0, i1, i2);
+    int kind, w;
+    field3(f, kind, w);
+    out[g] = (kind >= 3) ? o.y : o.x;
+}
+
+}  // namespace cdft
