@@ -157,7 +157,8 @@ void fft3_pass(const cdft_sys *s, Pass3 p, bool inverse)
     case 512: fft3_launch<T, AX, IN, OUT, 8>(s, p); break;
     case 256: fft3_launch<T, AX, IN, OUT, 4>(s, p); break;
     case 128: fft3_launch<T, AX, IN, OUT, 2>(s, p); break;
-    default: fft3_launch<T, AX, IN, OUT, 1>(s, p); break;
+    case 64: fft3_launch<T, AX, IN, OUT, 1>(s, p); break;
+    default: fft3_launch<T, AX, IN, OUT, 0>(s, p); break;        // 16, 32
     }
 }
 

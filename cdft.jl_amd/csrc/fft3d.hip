@@ -123,6 +123,7 @@ template <typename T, int AX, int IN, int OUT, int EPT>
 __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
 {
     constexpr int TB = TB3;
+    constexpr int EPR = EPT > 0 ? EPT : 1;     // radix-4 butterflies per thread and pass
     extern __shared__ __align__(16) unsigned char smem3[];
     cx<T> *a = reinterpret_cast<cx<T> *>(smem3);
     const int tid = threadIdx.x;
@@ -156,23 +157,31 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
         if (AX == 0) { j = e % len; b = e / len; } else { b = e % TB; j = e / TB; }
     };
     const int total = L * TB;
-    // ---- load
-    for (int e = tid; e < total; e += 256) {
-        int j, b;
-        split(e, L, j, b);
-        const long g = base + (long)j * lstride + (long)b * bstride;
-        cx<T> v;
-        if (IN == IN_CPLX) {
-            v = reinterpret_cast<const cx<T> *>(p.in)[g];
-        } else if (IN == IN_RHO) {
-            v = {T(reinterpret_cast<const double *>(p.in)[g]), T(0)};
-        } else if (IN == IN_PAIR) {
-            v = {reinterpret_cast<const T *>(p.in)[g], reinterpret_cast<const T *>(p.in2)[g]};
-        } else {  // IN_SPEC: P Z
-            const cx<T> s = reinterpret_cast<const cx<T> *>(p.in)[g];
-            v = cxmul(s, pair_z<T>(p, g));
+    // ---- load (unrolled: the tile's global reads are all in flight together).
+    // EPT > 0: L = 64 EPT, EL = L TB / 256 = 4 EPT elements per thread, no guard;
+    // EPT = 0: short lines (L = 16, 32) with a guard.
+    constexpr bool FULL = EPT > 0;
+    constexpr int EL = FULL ? 4 * EPT : 2;
+#pragma unroll
+    for (int q = 0; q < EL; q++) {
+        const int e = tid + q * 256;
+        if (FULL || e < total) {
+            int j, b;
+            split(e, L, j, b);
+            const long g = base + (long)j * lstride + (long)b * bstride;
+            cx<T> v;
+            if (IN == IN_CPLX) {
+                v = reinterpret_cast<const cx<T> *>(p.in)[g];
+            } else if (IN == IN_RHO) {
+                v = {T(reinterpret_cast<const double *>(p.in)[g]), T(0)};
+            } else if (IN == IN_PAIR) {
+                v = {reinterpret_cast<const T *>(p.in)[g], reinterpret_cast<const T *>(p.in2)[g]};
+            } else {  // IN_SPEC: P Z
+                const cx<T> s = reinterpret_cast<const cx<T> *>(p.in)[g];
+                v = cxmul(s, pair_z<T>(p, g));
+            }
+            a[lds(j, b)] = v;
         }
-        a[lds(j, b)] = v;
     }
     __syncthreads();
     // ---- Stockham radix-4 (+ final radix-2) along j for the TB lines
@@ -182,9 +191,9 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
     for (int pass = 0; pass < nr4; pass++) {
         const int nb = L >> 2;
         const int step = L / (Ns * 4);
-        cx<T> v[EPT][4];
+        cx<T> v[EPR][4];
 #pragma unroll
-        for (int q = 0; q < EPT; q++) {
+        for (int q = 0; q < EPR; q++) {
             const int e = tid + q * 256;
             if (e < nb * TB) {
                 int jj, b;
@@ -212,7 +221,7 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < EPT; q++) {
+        for (int q = 0; q < EPR; q++) {
             const int e = tid + q * 256;
             if (e < nb * TB) {
                 int jj, b;
@@ -228,9 +237,9 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
     }
     if (lg & 1) {
         const int nb = L >> 1;
-        cx<T> v[2 * EPT][2];
+        cx<T> v[2 * EPR][2];
 #pragma unroll
-        for (int q = 0; q < 2 * EPT; q++) {
+        for (int q = 0; q < 2 * EPR; q++) {
             const int e = tid + q * 256;
             if (e < nb * TB) {
                 int jj, b;
@@ -245,7 +254,7 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 2 * EPT; q++) {
+        for (int q = 0; q < 2 * EPR; q++) {
             const int e = tid + q * 256;
             if (e < nb * TB) {
                 int jj, b;
@@ -259,7 +268,10 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
         __syncthreads();
     }
     // ---- store
-    for (int e = tid; e < total; e += 256) {
+#pragma unroll
+    for (int q = 0; q < EL; q++) {
+        const int e = tid + q * 256;
+        if (!FULL && e >= total) continue;
         int j, b;
         split(e, L, j, b);
         const long g = base + (long)j * lstride + (long)b * bstride;
