@@ -97,6 +97,9 @@ void free_sys_buffers(cdft_sys *s)
     double **b3[] = {&s->G3, &s->A3, &s->B3, &s->S3};
     for (double **b : b3)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
+    float **f3[] = {&s->G3f, &s->n32, &s->dphi32};
+    for (float **b : f3)
+        if (*b) { (void)hipFree(*b); *b = nullptr; }
 }
 
 // Composite Simpson of _int over nd axes (integrals.jl:28-40): per-axis weights
@@ -166,76 +169,106 @@ void fft3_pass(const cdft_sys *s, Pass3 p, bool inverse)
 constexpr int kPair3[4][2] = {{1, 2}, {3, 4}, {5, 6}, {7, 8}};
 inline bool is_vec3(int f) { return f >= 3 && f <= 5; }
 
+template <typename T>
 Pass3 pair_pass(const cdft_sys *s, int c, int q)
 {
     Pass3 p;
     std::memset(&p, 0, sizeof(p));
     const int f1 = kPair3[q][0], f2 = kPair3[q][1];
-    p.g1 = s->G3 + (size_t)(c * 8 + f1 - 1) * s->N;
-    p.g2 = s->G3 + (size_t)(c * 8 + f2 - 1) * s->N;
+    const size_t o1 = (size_t)(c * 8 + f1 - 1) * s->N, o2 = (size_t)(c * 8 + f2 - 1) * s->N;
+    if (sizeof(T) == sizeof(double)) {
+        p.g1 = s->G3 + o1;
+        p.g2 = s->G3 + o2;
+    } else {
+        p.g1 = s->G3f + o1;
+        p.g2 = s->G3f + o2;
+    }
     p.v1 = is_vec3(f1);
     p.v2 = is_vec3(f2);
     return p;
 }
 
-// Weighted densities of src (layout [c N + k]) into nout[(c F + f) N + k], f >= 1.
-void launch_weighted3d(cdft_sys *s, const double *src, double *nout)
+// Weighted densities of src (FP64, layout [c N + k]) into nout[(c F + f) N + k],
+// f >= 1, computed in T.
+template <typename T>
+void launch_weighted3d(cdft_sys *s, const double *src, T *nout)
 {
-    using C = cx<double>;
     const int N = s->N, F = s->F;
     for (int c = 0; c < s->nc; c++) {
         Pass3 p;
         std::memset(&p, 0, sizeof(p));
         p.in = src + (size_t)c * N;
         p.out = s->A3;
-        fft3_pass<double, 0, IN_RHO, OUT_CPLX>(s, p, false);
+        fft3_pass<T, 0, IN_RHO, OUT_CPLX>(s, p, false);
         p.in = s->A3;
-        fft3_pass<double, 1, IN_CPLX, OUT_CPLX>(s, p, false);
-        fft3_pass<double, 2, IN_CPLX, OUT_CPLX>(s, p, false);
+        fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, p, false);
+        fft3_pass<T, 2, IN_CPLX, OUT_CPLX>(s, p, false);
         for (int q = 0; q < 4; q++) {
-            Pass3 pq = pair_pass(s, c, q);
+            Pass3 pq = pair_pass<T>(s, c, q);
             pq.in = s->A3;
             pq.out = s->B3;
-            fft3_pass<double, 2, IN_SPEC, OUT_CPLX>(s, pq, true);
+            fft3_pass<T, 2, IN_SPEC, OUT_CPLX>(s, pq, true);
             pq.in = s->B3;
-            fft3_pass<double, 1, IN_CPLX, OUT_CPLX>(s, pq, true);
+            fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, pq, true);
             pq.out = nout + (size_t)(c * F + kPair3[q][0]) * N;
             pq.out2 = nout + (size_t)(c * F + kPair3[q][1]) * N;
             pq.scale = s->NA / (double)N;
-            fft3_pass<double, 0, IN_CPLX, OUT_FIELDS>(s, pq, true);
+            fft3_pass<T, 0, IN_CPLX, OUT_FIELDS>(s, pq, true);
         }
     }
-    (void)sizeof(C);
 }
 
-// dtot[c N + k] = dphi[(c F) N + k] + sum_f (transposed convolution of dphi_f).
-void launch_transposed3d(cdft_sys *s, const double *dphi, double *dtot)
+// dtot[c N + k] = dphi[(c F) N + k] + sum_f (transposed convolution of dphi_f), in T.
+template <typename T>
+void launch_transposed3d(cdft_sys *s, const T *dphi, double *dtot)
 {
     const int N = s->N, F = s->F;
     for (int c = 0; c < s->nc; c++) {
         for (int q = 0; q < 4; q++) {
-            Pass3 pq = pair_pass(s, c, q);
+            Pass3 pq = pair_pass<T>(s, c, q);
             pq.in = dphi + (size_t)(c * F + kPair3[q][0]) * N;
             pq.in2 = dphi + (size_t)(c * F + kPair3[q][1]) * N;
             pq.out = s->B3;
-            fft3_pass<double, 0, IN_PAIR, OUT_CPLX>(s, pq, false);
+            fft3_pass<T, 0, IN_PAIR, OUT_CPLX>(s, pq, false);
             pq.in = s->B3;
-            fft3_pass<double, 1, IN_CPLX, OUT_CPLX>(s, pq, false);
+            fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, pq, false);
             pq.out = s->S3;
             pq.first = (q == 0);
-            fft3_pass<double, 2, IN_CPLX, OUT_ACCUM>(s, pq, false);
+            fft3_pass<T, 2, IN_CPLX, OUT_ACCUM>(s, pq, false);
         }
         Pass3 p;
         std::memset(&p, 0, sizeof(p));
         p.in = s->S3;
         p.out = s->S3;
-        fft3_pass<double, 2, IN_CPLX, OUT_CPLX>(s, p, true);
-        fft3_pass<double, 1, IN_CPLX, OUT_CPLX>(s, p, true);
+        fft3_pass<T, 2, IN_CPLX, OUT_CPLX>(s, p, true);
+        fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, p, true);
         p.out = dtot + (size_t)c * N;
         p.base = dphi + (size_t)(c * F) * N;
         p.scale = 1.0 / (double)N;
-        fft3_pass<double, 0, IN_CPLX, OUT_REAL>(s, p, true);
+        fft3_pass<T, 0, IN_CPLX, OUT_REAL>(s, p, true);
     }
+}
+
+template <int NC>
+void launch_phi32(cdft_sys *s)
+{
+    hipLaunchKernelGGL((k_phi32<NC, 3>), dim3(nblocks(s->N)), dim3(256), 0, s->ctx->stream, s->rho, s->n32,
+                       s->dphi32, s->N, s->NA, s->pc);
+}
+
+// The FP32 map of a mixed-precision 3D system: weighted densities, Phi's gradient
+// and transposed convolutions in FP32 (n3-singular factors in FP64); the sum
+// lands in FP64 (s->part) for the exponent assembly.
+void launch_functional32(cdft_sys *s)
+{
+    launch_weighted3d<float>(s, s->rho, s->n32);
+    switch (s->nc) {
+    case 1: launch_phi32<1>(s); break;
+    case 2: launch_phi32<2>(s); break;
+    case 3: launch_phi32<3>(s); break;
+    default: launch_phi32<4>(s); break;
+    }
+    launch_transposed3d<float>(s, s->dphi32, s->part);
 }
 
 template <int NC>
@@ -265,7 +298,7 @@ void launch_phi_any(cdft_sys *s)
 void launch_transposed(cdft_sys *s)
 {
     if (s->nd == 3) {
-        launch_transposed3d(s, s->dphi, s->part);
+        launch_transposed3d<double>(s, s->dphi, s->part);
         return;
     }
     int N = s->N, F = s->F;
@@ -277,7 +310,7 @@ void launch_transposed(cdft_sys *s)
 void launch_weighted(cdft_sys *s)
 {
     if (s->nd == 3) {
-        launch_weighted3d(s, s->rho, s->n);
+        launch_weighted3d<double>(s, s->rho, s->n);
         return;
     }
     int N = s->N, F = s->F;
@@ -311,7 +344,7 @@ void launch_hvp(cdft_sys *s, const double *v)
     hipStream_t st = s->ctx->stream;
     int N = s->N, F = s->F, nc = s->nc;
     if (s->nd == 3) {
-        launch_weighted3d(s, v, s->dphi);
+        launch_weighted3d<double>(s, v, s->dphi);
     } else {
         dim3 g(nblocks(N), nc * (F - 1));
         hipLaunchKernelGGL(k_circ1d<double>, g, dim3(256), 0, st, v, (long)N, 0L, s->h, s->dphi, (long)F * N,
@@ -448,6 +481,10 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
         (e = alloc(&s->fval, N)) != hipSuccess ||
         (d3 && ((e = alloc(&s->G3, (size_t)8 * nc * N)) != hipSuccess || (e = alloc(&s->A3, (size_t)2 * N)) != hipSuccess ||
                 (e = alloc(&s->B3, (size_t)2 * N)) != hipSuccess || (e = alloc(&s->S3, (size_t)2 * N)) != hipSuccess)) ||
+        (d3 && d->precision == CDFT_MIXED &&
+         ((e = hipMalloc((void **)&s->G3f, sizeof(float) * 8 * nc * (size_t)N)) != hipSuccess ||
+          (e = hipMalloc((void **)&s->n32, sizeof(float) * (size_t)F * nc * N)) != hipSuccess ||
+          (e = hipMalloc((void **)&s->dphi32, sizeof(float) * (size_t)F * nc * N)) != hipSuccess)) ||
         (e = alloc(&s->out, n)) != hipSuccess || (e = alloc(&s->sol, n)) != hipSuccess ||
         (e = alloc(&s->gx, n)) != hipSuccess || (e = alloc(&s->df, n)) != hipSuccess ||
         (e = alloc(&s->dg, n)) != hipSuccess || (e = alloc(&s->res, n)) != hipSuccess ||
@@ -502,7 +539,8 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
         for (int c = 0; c < nc && e == hipSuccess; c++)
             for (int f = 1; f <= 8; f++)
                 hipLaunchKernelGGL(k_setup_omega3, dim3(nblocks(N)), dim3(256), 0, ctx->stream, G, c, f,
-                                   s->G3 + (size_t)(c * 8 + f - 1) * N);
+                                   s->G3 + (size_t)(c * 8 + f - 1) * N,
+                                   s->G3f ? s->G3f + (size_t)(c * 8 + f - 1) * N : (float *)nullptr);
         if (e == hipSuccess) e = hipGetLastError();
     } else {
         double mult = ((double)N / (s->ub[0] - s->lb[0])) / (double)N;   // fftfreq spacing
@@ -596,15 +634,25 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     double *fin_h = nullptr;
     HIPCHECK_CTX(ctx, hipHostMalloc((void **)&fin_h, sizeof(double) * 4 * CDFT_NQ));
     struct HostFree { double *p; ~HostFree() { if (p) (void)hipHostFree(p); } } hf{fin_h};
-    int nh = 0, nevals = 0;
+    int nh = 0, nevals = 0, nevals32 = 0;
     auto hist = [&](double v) { if (o->hist && nh < o->hist_cap) o->hist[nh] = v; nh++; };
+    // Mixed precision (3D): the FP32 map until the residual reaches the FP32 noise
+    // floor (switch_factor tol, or no halving in stall_window iterations), then the
+    // FP64 map with a fresh Anderson history, as in the batched path.
+    bool fp32 = (!adj && s->precision == CDFT_MIXED && s->nd == 3);
+    const double switch_factor = 4.0;
+    const int stall_window = 40;
+    double best = 0.0;
+    int since_best = 0;
 
     // G evaluation + post-processing; returns the finalized reductions in fin_h[0..2]
     double *ptgt = adj ? adj->vbuf : s->rho;              // where a new point is expanded
     const double *pmask = adj ? adj->mask : nullptr;
     auto evalG = [&](int mode, double b) -> int {
         if (adj) launch_hvp(s, adj->vbuf);
+        else if (fp32) launch_functional32(s);
         else launch_functional(s);
+        if (fp32) nevals32++;
         hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part, s->vext, s->N, s->nc,
                            upd_src(s).F,
                            s->cpd, s->out, s->sol, s->gx, s->res, s->df, s->dg, b, mode, s->red, nb,
@@ -651,23 +699,52 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     int ring[CDFT_MAXM];
     for (int j = 0; j < CDFT_MAXM; j++) ring[j] = j;
     double dfn2 = 0.0, dfres = 0.0;
+    int ka = 0;                                   // Anderson steps since the (re)start
     if (!toosoon) {
         hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, ptgt, (int)n, pmask);
         k += 1;
+        ka = 1;
         EVAL(2, beta);
         resnorm = std::sqrt(fin_h[0]);
         dfn2 = fin_h[1];
         dfres = fin_h[2];
         hist(resnorm);
+        best = resnorm;
     }
 
     double *Qc = s->Q, *Qa = s->Qalt;
-    while (k < maxit + picard_maxit && resnorm > tol && !toosoon && resnorm < up_bd) {
+    while (k < maxit + picard_maxit && (resnorm > tol || fp32) && !toosoon && resnorm < up_bd) {
+        if (fp32) {
+            if (resnorm < 0.5 * best) {
+                best = resnorm;
+                since_best = 0;
+            } else {
+                since_best++;
+            }
+            if (resnorm <= switch_factor * tol || since_best >= stall_window) {
+                fp32 = false;                     // the FP64 map from here
+                EVAL(1, beta);
+                resnorm = std::sqrt(fin_h[0]);
+                hist(resnorm);
+                if (resnorm <= tol) break;
+                hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, ptgt, (int)n, pmask);
+                k += 1;
+                ka = 1;
+                EVAL(2, beta);
+                resnorm = std::sqrt(fin_h[0]);
+                dfn2 = fin_h[1];
+                dfres = fin_h[2];
+                hist(resnorm);
+                std::memset(R, 0, sizeof(R));
+                for (int j = 0; j < CDFT_MAXM; j++) ring[j] = j;
+                continue;
+            }
+        }
         if (m == 0) {
             hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, ptgt, (int)n, pmask);
         } else {
             // BuildDG!(DG, m, k+1, dg) with a ring of physical columns
-            int kk = k + 1, slot;
+            int kk = ka + 1, slot;
             if (m == 1) {
                 slot = ring[0];
             } else if (kk > m + 1) {
@@ -681,7 +758,7 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
             HIPCHECK_CTX(ctx, hipMemcpyAsync(s->DG + (size_t)slot * n, s->dg, sizeof(double) * n,
                                              hipMemcpyDeviceToDevice, st));
             // aa_qr_update!(Q, R, df, m, k-1, Qd)
-            int kq = k - 1, qcol;
+            int kq = ka - 1, qcol;
             double nrm, tres[CDFT_MAXM];
             if (kq == 0) {
                 nrm = std::sqrt(dfn2);
@@ -715,7 +792,7 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
                 for (int j = 0; j < kqq; j++) tres[j] = tq[j];
                 tres[kqq] = fin_h[2 * CDFT_NQ + 1] / nrm;
             }
-            int mk = std::min(m, k);
+            int mk = std::min(m, ka);
             AAMat th;
             std::memset(&th, 0, sizeof(th));
             for (int j = mk - 1; j >= 0; j--) {          // theta = RA \ tres
@@ -728,6 +805,7 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
                                Qc + (size_t)qcol * n, s->sol, s->gx, s->DG, th, mk, ptgt, pmask);
         }
         k += 1;
+        ka += 1;
         EVAL(2, beta);
         resnorm = std::sqrt(fin_h[0]);
         dfn2 = fin_h[1];
@@ -739,7 +817,7 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     int idid, errcode;
     if (toosoon) { idid = 1; errcode = -1; }
     else if (resnorm >= up_bd) { idid = 0; errcode = -2; }
-    else if (resnorm <= tol) { idid = 1; errcode = 0; }
+    else if (resnorm <= tol && !fp32) { idid = 1; errcode = 0; }
     else { idid = 0; errcode = 10; }
 
     if (!adj) hipLaunchKernelGGL(k_exp, dim3(nb), dim3(256), 0, st, s->sol, s->out, (int)n);
@@ -753,7 +831,7 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     r->nhist = nh;
     r->resnorm = resnorm;
     r->tol = tol;
-    r->nevals_fp32 = 0;
+    r->nevals_fp32 = nevals32;
     r->reserved = 0;
     r->device_ms = 0.0;
     return CDFT_OK;

@@ -66,6 +66,8 @@ struct cdft_sys {
     // 3D grids: tabulated spectra G [(c*8 + f-1)*N + k] and three complex work arrays
     cdft::Grid3 g3;
     double *G3 = nullptr, *A3 = nullptr, *B3 = nullptr, *S3 = nullptr;
+    // 3D mixed precision: FP32 spectra, weighted densities and gradients
+    float *G3f = nullptr, *n32 = nullptr, *dphi32 = nullptr;
     double *red = nullptr;     // block partials
     double *fin = nullptr;     // finalized reductions [CDFT_NQ]
     int nblk = 0;

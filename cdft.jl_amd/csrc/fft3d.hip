@@ -43,9 +43,9 @@ struct Pass3 {
     const void *in2;          // IN_PAIR: second field
     void *out;                // OUT_CPLX / OUT_ACCUM: complex T; OUT_FIELDS: T field 1; OUT_REAL: double
     void *out2;               // OUT_FIELDS: T field 2
-    const double *base;       // OUT_REAL: added to the result (dPhi/dn_rho), or null
+    const void *base;         // OUT_REAL: T values added to the result (dPhi/dn_rho), or null
     const cx<double> *tw;     // W_L^t for this axis, t < L
-    const double *g1, *g2;    // IN_SPEC / OUT_ACCUM: real spectra G of the pair's fields
+    const void *g1, *g2;      // IN_SPEC / OUT_ACCUM: real spectra G (type T) of the pair's fields
     int v1, v2;               // 1 where the field is a vector weight (H = i G)
     int first;                // OUT_ACCUM: overwrite instead of accumulate
     int inverse;
@@ -56,10 +56,10 @@ struct Pass3 {
 template <typename T>
 __device__ __attribute__((always_inline)) inline cx<T> pair_z(const Pass3 &p, long g)
 {
-    const double a = p.g1[g], b = p.g2[g];
-    const cx<double> h1 = p.v1 ? cx<double>{0.0, a} : cx<double>{a, 0.0};
-    const cx<double> h2 = p.v2 ? cx<double>{0.0, b} : cx<double>{b, 0.0};
-    return {T(h1.x - h2.y), T(h1.y + h2.x)};
+    const T a = reinterpret_cast<const T *>(p.g1)[g], b = reinterpret_cast<const T *>(p.g2)[g];
+    const cx<T> h1 = p.v1 ? cx<T>{T(0), a} : cx<T>{a, T(0)};
+    const cx<T> h2 = p.v2 ? cx<T>{T(0), b} : cx<T>{b, T(0)};
+    return {h1.x - h2.y, h1.y + h2.x};
 }
 
 // Field table of PC-SAFT in 3D (PCSAFT.jl:55-61): kind 1 int rho dz, 2 int rho z^2 dz,
@@ -282,7 +282,8 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
             reinterpret_cast<T *>(p.out)[g] = T(p.scale) * v.x;
             reinterpret_cast<T *>(p.out2)[g] = T(p.scale) * v.y;
         } else if (OUT == OUT_REAL) {
-            reinterpret_cast<double *>(p.out)[g] = p.scale * double(v.x) + (p.base ? p.base[g] : 0.0);
+            reinterpret_cast<double *>(p.out)[g] =
+                p.scale * double(v.x) + (p.base ? double(reinterpret_cast<const T *>(p.base)[g]) : 0.0);
         } else {  // OUT_ACCUM: S += conj(Z) A
             const cx<T> z = pair_z<T>(p, g);
             const cx<T> t = {v.x * z.x + v.y * z.y, v.y * z.x - v.x * z.y};
@@ -297,8 +298,9 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
     }
 }
 
-// Tabulated real spectra G of field f of component c (Omega_herm = G or i G).
-__global__ void k_setup_omega3(Grid3 G, int c, int f, double *out)
+// Tabulated real spectra G of field f of component c (Omega_herm = G or i G),
+// evaluated in FP64; out32 (optional) receives the FP32 copy.
+__global__ void k_setup_omega3(Grid3 G, int c, int f, double *out, float *out32)
 {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= G.N) return;
@@ -309,6 +311,7 @@ __global__ void k_setup_omega3(Grid3 G, int c, int f, double *out)
     int kind, w;
     field3(f, kind, w);
     out[g] = (kind >= 3) ? o.y : o.x;
+    if (out32) out32[g] = (float)out[g];
 }
 
 }  // namespace cdft

@@ -141,6 +141,29 @@ __global__ __launch_bounds__(256) void k_phi(const double *__restrict__ rho, con
     if (fval) fval[i] = v;
 }
 
+// FP32 map (mixed precision): Phi's gradient from FP32 weighted densities, with the
+// n3-singular hard-sphere factors in FP64 (phi_grad<float, double>).
+template <int NC, int ND>
+__global__ __launch_bounds__(256) void k_phi32(const double *__restrict__ rho, const float *__restrict__ n,
+                                               float *__restrict__ dphi, int N, double NA, PhiConst P)
+{
+    constexpr int F = 6 + ND;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float nl[NC][F], g[NC][F];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        nl[c][0] = float(rho[(size_t)c * N + i] * NA);
+#pragma unroll
+        for (int f = 1; f < F; f++) nl[c][f] = n[(size_t)(c * F + f) * N + i];
+    }
+    phi_grad<float, double, NC, ND>(P, nl, g);
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int f = 0; f < F; f++) dphi[(size_t)(c * F + f) * N + i] = g[c][f];
+}
+
 // Hessian-vector product of Phi at every point (no reference counterpart: the
 // second derivative behind the fixed-point adjoint).  Slot 0 moves along N_A v,
 // the other slots along their weighted densities ndot = W v (in ndio on entry);

@@ -117,3 +117,52 @@ def test_converge_3d_matches_oracle():
     e_ = rel(rho.reshape(-1), rho_o)
     say(f"  profile rel err {e_:.2e}")
     assert e_ < 1e-8, e_
+
+
+def test_converge_3d_mixed_precision():
+    say("3D converge, mixed precision vs FP64: methane, 16^3, one hard sphere, tol 1e-9")
+    from cdft_amd import DFTOptions
+    kw = dict(n=16, box=5.0, nspheres=1, seed=2, T=150.0, x=(1.0,), components=("methane",))
+    s64, rho0 = configs.matrix_3d(**kw)
+    s32, _ = configs.matrix_3d(options=DFTOptions(precision="mixed"), **kw)
+    r64, r32 = rho0.copy(), rho0.copy()
+    converge_(s64, r64, tol=1e-9)
+    converge_(s32, r32, tol=1e-9)
+    a, b = s64.last_result, s32.last_result
+    say(f"  FP64 errcode {a['errcode']} evals {a['nevals']} | mixed errcode {b['errcode']} evals {b['nevals']} "
+        f"(FP32 {b['nevals_fp32']})")
+    assert a["errcode"] == 0 and b["errcode"] == 0
+    assert b["nevals_fp32"] > 0
+    e_ = rel(r32.reshape(-1), r64.reshape(-1))
+    say(f"  profile rel diff {e_:.2e}")
+    assert e_ < 1e-7, e_
+
+
+def test_adjoint_3d_adsorption_sensitivity():
+    say("3D adjoint: d(adsorption)/d(sigma, eps, m), methane around a hard sphere, 16^3, vs FD of full runs")
+    from cdft_amd import adjoint as adj
+    s, rho0 = configs.matrix_3d(n=16, box=5.0, nspheres=1, seed=2, T=150.0, x=(1.0,), components=("methane",))
+    kw = dict(tol=1e-11, maxit=5000)
+    rho = rho0.copy()
+    converge_(s, rho, **kw)
+    assert s.last_result["errcode"] == 0, s.last_result
+    rho = rho.reshape(-1)
+    sens, lam, res = adj.sensitivities(s, rho, "adsorption", 0, tol=1e-11)
+    say(f"  adjoint: errcode {res['errcode']} evals {res['nevals']}")
+    assert res["errcode"] == 0, res
+    for (prm, i), val in sens.items():
+        base = adj.model_params(s.model)[s.model.components[i]][{"segment": 0, "sigma": 1, "epsilon": 2}[prm]]
+        h = 1e-5 * base
+        vals = []
+        for sg in (+1, -1):
+            sp = adj.perturbed_system(s, i, prm, sg * h)
+            r = rho0.copy()          # from the seed: aasol's divergence guard is relative to res0
+            converge_(sp, r, **kw)
+            rr = sp.last_result
+            say(f"    FD run {prm} {sg:+d}h: errcode {rr['errcode']} evals {rr['nevals']} resnorm {rr['resnorm']:.2e}")
+            # a run that stops at maxit on a residual floor below 1e-9 still fixes Gamma to ~1e-12
+            assert rr["errcode"] == 0 or (rr["errcode"] == 10 and rr["resnorm"] < 1e-9), rr
+            vals.append(adj.objective(sp, r, "adsorption", 0))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        say(f"  dGamma/d {prm}: adjoint {val:.10e}  FD {fd:.10e}  rel {abs(val - fd) / abs(fd):.2e}")
+        assert abs(val - fd) <= 1e-4 * abs(fd), (i, val, fd)
