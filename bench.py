@@ -2,14 +2,20 @@
 mixed-precision path on 512 methane slit pores, with the plain FP64 path and the
 CPU oracle timed beside it on samples of the same workload.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--npores 512]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--npores 512] [--no-extras]
 
 A step is one batched solve of all pores from their seed profiles (inputs resident
-in HBM).  Prints one JSON line.
+in HBM).  Prints one JSON line.  Besides the headline workload the line carries
+two further measurements of the same library (skipped with --no-extras):
+  matrix_3d   the 3D hard-sphere-matrix ternary on a 256^3 grid: one mixed-precision
+              solve, stream time per map evaluation and the algorithmic-byte rate;
+  adjoint     surface-tension sensitivities of the methane / n-decane interface
+              (2048 points): adjoint solve versus finite differences of full solves.
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -18,6 +24,7 @@ sys.path.insert(0, os.path.join(ROOT, "cdft.jl_amd"))
 
 METRIC = "converged profiles/sec (res<1e-8): batched FP32/FP64 vs plain FP64, same GPU"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_batched_slit_pores_512_traffic.md")
 
 
 def bytes_per_point_eval(s, m=5, F=7):
@@ -27,28 +34,18 @@ def bytes_per_point_eval(s, m=5, F=7):
     return (1 + F) * s + 2 * F * s + (F + 1) * s + (s + 24) + 8 * (2 * m + 6)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--npores", type=int, default=512)
-    ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--plain-sample", type=int, default=8)
-    ap.add_argument("--cpu-sample", type=int, default=8)
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; the 512 pores are independent and would be "
-              "split across GPUs by pore (no collectives), which is not built here", file=sys.stderr)
-        sys.exit(2)
+def measured_traffic():
+    """HBM bytes per k_batched launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (None when absent)."""
+    try:
+        txt = open(TRAFFIC_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
 
-    import numpy as np
-    import torch
 
-    from cdft_amd import DFTOptions, _lib, configs, converge_
-
-    # ---------------- workload
+def run_c3(args, np, torch, DFTOptions, _lib, configs):
     pores = configs.slit_pores(n=args.npores, seed=0, options=DFTOptions(precision=args.precision))
     systems = [s.device_system for s, _ in pores]
     ctx = systems[0].ctx
@@ -65,6 +62,13 @@ def main():
         torch.cuda.synchronize()
         return _lib.converge_batch(ctx, systems, ptrs, tol=args.tol, device_ptrs=True)
 
+    if args.copy_probe_mib:
+        # a device-to-device copy of known size, the reference for the profiler's
+        # FETCH_SIZE / WRITE_SIZE scale
+        a = torch.empty(args.copy_probe_mib * 131072, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -76,6 +80,95 @@ def main():
         kern_ms += ctx.last_kernel_ms()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    return pores, systems, sizes, results, dt, kern_ms
+
+
+def run_matrix_3d(np, DFTOptions, configs, converge_):
+    s, rho = configs.matrix_3d(n=256, options=DFTOptions(precision="mixed"))
+    dev = s.device_system
+    r = rho.copy()
+    t = time.perf_counter()
+    converge_(s, r, tol=1e-8, **configs.MATRIX_3D_SOLVER)
+    wall = time.perf_counter() - t
+    res = s.last_result
+    n32, n64 = res["nevals_fp32"], res["nevals"] - res["nevals_fp32"]
+    F = 9
+    alg = s.N * s.nc * (n32 * bytes_per_point_eval(4, F=F) + n64 * bytes_per_point_eval(8, F=F))
+    ms = res["device_ms"]
+    from cdft_amd.methods import adsorption_system
+    return {
+        "workload": "methane/ethane/propane x=[0.6,0.3,0.1], T=200 K, p=3e6 Pa, 30 sigma periodic cube on "
+                    "256^3 points, 64 hard spheres r~U[1,2.5] sigma (seed 0); mixed precision, tol 1e-8, "
+                    "anderson_start 0.5",
+        "errcode": res["errcode"], "nevals": res["nevals"], "nevals_fp32": n32,
+        "solve_s": wall, "stream_ms": ms, "ms_per_eval": ms / max(res["nevals"], 1),
+        "algorithmic_GBps": alg / (ms * 1e-3) / 1e9,
+        "adsorption_mol_m3": [float(v) for v in adsorption_system(s, r)],
+    }
+
+
+def run_adjoint(np, configs, converge_):
+    from cdft_amd import adjoint as adj
+    s, rho0 = configs.c1c10_interface(ngrid=2048, box=20.0)
+    r = rho0.copy()
+    t = time.perf_counter()
+    converge_(s, r, tol=1e-12, maxit=20000)
+    t_solve = time.perf_counter() - t
+    rho = r.reshape(-1)
+    t = time.perf_counter()
+    sens, lam, res = adj.sensitivities(s, rho, "gamma", tol=1e-10)
+    t_adj = time.perf_counter() - t
+    # finite differences of full solves for two of the six parameters
+    fd = {}
+    t = time.perf_counter()
+    for key in (("epsilon", 0), ("sigma", 1)):
+        prm, i = key
+        b0 = adj.model_params(s.model)[s.model.components[i]][{"segment": 0, "sigma": 1, "epsilon": 2}[prm]]
+        h = 1e-5 * abs(b0)
+        vals = []
+        for sg in (+1, -1):
+            sp = adj.perturbed_system(s, i, prm, sg * h)
+            rr = rho.copy().reshape(s.nc, s.N)
+            converge_(sp, rr, tol=1e-12, maxit=20000)
+            vals.append(adj.objective(sp, rr, "gamma"))
+        fd[key] = (vals[0] - vals[1]) / (2 * h)
+    t_fd = (time.perf_counter() - t) / len(fd)
+    return {
+        "workload": "d(surface tension)/d(sigma_i, eps_i, m_i), methane/n-decane interface, 2048 points, "
+                    "40 sigma box, T=300 K; FP64",
+        "solve_s": t_solve, "adjoint_and_partials_s": t_adj, "adjoint_errcode": res["errcode"],
+        "adjoint_nevals": res["nevals"], "fd_full_solves_s_per_parameter": t_fd,
+        "dgamma": {f"{k[0]}_{k[1]}": v for k, v in sens.items()},
+        "max_rel_diff_vs_fd": max(abs(sens[k] - v) / abs(v) for k, v in fd.items()),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--npores", type=int, default=512)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--cpu-sample", type=int, default=8)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--no-extras", action="store_true", help="skip the 3D and adjoint measurements")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run a device copy of this many MiB first (profiler counter scale)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; the 512 pores are independent and would be "
+              "split across GPUs by pore (no collectives), which is not built here", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from cdft_amd import DFTOptions, _lib, configs, converge_
+
+    # ---------------- headline workload
+    pores, systems, sizes, results, dt, kern_ms = run_c3(args, np, torch, DFTOptions, _lib, configs)
     nconv = sum(1 for r in results if r["errcode"] == 0)
     codes = {}
     for r in results:
@@ -104,20 +197,22 @@ def main():
         converge_(s, rho, tol=args.tol)
         t_plain += time.perf_counter() - t1
         nplain += s.last_result["errcode"] == 0
-    plain_rate = len(idx) / t_plain
+    plain_rate = nplain / t_plain
 
     # ---------------- CPU oracle (1 thread, FP64) on a sample
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     cidx = np.linspace(0, args.npores - 1, args.cpu_sample).round().astype(int)
     t_cpu = 0.0
+    ncpu = 0
     for i in cidx:
         s, r0 = plain_pores[i]
         o = po.Oracle(**s.descriptor())
         t1 = time.perf_counter()
-        o.converge(r0.reshape(-1), tol=args.tol)
+        _, ro, _ = o.converge(r0.reshape(-1), tol=args.tol)
         t_cpu += time.perf_counter() - t1
-    cpu_rate = len(cidx) / t_cpu
+        ncpu += ro.errcode == 0
+    cpu_rate = ncpu / t_cpu
 
     out = {
         "metric": METRIC,
@@ -139,13 +234,16 @@ def main():
             "grid_points_total": int(sum(sizes)), "errcodes": codes,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_batched", "kernel_ms": kms, "algorithmic_bytes": alg_bytes},
         "plain_fp64": {"value": plain_rate, "unit": "profiles/s", "sample": f"{len(idx)} pores spread over the batch",
                        "converged": int(nplain), "speedup_batched_vs_plain": value / plain_rate},
         "cpu_baseline": {"value": cpu_rate, "unit": "profiles/s", "cores": 1, "kind": "port",
                          "sample": f"{len(cidx)} pores spread over the batch, FP64 C oracle, 1 thread"},
     }
+    if not args.no_extras:
+        out["matrix_3d"] = run_matrix_3d(np, DFTOptions, configs, converge_)
+        out["adjoint"] = run_adjoint(np, configs, converge_)
     print(json.dumps(out))
 
 

@@ -674,6 +674,7 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
         hipLaunchKernelGGL(k_init_x, dim3(nb), dim3(256), 0, st, s->out, s->sol, s->rho, (int)n);
     }
 
+    HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev0, st));
     int k = 0;
     EVAL(1, beta);
     double resnorm = std::sqrt(fin_h[0]);
@@ -822,6 +823,7 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
 
     if (!adj) hipLaunchKernelGGL(k_exp, dim3(nb), dim3(256), 0, st, s->sol, s->out, (int)n);
     HIPCHECK_CTX(ctx, hipGetLastError());
+    HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev1, st));
     HIPCHECK_CTX(ctx, hipMemcpyAsync(rho_inout, adj ? s->sol : s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
     r->idid = idid;
@@ -833,7 +835,10 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     r->tol = tol;
     r->nevals_fp32 = nevals32;
     r->reserved = 0;
-    r->device_ms = 0.0;
+    float ms = 0.0f;
+    HIPCHECK_CTX(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_kernel_ms = ms;
+    r->device_ms = ms;
     return CDFT_OK;
 }
 

@@ -93,7 +93,8 @@ typedef struct {
     double tol;            /* rtol*res0 + atol                                    */
     int nevals_fp32;       /* map evaluations done in FP32 (CDFT_MIXED)           */
     int reserved;
-    double device_ms;      /* batched path: this system's time in its workgroup   */
+    double device_ms;      /* batched: this system's time in its workgroup; plain:
+                              stream time of the solve                          */
 } cdft_result;
 
 /* Context: device selection and one HIP stream.  (No counterpart: the
@@ -134,8 +135,9 @@ int cdft_converge_batch(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *c
 int cdft_converge_batch_device(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_dev,
                                const cdft_solver_opts *opts, cdft_result *results);
 
-/* Kernel time of the last cdft_converge_batch* launch in ms, measured with HIP
- * events on the context's stream. */
+/* Stream time in ms of the last solve on the context, measured with HIP events
+ * on its stream: the persistent kernel of cdft_converge_batch*, or first to last
+ * kernel of cdft_converge / cdft_adjoint_solve (their result's device_ms too). */
 double cdft_last_kernel_ms(const cdft_ctx *ctx);
 
 /* Post-processing of a converged profile.
