@@ -109,7 +109,9 @@ def test_converge_wall_tight(c0):
     rho_o, ro, _ = o.converge(rho0.reshape(-1), tol=1e-12)
     assert s.last_result["errcode"] == 0 and ro.errcode == 0
     e_ = rel(rho.reshape(-1), rho_o)
-    assert e_ < 1e-5, e_
+    say(f"  GPU evals {s.last_result['nevals']} res {s.last_result['resnorm']:.3e} | oracle evals {ro.nevals} "
+        f"res {ro.resnorm:.3e} | profile rel diff {e_:.2e}")
+    assert e_ < 1e-10, e_     # the plain FP64 path's parity bar on converged profiles
 
 
 def test_converge_errcodes(c0):
@@ -141,4 +143,5 @@ def test_converge_interface_small_parity():
     rho_o, ro, _ = o.converge(rho0.reshape(-1), tol=1e-11)
     assert s.last_result["errcode"] == 0 and ro.errcode == 0
     e_ = rel(rho.reshape(-1), rho_o)
+    say(f"  profile rel diff {e_:.2e}")
     assert e_ < 1e-7, e_   # the interface position drifts at the solver tolerance
