@@ -135,6 +135,39 @@ function converge_batch!(systems::AbstractVector, rhos::AbstractVector; maxit = 
     return res
 end
 
-export HIPNative, converge_batch!
+"""Pullback of sum_i cot_i Phi_i (cdft_phi_vjp): the rrule of F_res with cot = 1."""
+function phi_vjp(system, rho::AbstractArray, cot::AbstractVector)
+    ctx = _ctx(system.options.device.device)
+    sys = _system(system, ctx)
+    out = zeros(Float64, length(rho))
+    try
+        _check(ccall((:cdft_phi_vjp, libcdft), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
+                     sys, Array{Float64}(vec(rho)), Array{Float64}(cot), out), ctx)
+    finally
+        ccall((:cdft_system_destroy, libcdft), Cint, (Ptr{Cvoid},), sys)
+    end
+    return reshape(out, size(rho))
+end
+
+"""Adjoint of the converged fixed point (cdft_adjoint_solve): lambda with (I - J') lambda = g."""
+function adjoint_solve(system, rho::AbstractArray, g::AbstractArray; maxit = 10_000, beta = 1e-2, tol = 1e-10,
+                       anderson_start = 1e-1, anderson_m = 5)
+    ctx = _ctx(system.options.device.device)
+    sys = _system(system, ctx)
+    lam = zeros(Float64, length(rho))
+    res = Ref{CdftResult}()
+    opts = CdftSolverOpts(maxit, beta, tol, anderson_start, anderson_m, 0, C_NULL)
+    try
+        _check(ccall((:cdft_adjoint_solve, libcdft), Cint,
+                     (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ref{CdftSolverOpts}, Ref{CdftResult}),
+                     sys, Array{Float64}(vec(rho)), Array{Float64}(vec(g)), lam, opts, res), ctx)
+    finally
+        ccall((:cdft_system_destroy, libcdft), Cint, (Ptr{Cvoid},), sys)
+    end
+    LAST[] = res[]
+    return reshape(lam, size(rho))
+end
+
+export HIPNative, converge_batch!, phi_vjp, adjoint_solve
 
 end # module
