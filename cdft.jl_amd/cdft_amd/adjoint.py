@@ -109,7 +109,8 @@ def log_map(system, rho):
 
 
 def translation_mode(system, rho):
-    """d(log rho)/dz on the periodic grid (spectral derivative), shape (nc * N,).
+    """d(log rho)/d(phase), phase = 2 pi (z - lb)/(ub - lb): the spectral derivative of
+    log rho on the periodic grid (a multiple of d(log rho)/dz), shape (nc * N,).
 
     Without an external field every operator of the map is circulant, so the
     fixed point comes in a family of translates and I - dG/dx is singular along
