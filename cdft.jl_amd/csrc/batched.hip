@@ -102,7 +102,8 @@ template <typename T, int NARR, class Load0, class StoreL>
 CDFT_INLINE void fft_fused(c2<T> *W, const c2<double> *tw, int M, int lg, bool inverse, Load0 &&load0,
                            StoreL &&storeL)
 {
-    const int j = threadIdx.x;
+    int j = threadIdx.x;
+    asm volatile("" : "+v"(j));           // opaque: keeps the index arithmetic inside the solver loop
     const T sg = inverse ? T(1) : T(-1);
     const int nr4 = lg >> 1;
     const bool odd = (lg & 1) != 0;
@@ -220,9 +221,15 @@ CDFT_INLINE double sum8(double x)
 // Sum of NQ values over the workgroup; every thread receives the same totals
 // (fixed summation order, so the result is reproducible and uniform): rows of 16
 // lanes by DPP, the four rows of a wave by lane reads, the waves through LDS.
+// Consecutive calls alternate between two halves of red (2 x 64 doubles), so one
+// barrier per call suffices: a half is rewritten two calls later, after the
+// intervening call's barrier, when every thread has read it.
 template <int NQ>
-CDFT_INLINE void wg_sum(double (&v)[NQ], double *red)
+CDFT_INLINE void wg_sum(double (&v)[NQ], double *red2, int &par)
 {
+    static_assert(NQ * 4 <= 64, "reduction slots");
+    double *red = red2 + (par ? 64 : 0);
+    par ^= 1;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
@@ -236,7 +243,6 @@ CDFT_INLINE void wg_sum(double (&v)[NQ], double *red)
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < NQ; q++) v[q] = (red[q * 4 + 0] + red[q * 4 + 1]) + (red[q * 4 + 2] + red[q * 4 + 3]);
-    __syncthreads();
 }
 
 // One system's data, read once from its descriptor into registers.
@@ -338,10 +344,11 @@ CDFT_INLINE void pair_multipliers(const double *Gc, int G1, int M, int k, c2<T> 
 // (1: first/Picard step, 2: aa_point!).  out = |res|^2, |df|^2, df.res.
 // LDS: tw (M, FP64), spectra G (6 NC (M/2+1) FP64, or global), three work arrays W
 // and the accumulator P (M complex T each, at FP64 stride), fields fld (F*N*NC of T
-// at FP64 stride), red (64 doubles).
+// at FP64 stride), red (2 x 64 doubles, used alternately).
 template <typename T, int NC, bool REG>
 CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, const c2<double> *tw, const double *G,
-                       c2<T> *W, c2<T> *Pc, T *fld, double *red, int mode, double b, double (&out)[3])
+                       c2<T> *W, c2<T> *Pc, T *fld, double *red, int &rpar, int mode, double b,
+                       double (&out)[3])
 {
     constexpr int F = 7;
     const int tid = threadIdx.x;
@@ -468,7 +475,7 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
         __syncthreads();
     }
     double v[3] = {v0, v1, v2};
-    wg_sum<3>(v, red);
+    wg_sum<3>(v, red, rpar);
     out[0] = v[0];
     out[1] = v[1];
     out[2] = v[2];
@@ -555,7 +562,7 @@ struct AState {
 // they live in LDS, the fields and the reduction slots.
 __host__ __device__ inline size_t batched_lds_layout(int N, int M, int nc, bool g_lds)
 {
-    return (size_t)5 * M * 16 + (g_lds ? (size_t)6 * nc * ((M >> 1) + 1) * 8 : 0) + (size_t)7 * N * nc * 8 + 64 * 8;
+    return (size_t)5 * M * 16 + (g_lds ? (size_t)6 * nc * ((M >> 1) + 1) * 8 : 0) + (size_t)7 * N * nc * 8 + 128 * 8;
 }
 
 template <int NC, bool REG>
@@ -621,6 +628,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
     const double picard_rtol = o.anderson_start, picard_atol = o.anderson_start;
     bool fp32 = (o.mixed != 0);
     int nevals = 0, nevals32 = 0;
+    int rpar = 0;                                          // which half of red the next sum uses
     double fo[3];
 
     for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = log(S.rho[i]); });   // x0 = log(rho)
@@ -713,7 +721,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                                 v1[(AM - 1) + j] += qv[j] * rv;
                             }
                     });
-                    wg_sum<2 * (AM - 1)>(v1, red);
+                    wg_sum<2 * (AM - 1)>(v1, red, rpar);
                     // pass 2: df -= Q h1; h2 = Q^T df
                     double v2[AM - 1];
 #pragma unroll
@@ -732,7 +740,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                         for (int j = 0; j < AM - 1; j++)
                             if (j < kqq) v2[j] += qv[j] * d;
                     });
-                    wg_sum<AM - 1>(v2, red);
+                    wg_sum<AM - 1>(v2, red, rpar);
                     // pass 3: df -= Q h2; |df|^2, df.res
                     double v3[2] = {0.0, 0.0};
                     for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) {
@@ -745,7 +753,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                         v3[0] += d * d;
                         v3[1] += d * A.res(b, i);
                     });
-                    wg_sum<2>(v3, red);
+                    wg_sum<2>(v3, red, rpar);
                     nrm = sqrt(v3[0]);
                     if (tid == 0) {
 #pragma unroll
@@ -798,10 +806,10 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
 
         // ---------------- the one map evaluation
         if (fp32) {
-            evalG<float, NC, REG>(S, A, pcs, tw, G, W32, P32, fld32, red, mode, bb, fo);
+            evalG<float, NC, REG>(S, A, pcs, tw, G, W32, P32, fld32, red, rpar, mode, bb, fo);
             nevals32++;
         } else {
-            evalG<double, NC, REG>(S, A, pcs, tw, G, W64, P64, fld64, red, mode, bb, fo);
+            evalG<double, NC, REG>(S, A, pcs, tw, G, W64, P64, fld64, red, rpar, mode, bb, fo);
         }
         nevals++;
         resnorm = sqrt(fo[0]);

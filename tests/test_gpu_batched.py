@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from cdft_amd import DFTOptions, _lib, configs, converge_
-from cdft_amd.methods import adsorption_system
+from cdft_amd.methods import adsorption_system, surface_tension_system
 from helpers import oracle_of
 
 pytestmark = pytest.mark.gpu
@@ -145,6 +145,11 @@ def test_batched_mixed_matches_plain_fp64(pores):
         ads_a = adsorption_system(s64, rp)
         ads_b = adsorption_system(s, b)
         assert abs(ads_b[0] - ads_a[0]) <= 1e-6 * abs(ads_a[0])
+        # excess grand potential per area (surface_tension(system, rho)), the tension of the pore walls
+        g_a = surface_tension_system(s64, rp)
+        g_b = surface_tension_system(s, b)
+        say(f"    adsorption rel {abs(ads_b[0] - ads_a[0]) / abs(ads_a[0]):.1e}  tension rel {abs(g_b - g_a) / abs(g_a):.1e}")
+        assert abs(g_b - g_a) <= 1e-6 * abs(g_a)
 
 
 def test_batched_mixed_maxit(pores):
