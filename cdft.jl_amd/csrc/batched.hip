@@ -341,14 +341,16 @@ CDFT_INLINE void pair_multipliers(const double *Gc, int G1, int M, int k, c2<T> 
 }
 
 // One evaluation of the fixed-point map plus the Anderson book-keeping of mode
-// (1: first/Picard step, 2: aa_point!).  out = |res|^2, |df|^2, df.res.
+// (1: first/Picard step, 2: aa_point!).  out = |res|^2, |df|^2, df.res.  The last
+// wave runs side() (whole wave) after its Phi points, before the barrier that ends
+// that phase.
 // LDS: tw (M, FP64), spectra G (6 NC (M/2+1) FP64, or global), three work arrays W
 // and the accumulator P (M complex T each, at FP64 stride), fields fld (F*N*NC of T
 // at FP64 stride), red (2 x 64 doubles, used alternately).
-template <typename T, int NC, bool REG>
+template <typename T, int NC, bool REG, class Side>
 CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, const c2<double> *tw, const double *G,
                        c2<T> *W, c2<T> *Pc, T *fld, double *red, int &rpar, int mode, double b,
-                       double (&out)[3])
+                       double (&out)[3], Side &&side)
 {
     constexpr int F = 7;
     const int tid = threadIdx.x;
@@ -413,6 +415,7 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
 #pragma unroll
             for (int f = 0; f < F; f++) fld[(c * F + f) * N + i] = g[c][f];
     }
+    if ((tid >> 6) == BT / 64 - 1) side();          // the last wave has the fewest Phi points
     __syncthreads();
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
     for (int c = 0; c < NC; c++) {
@@ -485,11 +488,11 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
 // R[:, 2:m]): Householder QR of the m x (m-1) matrix R[:, 1:m] with LAPACK geqrf
 // conventions (beta = -sign(alpha) |x|, tau = (beta - alpha) / beta, reflector
 // scaled to a unit leading entry), its explicit Q factor Qx (m x (m-1), column-major
-// ld m) and the new R (upper (m-1) x (m-1), last column zero).  Each wave holds
-// the matrix one element per lane, row i = lane & 7 and column j = lane >> 3, and
-// reduces over the 8 lanes of a column with xor shuffles.  Every wave computes the
-// same values; wave 0 writes them.  Called by all threads of the workgroup.
-CDFT_INLINE void dev_householder_downdate(double *R, int m, double *Qx)
+// ld m) and the new R (upper (m-1) x (m-1), last column zero), written over R in
+// LDS.  Run by one whole wave, which holds the matrix one element per lane, row
+// i = lane & 7 and column j = lane >> 3, and reduces over the 8 lanes of a column
+// with DPP steps.
+CDFT_INLINE void wave_downdate(double *R, int m, double *Qx)
 {
     const int lane = threadIdx.x & 63, i = lane & 7, j = lane >> 3;
     const int nr = m, ncol = m - 1;
@@ -523,27 +526,34 @@ CDFT_INLINE void dev_householder_downdate(double *R, int m, double *Qx)
         const double w = sum8(v * q) * tau[h];
         q -= w * v;
     }
-    __syncthreads();                                                // every wave has read R
-    if (threadIdx.x < 64) {
-        if (i < nr && j < ncol) Qx[j * nr + i] = q;
-        if (i < ncol && j < ncol) R[j * m + i] = (i <= j) ? a : 0.0;
-        if (j == 0 && i < m) R[(m - 1) * m + i] = 0.0;
-    }
+    // one wave: every lane read R above, before these writes
+    if (i < nr && j < ncol) Qx[j * nr + i] = q;
+    if (i < ncol && j < ncol) R[j * m + i] = (i <= j) ? a : 0.0;
+    if (j == 0 && i < m) R[(m - 1) * m + i] = 0.0;
 }
 
-// theta = R[0:mk, 0:mk] \ tres by column-oriented back substitution; lane l of every
-// wave ends holding theta_l (l < mk).  Called by all threads; R, tres in LDS.
-CDFT_INLINE double dev_triangular_solve(const double *R, const double *tres, int m, int mk)
+// theta = R[0:mk, 0:mk] \ tr by column-oriented back substitution.  The last column
+// (mk - 1) and tr come in registers (uniform over the workgroup), the other columns
+// from R in LDS; lane l of every wave ends holding theta_l (l < mk).
+CDFT_INLINE double dev_triangular_solve(const double *R, const double (&rc)[CDFT_MAXM], const double (&tr)[CDFT_MAXM],
+                                        int m, int mk)
 {
     const int lane = threadIdx.x & 63;
-    double x = (lane < mk) ? tres[lane] : 0.0;
+    double x = 0.0, last = 0.0;
+#pragma unroll
+    for (int l = 0; l < CDFT_MAXM; l++) {
+        x = (lane == l && l < mk) ? tr[l] : x;
+        last = (lane == l) ? rc[l] : last;
+    }
     double th = 0.0;
 #pragma unroll
     for (int j = CDFT_MAXM - 1; j >= 0; j--) {
         if (j >= mk) continue;
-        const double tj = readlane_f64(x, j) / R[j * m + j];
+        const bool lastc = (j == mk - 1);
+        const double rjj = lastc ? rc[j] : R[j * m + j];
+        const double tj = readlane_f64(x, j) / rjj;
         if (lane == j) th = tj;
-        if (lane < j) x -= R[j * m + lane] * tj;
+        if (lane < j) x -= (lastc ? last : R[j * m + lane]) * tj;
     }
     return th;
 }
@@ -552,9 +562,6 @@ CDFT_INLINE double dev_triangular_solve(const double *R, const double *tres, int
 struct AState {
     double R[CDFT_MAXM * CDFT_MAXM];
     double Qx[CDFT_MAXM * CDFT_MAXM];
-    double theta[CDFT_MAXM];
-    double tres[CDFT_MAXM];
-    int ring[CDFT_MAXM];
 };
 
 // Dynamic LDS bytes of one system (FP64 layout; the FP32 map uses the same slots):
@@ -649,10 +656,8 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
         if (phase == PH_PICARD) {
             bb = picard_beta;
         } else if (phase == PH_RESTART) {
-            if (tid == 0) {
+            if (tid == 0)
                 for (int j = 0; j < CDFT_MAXM * CDFT_MAXM; j++) st.R[j] = 0.0;
-                for (int j = 0; j < CDFT_MAXM; j++) st.ring[j] = j;
-            }
             for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });   // copy!(sol, gx)
             __syncthreads();
             k += 1;
@@ -665,35 +670,39 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                 for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });
                 __syncthreads();
             } else {
-                const int kk = ka + 1;
+                // aa_point! (anderson.jl) with the Delta-G history as a ring: the
+                // BuildDG! shift rotates the ring once per step past the m-th, so ring
+                // position j holds physical column (rot + j) mod m and the new column
+                // goes to (ka - 1) mod m.  Q and R follow aa_qr_update!; when the oldest
+                // column leaves, R and the rotation Qx were already downdated by the
+                // last wave during the map evaluation.  The new R column and Q^T res
+                // stay in registers (uniform), so the solve needs no barrier.
                 const int kq = ka - 1;
-                if (tid == 0 && m > 1 && kk > m + 1) {               // BuildDG! shift, as a ring
-                    int first = st.ring[0];
-                    for (int j = 0; j < m - 1; j++) st.ring[j] = st.ring[j + 1];
-                    st.ring[m - 1] = first;
-                }
-                __syncthreads();
-                const int slot = (m == 1) ? st.ring[0] : (kk > m + 1 ? st.ring[m - 1] : st.ring[kk - 2]);
-                for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.setDG(b, i, slot, A.dg(b, i)); });
+                const int rot = ka > m ? ka - m : 0;
+                const int slot = (ka - 1) % m;
+                const int mk = m < ka ? m : ka;
+                const int kqq = kq < m - 1 ? kq : m - 1;            // the new Q / R column
+                const bool down = (kq > m - 1);
                 double nrm;
-                int qcol;
+                double rc[CDFT_MAXM], tr[CDFT_MAXM];                // new R column, Q^T res
+#pragma unroll
+                for (int j = 0; j < CDFT_MAXM; j++) {
+                    rc[j] = 0.0;
+                    tr[j] = 0.0;
+                }
                 if (kq == 0) {                                      // aa_qr_update! first column
+                    for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.setDG(b, i, slot, A.dg(b, i)); });
                     nrm = sqrt(dfn2);
-                    if (tid == 0) {
-                        st.R[0] = nrm;
-                        st.tres[0] = dfres / nrm;
-                    }
-                    qcol = 0;
+                    rc[0] = nrm;
+                    tr[0] = dfres / nrm;
                 } else {
-                    const int down = (kq > m - 1);
-                    if (down) dev_householder_downdate(st.R, m, st.Qx);
-                    __syncthreads();
-                    const int kqq = kq < m - 1 ? kq : m - 1;
-                    // pass 1: optional downdate of Q (pointwise, in place); h1 = Q^T df, t = Q^T res
+                    // pass 1: store the Delta-G column; rotate Q (Q <- Q Qx) when the
+                    // oldest column left; h1 = Q^T df, t = Q^T res
                     double v1[2 * (AM - 1)];
 #pragma unroll
                     for (int j = 0; j < 2 * (AM - 1); j++) v1[j] = 0.0;
                     for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) {
+                        A.setDG(b, i, slot, A.dg(b, i));
                         double qv[AM];
                         if (down) {
                             double qo[AM];
@@ -755,21 +764,27 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                     });
                     wg_sum<2>(v3, red, rpar);
                     nrm = sqrt(v3[0]);
-                    if (tid == 0) {
 #pragma unroll
-                        for (int j = 0; j < AM - 1; j++)
-                            if (j < kqq) {
-                                st.R[kqq * m + j] = v1[j] + v2[j];
-                                st.tres[j] = v1[(AM - 1) + j];
-                            }
-                        st.R[kqq * m + kqq] = nrm;
-                        st.tres[kqq] = v3[1] / nrm;
-                    }
-                    qcol = kqq;
+                    for (int j = 0; j < AM - 1; j++)
+                        if (j < kqq) {
+                            rc[j] = v1[j] + v2[j];
+                            tr[j] = v1[(AM - 1) + j];
+                        }
+#pragma unroll
+                    for (int j = 0; j < AM; j++)
+                        if (j == kqq) {
+                            rc[j] = nrm;
+                            tr[j] = v3[1] / nrm;
+                        }
                 }
-                const int mk = m < ka ? m : ka;
-                __syncthreads();                                    // R, tres written by thread 0
-                const double thl = dev_triangular_solve(st.R, st.tres, m, mk);
+                // R's new column for the next downdate; nothing reads it in this step
+                if (tid < CDFT_MAXM && tid <= kqq) {
+                    double r = 0.0;
+#pragma unroll
+                    for (int j = 0; j < CDFT_MAXM; j++) r = (j == tid) ? rc[j] : r;
+                    st.R[kqq * m + tid] = r;
+                }
+                const double thl = dev_triangular_solve(st.R, rc, tr, m, mk);
                 // coefficient of each physical Delta-G column: theta of its ring position
                 double cf[AM], th[CDFT_MAXM];
 #pragma unroll
@@ -778,14 +793,14 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                 for (int j = 0; j < CDFT_MAXM; j++) {
                     const double thj = readlane_f64(thl, j);
                     th[j] = thj;
-                    const int col = (j < mk) ? st.ring[j] : -1;
+                    const int col = (j < mk) ? (rot + j) % m : -1;
 #pragma unroll
                     for (int s2 = 0; s2 < AM; s2++) cf[s2] = (s2 == col) ? thj : cf[s2];
                 }
                 // new Q column; sol = gx - DG theta
                 for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) {
                     const double dv = A.df(b, i);
-                    A.setQ(b, i, qcol, dv / nrm);
+                    A.setQ(b, i, kqq, dv / nrm);
                     double a = 0.0;
                     if constexpr (REG) {
 #pragma unroll
@@ -793,7 +808,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                     } else {
 #pragma unroll
                         for (int j = 0; j < CDFT_MAXM; j++)
-                            if (j < mk) a += A.getDG(b, i, st.ring[j]) * th[j];
+                            if (j < mk) a += A.getDG(b, i, (rot + j) % m) * th[j];
                     }
                     A.sol(b, i) = A.gx(b, i) - a;
                 });
@@ -805,11 +820,17 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
         }
 
         // ---------------- the one map evaluation
+        // the QR downdate the next Anderson step needs (it drops the oldest column
+        // when ka > m) depends only on R, so the last wave does it meanwhile
+        const bool dd = (phase == PH_STEP || phase == PH_RESTART) && m >= 1 && ka > m;
+        auto side = [&]() __attribute__((always_inline)) {
+            if (dd) wave_downdate(st.R, m, st.Qx);
+        };
         if (fp32) {
-            evalG<float, NC, REG>(S, A, pcs, tw, G, W32, P32, fld32, red, rpar, mode, bb, fo);
+            evalG<float, NC, REG>(S, A, pcs, tw, G, W32, P32, fld32, red, rpar, mode, bb, fo, side);
             nevals32++;
         } else {
-            evalG<double, NC, REG>(S, A, pcs, tw, G, W64, P64, fld64, red, rpar, mode, bb, fo);
+            evalG<double, NC, REG>(S, A, pcs, tw, G, W64, P64, fld64, red, rpar, mode, bb, fo, side);
         }
         nevals++;
         resnorm = sqrt(fo[0]);
