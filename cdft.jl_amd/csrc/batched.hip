@@ -98,10 +98,14 @@ CDFT_INLINE void radix4(c2<T> (&v)[4], T sg)
 // load0(idx, vals[NARR]) and the last pass hands its outputs to
 // storeL(idx, vals[NARR]); the passes between work in place on the LDS arrays
 // W + q M.  Unnormalised; sign -1 forward, +1 inverse.  Ends with a barrier.
-template <typename T, int NARR, class Load0, class StoreL>
+// PAD: array q starts at q (M + M/16) and element x sits at x + x/16, one spare
+// slot per 16, which spreads the strided stores of the early passes over the banks.
+template <typename T, int NARR, bool PAD = false, class Load0, class StoreL>
 CDFT_INLINE void fft_fused(c2<T> *W, const c2<double> *tw, int M, int lg, bool inverse, Load0 &&load0,
                            StoreL &&storeL)
 {
+    const int MA = PAD ? M + (M >> 4) : M;
+    auto wi = [&](int q, int x) __attribute__((always_inline)) { return q * MA + (PAD ? x + (x >> 4) : x); };
     int j = threadIdx.x;
     asm volatile("" : "+v"(j));           // opaque: keeps the index arithmetic inside the solver loop
     const T sg = inverse ? T(1) : T(-1);
@@ -126,7 +130,7 @@ CDFT_INLINE void fft_fused(c2<T> *W, const c2<double> *tw, int M, int lg, bool i
 #pragma unroll
                 for (int q = 0; q < NARR; q++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) v[q][r] = W[q * M + j + r * nb];
+                    for (int r = 0; r < 4; r++) v[q][r] = W[wi(q, j + r * nb)];
                 const c2<double> wd = tw[k << (lg - 2 - lNs)];
                 const c2<T> w1 = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
                 const c2<T> w2 = cmul(w1, w1), w3 = cmul(w1, w2);
@@ -155,7 +159,7 @@ CDFT_INLINE void fft_fused(c2<T> *W, const c2<double> *tw, int M, int lg, bool i
 #pragma unroll
                 for (int q = 0; q < NARR; q++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) W[q * M + d + r * Ns] = v[q][r];
+                    for (int r = 0; r < 4; r++) W[wi(q, d + r * Ns)] = v[q][r];
             }
         }
         __syncthreads();
@@ -171,7 +175,7 @@ CDFT_INLINE void fft_fused(c2<T> *W, const c2<double> *tw, int M, int lg, bool i
             const c2<T> w = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
 #pragma unroll
             for (int q = 0; q < NARR; q++) {
-                const c2<T> x0 = W[q * M + j], x1 = cmul(W[q * M + j + nb], w);
+                const c2<T> x0 = W[wi(q, j)], x1 = cmul(W[wi(q, j + nb)], w);
                 v[q][0] = {x0.x + x1.x, x0.y + x1.y};
                 v[q][1] = {x0.x - x1.x, x0.y - x1.y};
             }
@@ -340,17 +344,59 @@ CDFT_INLINE void pair_multipliers(const double *Gc, int G1, int M, int k, c2<T> 
     Z[2] = {T(Gc[4 * G1 + kk]), T(Gc[5 * G1 + kk])};
 }
 
+// Phi and dPhi/dn at the points tid, tid + nthr, tid + 2 nthr, ... (threads
+// tid < nthr; nthr a multiple of 64), in place over the field slots.  A wave whose
+// first lane has a second point evaluates two points per lane side by side (a lane
+// without one repeats its first and discards it): with CHAIN = false the two
+// inlined bodies are branch free and interleave.
+template <typename T, int NC, bool CHAIN>
+CDFT_INLINE void phi_points(const PhiConst &P, T *fld, int N, int nthr)
+{
+    constexpr int F = 7;
+    const int tid = threadIdx.x;
+    auto load = [&](T (&nl)[NC][F], int i) __attribute__((always_inline)) {
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int f = 0; f < F; f++) nl[c][f] = fld[(c * F + f) * N + i];
+    };
+    auto store = [&](const T (&g)[NC][F], int i) __attribute__((always_inline)) {
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int f = 0; f < F; f++) fld[(c * F + f) * N + i] = g[c][f];
+    };
+    for (int i = tid; i < N; i += 2 * nthr) {
+        T nl[NC][F], g[NC][F];
+        load(nl, i);
+        if (i - (tid & 63) + nthr < N) {             // uniform over the wave
+            const int i2 = i + nthr;
+            const bool two = i2 < N;
+            T nl2[NC][F], g2[NC][F];
+            load(nl2, two ? i2 : i);
+            phi_grad<T, double, NC, 1, CHAIN>(P, nl, g);
+            phi_grad<T, double, NC, 1, CHAIN>(P, nl2, g2);
+            store(g, i);
+            if (two) store(g2, i2);
+        } else {
+            phi_grad<T, double, NC, 1, CHAIN>(P, nl, g);
+            store(g, i);
+        }
+    }
+}
+
 // One evaluation of the fixed-point map plus the Anderson book-keeping of mode
 // (1: first/Picard step, 2: aa_point!).  out = |res|^2, |df|^2, df.res.  The last
 // wave runs side() (whole wave) after its Phi points, before the barrier that ends
-// that phase.
+// that phase; with side_on it gets no Phi points when the other waves can take
+// them all in as many rounds.
 // LDS: tw (M, FP64), spectra G (6 NC (M/2+1) FP64, or global), three work arrays W
 // and the accumulator P (M complex T each, at FP64 stride), fields fld (F*N*NC of T
 // at FP64 stride), red (2 x 64 doubles, used alternately).
 template <typename T, int NC, bool REG, class Side>
 CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, const c2<double> *tw, const double *G,
                        c2<T> *W, c2<T> *Pc, T *fld, double *red, int &rpar, int mode, double b,
-                       double (&out)[3], Side &&side)
+                       double (&out)[3], bool side_on, Side &&side)
 {
     constexpr int F = 7;
     const int tid = threadIdx.x;
@@ -403,17 +449,16 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
             });
     }
     // Phi and dPhi/dn, in place over the field slots
-    for (int i = tid; i < N; i += BT) {
-        T nl[NC][F], g[NC][F];
+    {
+        const int r_all = (N + 2 * BT - 1) / (2 * BT), r_part = (N + 2 * (BT - 64) - 1) / (2 * (BT - 64));
+        const int nthr = (side_on && r_part <= r_all) ? BT - 64 : BT;
+        bool chain = false;
 #pragma unroll
-        for (int c = 0; c < NC; c++)
-#pragma unroll
-            for (int f = 0; f < F; f++) nl[c][f] = fld[(c * F + f) * N + i];
-        phi_grad<T, double, NC, 1>(P, nl, g);
-#pragma unroll
-        for (int c = 0; c < NC; c++)
-#pragma unroll
-            for (int f = 0; f < F; f++) fld[(c * F + f) * N + i] = g[c][f];
+        for (int c = 0; c < NC; c++) chain = chain || (P.m[c] != 1.0);
+        if (tid < nthr) {
+            if (chain) phi_points<T, NC, true>(P, fld, N, nthr);
+            else phi_points<T, NC, false>(P, fld, N, nthr);
+        }
     }
     if ((tid >> 6) == BT / 64 - 1) side();          // the last wave has the fewest Phi points
     __syncthreads();
@@ -827,10 +872,10 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
             if (dd) wave_downdate(st.R, m, st.Qx);
         };
         if (fp32) {
-            evalG<float, NC, REG>(S, A, pcs, tw, G, W32, P32, fld32, red, rpar, mode, bb, fo, side);
+            evalG<float, NC, REG>(S, A, pcs, tw, G, W32, P32, fld32, red, rpar, mode, bb, fo, dd, side);
             nevals32++;
         } else {
-            evalG<double, NC, REG>(S, A, pcs, tw, G, W64, P64, fld64, red, rpar, mode, bb, fo, side);
+            evalG<double, NC, REG>(S, A, pcs, tw, G, W64, P64, fld64, red, rpar, mode, bb, fo, dd, side);
         }
         nevals++;
         resnorm = sqrt(fo[0]);

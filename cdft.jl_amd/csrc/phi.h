@@ -82,8 +82,10 @@ __device__ __forceinline__ void i_lite(const double (&corr)[7][3], T mb, T eta, 
     Ie = ve;
 }
 
-// Phi at one point; g receives dPhi/dn for every (component, slot).
-template <typename T, typename TS, int NC, int ND>
+// Phi at one point; g receives dPhi/dn for every (component, slot).  CHAIN = false
+// is for callers that know every segment number is 1 (no chain term): the body is
+// then free of branches, so two inlined calls can be scheduled side by side.
+template <typename T, typename TS, int NC, int ND, bool CHAIN = true>
 __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + ND], T (&g)[NC][6 + ND])
 {
     constexpr int F = 6 + ND;
@@ -121,16 +123,18 @@ __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + N
     TS dn = TS(1) - n3;
     TS log1dn = log(fabs(dn));
     TS inv1dn = TS(1) / dn;
-    TS bkt, dB;
-    if (std::is_same<T, float>::value && fabs(n3) < TS(1e-5)) {
+    TS bkt = log1dn / (twelvepi * n3 * n3) + TS(1) / (twelvepi * n3 * dn * dn);
+    TS dB = (TS(-1) / (n3 * n3 * dn) - TS(2) * log1dn / (n3 * n3 * n3) +
+             (TS(3) * n3 - TS(1)) / (n3 * n3 * dn * dn * dn)) / twelvepi;
+    if (std::is_same<T, float>::value) {
         // FP32 inputs: n3 is at rounding-noise level inside walls and may be 0.
-        // Series of the same bracket, sum_k ((k+2)^2 - 1)/(k+2) n3^k / (12 pi).
-        bkt = (TS(1.5) + n3 * (TS(8.0 / 3.0) + n3 * (TS(3.75) + n3 * TS(4.8)))) / twelvepi;
-        dB = (TS(8.0 / 3.0) + n3 * (TS(7.5) + n3 * TS(14.4))) / twelvepi;
-    } else {
-        bkt = log1dn / (twelvepi * n3 * n3) + TS(1) / (twelvepi * n3 * dn * dn);
-        dB = (TS(-1) / (n3 * n3 * dn) - TS(2) * log1dn / (n3 * n3 * n3) +
-              (TS(3) * n3 - TS(1)) / (n3 * n3 * dn * dn * dn)) / twelvepi;
+        // Series of the same bracket, sum_k ((k+2)^2 - 1)/(k+2) n3^k / (12 pi);
+        // selected rather than branched to, so the body stays one block.
+        const bool small = fabs(n3) < TS(1e-5);
+        const TS bs = (TS(1.5) + n3 * (TS(8.0 / 3.0) + n3 * (TS(3.75) + n3 * TS(4.8)))) / twelvepi;
+        const TS ds = (TS(8.0 / 3.0) + n3 * (TS(7.5) + n3 * TS(14.4))) / twelvepi;
+        bkt = small ? bs : bkt;
+        dB = small ? ds : dB;
     }
     T lg = T(log1dn), iv = T(inv1dn), bk = T(bkt);
     T n2c = n2 * n2 * n2 / T(3) - n2 * nv2sq;
@@ -155,42 +159,44 @@ __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + N
     }
 
     // ---- hard chain ----
-    T z3 = 0, z2 = 0;
+    if (CHAIN) {
+        T z3 = 0, z2 = 0;
 #pragma unroll
-    for (int i = 0; i < NC; i++) {
-        T rh = nl[i][IZ];
-        z3 += T(P.m[i]) * rh;
-        z2 += T(P.m[i]) * rh / T(P.dd[i]);
-    }
-    z3 /= T(8);
-    z2 /= T(8);
-    T inv = T(1) / (T(1) - z3);
-    T G3 = 0, G2 = 0;
-#pragma unroll
-    for (int i = 0; i < NC; i++) {
-        T mi1 = T(P.m[i]) - T(1);
-        if (P.m[i] == 1.0) continue;     // the chain term and its gradient vanish exactly for m = 1
-        T di = T(P.dd[i]);
-        T rhoi = nl[i][0];
-        T lsl = nl[i][IL];
-        if (std::is_same<T, float>::value) {   // FP32: keep underflowed densities off 0/0
-            rhoi = fmax(rhoi, T(1e-30));
-            lsl = copysign(fmax(fabs(lsl), T(1e-30)), lsl);
+        for (int i = 0; i < NC; i++) {
+            T rh = nl[i][IZ];
+            z3 += T(P.m[i]) * rh;
+            z2 += T(P.m[i]) * rh / T(P.dd[i]);
         }
-        T lam = lsl / (T(2) * di);
-        T y = inv + T(1.5) * di * z2 * inv * inv + T(0.5) * di * di * z2 * z2 * inv * inv * inv;
-        T l = T(log(fabs(TS(y * lam / rhoi))));
-        phi += -rhoi * mi1 * l;
-        g[i][0] += -mi1 * (l - T(1));
-        g[i][IL] += -rhoi * mi1 / lsl;
-        T gy = -rhoi * mi1 / y;
-        T dy3 = inv * inv + T(3) * di * z2 * inv * inv * inv + T(1.5) * di * di * z2 * z2 * inv * inv * inv * inv;
-        T dy2 = T(1.5) * di * inv * inv + di * di * z2 * inv * inv * inv;
-        G3 += gy * dy3;
-        G2 += gy * dy2;
-    }
+        z3 /= T(8);
+        z2 /= T(8);
+        T inv = T(1) / (T(1) - z3);
+        T G3 = 0, G2 = 0;
 #pragma unroll
-    for (int k = 0; k < NC; k++) g[k][IZ] += T(P.m[k]) / T(8) * G3 + T(P.m[k]) / (T(8) * T(P.dd[k])) * G2;
+        for (int i = 0; i < NC; i++) {
+            T mi1 = T(P.m[i]) - T(1);
+            if (P.m[i] == 1.0) continue;     // the chain term and its gradient vanish exactly for m = 1
+            T di = T(P.dd[i]);
+            T rhoi = nl[i][0];
+            T lsl = nl[i][IL];
+            if (std::is_same<T, float>::value) {   // FP32: keep underflowed densities off 0/0
+                rhoi = fmax(rhoi, T(1e-30));
+                lsl = copysign(fmax(fabs(lsl), T(1e-30)), lsl);
+            }
+            T lam = lsl / (T(2) * di);
+            T y = inv + T(1.5) * di * z2 * inv * inv + T(0.5) * di * di * z2 * z2 * inv * inv * inv;
+            T l = T(log(fabs(TS(y * lam / rhoi))));
+            phi += -rhoi * mi1 * l;
+            g[i][0] += -mi1 * (l - T(1));
+            g[i][IL] += -rhoi * mi1 / lsl;
+            T gy = -rhoi * mi1 / y;
+            T dy3 = inv * inv + T(3) * di * z2 * inv * inv * inv + T(1.5) * di * di * z2 * z2 * inv * inv * inv * inv;
+            T dy2 = T(1.5) * di * inv * inv + di * di * z2 * inv * inv * inv;
+            G3 += gy * dy3;
+            G2 += gy * dy2;
+        }
+#pragma unroll
+        for (int k = 0; k < NC; k++) g[k][IZ] += T(P.m[k]) / T(8) * G3 + T(P.m[k]) / (T(8) * T(P.dd[k])) * G2;
+    }
 
     // ---- dispersion ----
     const T psi = T(1.3862);
