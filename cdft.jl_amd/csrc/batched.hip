@@ -328,20 +328,35 @@ CDFT_INLINE void for_points(int n, Fn &&fn)
     }
 }
 
-// Spectral multipliers of the three field pairs of one component at ring index k:
-// Z_q = H_{1+2q} + i H_{2+2q} with H_f = G_f (even kernels) or i G_f (the vector
-// weight, slot 3), from the half-spectra Gc[(f-1)*G1 + k'] with k' = min(k, M-k)
-// and G_f(M-k) = +-G_f(k).  The transposed operator uses conj(Z_q): the vector
-// weight's transpose is its negative, the scalar weights are symmetric.
-template <typename T>
-CDFT_INLINE void pair_multipliers(const double *Gc, int G1, int M, int k, c2<T> (&Z)[3])
+// Field-slot pairs of the convolutions, slot 2q+1 (real part) and 2q+2 (imaginary
+// part) of pair q when NP = 3.  A chain-free system (every segment number 1) needs
+// neither the zeta (4) nor the lambda (5) slot: Phi does not read them and their
+// gradients vanish, so with NP = 2 the pairs are (1, 2) and (3, 6).
+template <int NP>
+CDFT_INLINE int pair_slot(int q, int h)
+{
+    return NP == 3 ? 1 + 2 * q + h : (q == 0 ? 1 + h : (h ? 6 : 3));
+}
+
+// Spectral multipliers of the NP field pairs of one component at ring index k:
+// Z_q = H_a + i H_b for the pair's slots (a, b), with H_f = G_f (even kernels) or
+// i G_f (the vector weight, slot 3), from the half-spectra Gc[(f-1)*G1 + k'] with
+// k' = min(k, M-k) and G_f(M-k) = +-G_f(k).  The transposed operator uses
+// conj(Z_q): the vector weight's transpose is its negative, the scalar weights are
+// symmetric.
+template <typename T, int NP>
+CDFT_INLINE void pair_multipliers(const double *Gc, int G1, int M, int k, c2<T> (&Z)[NP])
 {
     const bool up = k > (M >> 1);
     const int kk = up ? M - k : k;
     const double g3 = up ? -Gc[2 * G1 + kk] : Gc[2 * G1 + kk];
     Z[0] = {T(Gc[kk]), T(Gc[G1 + kk])};
-    Z[1] = {T(0), T(g3 + Gc[3 * G1 + kk])};
-    Z[2] = {T(Gc[4 * G1 + kk]), T(Gc[5 * G1 + kk])};
+    if constexpr (NP == 3) {
+        Z[1] = {T(0), T(g3 + Gc[3 * G1 + kk])};
+        Z[2] = {T(Gc[4 * G1 + kk]), T(Gc[5 * G1 + kk])};
+    } else {
+        Z[1] = {T(0), T(g3 + Gc[5 * G1 + kk])};
+    }
 }
 
 // Phi and dPhi/dn at the points tid, tid + nthr, tid + 2 nthr, ... (threads
@@ -403,6 +418,11 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
     const int N = S.N, M = S.M, lg = S.lg, G1 = (S.M >> 1) + 1;
     const T invM = T(1) / T(M);
     const T sc = T(S.NA) * invM;
+    bool chain = false;
+#pragma unroll
+    for (int c = 0; c < NC; c++) chain = chain || (P.m[c] != 1.0);
+    using NP3 = std::integral_constant<int, 3>;
+    using NP2 = std::integral_constant<int, 2>;
 
     for (int c = 0; c < NC; c++) {
         const double *Gc = G + (size_t)c * (F - 1) * G1;
@@ -427,34 +447,36 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
             W, tw, M, lg, false,
             [&](int idx, c2<T>(&v)[1]) __attribute__((always_inline)) { v[0] = Pc[idx]; },
             [&](int idx, const c2<T>(&v)[1]) __attribute__((always_inline)) { Pc[idx] = v[0]; });
-        // n_{1+2q} + i n_{2+2q} = IFFT(P Z_q): the multiply feeds the first pass, the
-        // last pass writes the weighted densities
-        fft_fused<T, 3>(
-            W, tw, M, lg, true,
-            [&](int idx, c2<T>(&v)[3]) __attribute__((always_inline)) {
-                c2<T> Z[3];
-                pair_multipliers<T>(Gc, G1, M, idx, Z);
-                const c2<T> p = Pc[idx];
+        // n_a + i n_b = IFFT(P Z_q) for the slot pairs: the multiply feeds the first
+        // pass, the last pass writes the weighted densities
+        auto weighted = [&](auto np) __attribute__((always_inline)) {
+            constexpr int NP = decltype(np)::value;
+            fft_fused<T, NP>(
+                W, tw, M, lg, true,
+                [&](int idx, c2<T>(&v)[NP]) __attribute__((always_inline)) {
+                    c2<T> Z[NP];
+                    pair_multipliers<T, NP>(Gc, G1, M, idx, Z);
+                    const c2<T> p = Pc[idx];
 #pragma unroll
-                for (int q = 0; q < 3; q++) v[q] = cmul(p, Z[q]);
-            },
-            [&](int idx, const c2<T>(&v)[3]) __attribute__((always_inline)) {
-                if (idx < N) {
+                    for (int q = 0; q < NP; q++) v[q] = cmul(p, Z[q]);
+                },
+                [&](int idx, const c2<T>(&v)[NP]) __attribute__((always_inline)) {
+                    if (idx < N) {
 #pragma unroll
-                    for (int q = 0; q < 3; q++) {
-                        fld[(c * F + 1 + 2 * q) * N + idx] = sc * v[q].x;
-                        fld[(c * F + 2 + 2 * q) * N + idx] = sc * v[q].y;
+                        for (int q = 0; q < NP; q++) {
+                            fld[(c * F + pair_slot<NP>(q, 0)) * N + idx] = sc * v[q].x;
+                            fld[(c * F + pair_slot<NP>(q, 1)) * N + idx] = sc * v[q].y;
+                        }
                     }
-                }
-            });
+                });
+        };
+        if (chain) weighted(NP3{});
+        else weighted(NP2{});
     }
     // Phi and dPhi/dn, in place over the field slots
     {
         const int r_all = (N + 2 * BT - 1) / (2 * BT), r_part = (N + 2 * (BT - 64) - 1) / (2 * (BT - 64));
         const int nthr = (side_on && r_part <= r_all) ? BT - 64 : BT;
-        bool chain = false;
-#pragma unroll
-        for (int c = 0; c < NC; c++) chain = chain || (P.m[c] != 1.0);
         if (tid < nthr) {
             if (chain) phi_points<T, NC, true>(P, fld, N, nthr);
             else phi_points<T, NC, false>(P, fld, N, nthr);
@@ -465,27 +487,34 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
     for (int c = 0; c < NC; c++) {
         const double *Gc = G + (size_t)c * (F - 1) * G1;
-        // transposed convolutions: Re IFFT(sum_q FFT(u_{1+2q} + i u_{2+2q}) conj(Z_q)); the
-        // first pass reads the gradient pairs, the last one accumulates the spectrum
-        fft_fused<T, 3>(
-            W, tw, M, lg, false,
-            [&](int idx, c2<T>(&v)[3]) __attribute__((always_inline)) {
+        // transposed convolutions: Re IFFT(sum_q FFT(u_a + i u_b) conj(Z_q)) over the slot
+        // pairs; the first pass reads the gradient pairs, the last one accumulates the
+        // spectrum
+        auto transposed = [&](auto np) __attribute__((always_inline)) {
+            constexpr int NP = decltype(np)::value;
+            fft_fused<T, NP>(
+                W, tw, M, lg, false,
+                [&](int idx, c2<T>(&v)[NP]) __attribute__((always_inline)) {
 #pragma unroll
-                for (int q = 0; q < 3; q++)
-                    v[q] = (idx < N) ? c2<T>{fld[(c * F + 1 + 2 * q) * N + idx], fld[(c * F + 2 + 2 * q) * N + idx]}
-                                     : c2<T>{T(0), T(0)};
-            },
-            [&](int idx, const c2<T>(&v)[3]) __attribute__((always_inline)) {
-                c2<T> Z[3];
-                pair_multipliers<T>(Gc, G1, M, idx, Z);
-                c2<T> acc = {T(0), T(0)};
+                    for (int q = 0; q < NP; q++)
+                        v[q] = (idx < N) ? c2<T>{fld[(c * F + pair_slot<NP>(q, 0)) * N + idx],
+                                                 fld[(c * F + pair_slot<NP>(q, 1)) * N + idx]}
+                                         : c2<T>{T(0), T(0)};
+                },
+                [&](int idx, const c2<T>(&v)[NP]) __attribute__((always_inline)) {
+                    c2<T> Z[NP];
+                    pair_multipliers<T, NP>(Gc, G1, M, idx, Z);
+                    c2<T> acc = {T(0), T(0)};
 #pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    acc.x += v[q].x * Z[q].x + v[q].y * Z[q].y;          // v * conj(Z)
-                    acc.y += v[q].y * Z[q].x - v[q].x * Z[q].y;
-                }
-                Pc[idx] = acc;
-            });
+                    for (int q = 0; q < NP; q++) {
+                        acc.x += v[q].x * Z[q].x + v[q].y * Z[q].y;      // v * conj(Z)
+                        acc.y += v[q].y * Z[q].x - v[q].x * Z[q].y;
+                    }
+                    Pc[idx] = acc;
+                });
+        };
+        if (chain) transposed(NP3{});
+        else transposed(NP2{});
         fft_fused<T, 1>(
             W, tw, M, lg, true,
             [&](int idx, c2<T>(&v)[1]) __attribute__((always_inline)) { v[0] = Pc[idx]; },
