@@ -282,6 +282,18 @@ struct AStore {
     CDFT_INLINE double &dg(int b, int i) { if constexpr (REG) return dg_[b]; else return gdg[i]; }
     // column j known at compile time after unrolling (j < AM for REG)
     CDFT_INLINE double &Q(int b, int i, int j) { if constexpr (REG) return Q_[b][j]; else return gQ[j * n + i]; }
+    // Q column j when live (registers: read regardless, the caller selects)
+    CDFT_INLINE double Qif(int b, int i, int j, bool live)
+    {
+        if constexpr (REG) return Q_[b][j];
+        else return live ? gQ[j * n + i] : 0.0;
+    }
+    // Q column j = v when c (registers: a select, so no branch splits the caller)
+    CDFT_INLINE void setQif(int b, int i, int j, bool c, double v)
+    {
+        if constexpr (REG) Q_[b][j] = c ? v : Q_[b][j];
+        else if (c) gQ[j * n + i] = v;
+    }
     CDFT_INLINE double getDG(int b, int i, int col)
     {
         if constexpr (REG) {
@@ -312,6 +324,10 @@ struct AStore {
         }
     }
 };
+
+// a + x y when c, else a: a select rather than a branch, so that the loops over
+// history columns stay one basic block (same rounding as the guarded a += x y).
+CDFT_INLINE double fma_if(bool c, double a, double x, double y) { return c ? a + x * y : a; }
 
 // Calls fn(b, i) for the points i < n of this thread: i = tid + b BT.
 template <bool REG, typename Fn>
@@ -752,8 +768,9 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                 // last wave during the map evaluation.  The new R column and Q^T res
                 // stay in registers (uniform), so the solve needs no barrier.
                 const int kq = ka - 1;
-                const int rot = ka > m ? ka - m : 0;
-                const int slot = (ka - 1) % m;
+                const int rot = ka > m ? (ka - m) % m : 0;         // ring offset, < m
+                const int pos = ka > m ? m - 1 : ka - 1;            // ring position of the new column
+                const int slot = rot + pos - (rot + pos >= m ? m : 0);
                 const int mk = m < ka ? m : ka;
                 const int kqq = kq < m - 1 ? kq : m - 1;            // the new Q / R column
                 const bool down = (kq > m - 1);
@@ -786,23 +803,21 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                             for (int j = 0; j < AM - 1; j++) {
                                 double a = 0.0;
 #pragma unroll
-                                for (int l = 0; l < AM; l++)
-                                    if (l < m) a += qo[l] * st.Qx[j * m + l];
+                                for (int l = 0; l < AM; l++) a = fma_if(l < m, a, qo[l], st.Qx[j * m + l]);
                                 qv[j] = a;
-                                if (j < m - 1) A.Q(b, i, j) = a;
+                                A.setQif(b, i, j, j < m - 1, a);
                             }
                             A.setQ(b, i, m - 1, 0.0);
                         } else {
 #pragma unroll
-                            for (int j = 0; j < AM - 1; j++) qv[j] = (j < kqq) ? A.Q(b, i, j) : 0.0;
+                            for (int j = 0; j < AM - 1; j++) qv[j] = (j < kqq) ? A.Qif(b, i, j, j < kqq) : 0.0;
                         }
                         const double dv = A.df(b, i), rv = A.res(b, i);
 #pragma unroll
-                        for (int j = 0; j < AM - 1; j++)
-                            if (j < kqq) {
-                                v1[j] += qv[j] * dv;
-                                v1[(AM - 1) + j] += qv[j] * rv;
-                            }
+                        for (int j = 0; j < AM - 1; j++) {
+                            v1[j] = fma_if(j < kqq, v1[j], qv[j], dv);
+                            v1[(AM - 1) + j] = fma_if(j < kqq, v1[(AM - 1) + j], qv[j], rv);
+                        }
                     });
                     wg_sum<2 * (AM - 1)>(v1, red, rpar);
                     // pass 2: df -= Q h1; h2 = Q^T df
@@ -812,16 +827,14 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                     for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) {
                         double qv[AM], a = 0.0;
 #pragma unroll
-                        for (int j = 0; j < AM - 1; j++)
-                            if (j < kqq) {
-                                qv[j] = A.Q(b, i, j);
-                                a += qv[j] * v1[j];
-                            }
+                        for (int j = 0; j < AM - 1; j++) {
+                            qv[j] = A.Qif(b, i, j, j < kqq);
+                            a = fma_if(j < kqq, a, qv[j], v1[j]);
+                        }
                         const double d = A.df(b, i) - a;
                         A.df(b, i) = d;
 #pragma unroll
-                        for (int j = 0; j < AM - 1; j++)
-                            if (j < kqq) v2[j] += qv[j] * d;
+                        for (int j = 0; j < AM - 1; j++) v2[j] = fma_if(j < kqq, v2[j], qv[j], d);
                     });
                     wg_sum<AM - 1>(v2, red, rpar);
                     // pass 3: df -= Q h2; |df|^2, df.res
@@ -829,8 +842,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                     for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) {
                         double a = 0.0;
 #pragma unroll
-                        for (int j = 0; j < AM - 1; j++)
-                            if (j < kqq) a += A.Q(b, i, j) * v2[j];
+                        for (int j = 0; j < AM - 1; j++) a = fma_if(j < kqq, a, A.Qif(b, i, j, j < kqq), v2[j]);
                         const double d = A.df(b, i) - a;
                         A.df(b, i) = d;
                         v3[0] += d * d;
@@ -867,7 +879,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                 for (int j = 0; j < CDFT_MAXM; j++) {
                     const double thj = readlane_f64(thl, j);
                     th[j] = thj;
-                    const int col = (j < mk) ? (rot + j) % m : -1;
+                    const int col = (j < mk) ? rot + j - (rot + j >= m ? m : 0) : -1;
 #pragma unroll
                     for (int s2 = 0; s2 < AM; s2++) cf[s2] = (s2 == col) ? thj : cf[s2];
                 }
@@ -882,7 +894,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                     } else {
 #pragma unroll
                         for (int j = 0; j < CDFT_MAXM; j++)
-                            if (j < mk) a += A.getDG(b, i, (rot + j) % m) * th[j];
+                            if (j < mk) a += A.getDG(b, i, rot + j - (rot + j >= m ? m : 0)) * th[j];
                     }
                     A.sol(b, i) = A.gx(b, i) - a;
                 });
