@@ -94,7 +94,7 @@ void free_sys_buffers(cdft_sys *s)
     for (double **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (s->H64) { (void)hipFree(s->H64); s->H64 = nullptr; }
-    double **b3[] = {&s->G3, &s->A3, &s->B3, &s->S3};
+    double **b3[] = {&s->G3, &s->A3, &s->B3, &s->S3, &s->B3x[0], &s->B3x[1], &s->B3x[2]};
     for (double **b : b3)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     float **f3[] = {&s->G3f, &s->n32, &s->dphi32};
@@ -188,6 +188,49 @@ Pass3 pair_pass(const cdft_sys *s, int c, int q)
     return p;
 }
 
+// The transposed all-pairs z pass (k_fft3z_accum) of component c: spectra and the
+// pairs' work arrays.
+template <typename T>
+ZPairs3 zpairs_for(const cdft_sys *s, int c)
+{
+    ZPairs3 p;
+    std::memset(&p, 0, sizeof(p));
+    double *bq[4] = {s->B3, s->B3x[0], s->B3x[1], s->B3x[2]};
+    p.np = 4;
+    for (int q = 0; q < 4; q++) {
+        const Pass3 pq = pair_pass<T>(s, c, q);
+        p.g1[q] = pq.g1;
+        p.g2[q] = pq.g2;
+        p.v1[q] = pq.v1;
+        p.v2[q] = pq.v2;
+        p.b[q] = bq[q];
+    }
+    p.tw = reinterpret_cast<const cx<double> *>(s->ctx->twiddles.at(s->g3.n[2]));
+    return p;
+}
+
+template <typename T, int EPT>
+void zpairs_launch_ept(const cdft_sys *s, const ZPairs3 &p)
+{
+    const Grid3 &G = s->g3;
+    const size_t lds = (size_t)2 * G.n[2] * TBZ * sizeof(cx<T>);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_fft3z_accum<T, EPT>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    const long nblk = (long)(G.n[0] / TBZ) * G.n[1];
+    hipLaunchKernelGGL((k_fft3z_accum<T, EPT>), dim3((unsigned)nblk), dim3(256), lds, s->ctx->stream, G, p);
+}
+
+template <typename T>
+void zpairs_launch(const cdft_sys *s, const ZPairs3 &p)
+{
+    if (s->g3.n[2] == 256) zpairs_launch_ept<T, 2>(s, p);
+    else zpairs_launch_ept<T, 1>(s, p);                           // 128
+}
+
 // Weighted densities of src (FP64, layout [c N + k]) into nout[(c F + f) N + k],
 // f >= 1, computed in T.
 template <typename T>
@@ -219,11 +262,38 @@ void launch_weighted3d(cdft_sys *s, const double *src, T *nout)
 }
 
 // dtot[c N + k] = dphi[(c F) N + k] + sum_f (transposed convolution of dphi_f), in T.
+// With the extra work arrays the four accumulating z passes and the first inverse z
+// pass are one all-pairs launch.
 template <typename T>
 void launch_transposed3d(cdft_sys *s, const T *dphi, double *dtot)
 {
     const int N = s->N, F = s->F;
     for (int c = 0; c < s->nc; c++) {
+        if (s->B3x[2]) {
+            ZPairs3 zp = zpairs_for<T>(s, c);
+            double *bq[4] = {s->B3, s->B3x[0], s->B3x[1], s->B3x[2]};
+            for (int q = 0; q < 4; q++) {
+                Pass3 pq = pair_pass<T>(s, c, q);
+                pq.in = dphi + (size_t)(c * F + kPair3[q][0]) * N;
+                pq.in2 = dphi + (size_t)(c * F + kPair3[q][1]) * N;
+                pq.out = bq[q];
+                fft3_pass<T, 0, IN_PAIR, OUT_CPLX>(s, pq, false);
+                pq.in = bq[q];
+                fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, pq, false);
+            }
+            zp.out = s->S3;
+            zpairs_launch<T>(s, zp);
+            Pass3 p;
+            std::memset(&p, 0, sizeof(p));
+            p.in = s->S3;
+            p.out = s->S3;
+            fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, p, true);
+            p.out = dtot + (size_t)c * N;
+            p.base = dphi + (size_t)(c * F) * N;
+            p.scale = 1.0 / (double)N;
+            fft3_pass<T, 0, IN_CPLX, OUT_REAL>(s, p, true);
+            continue;
+        }
         for (int q = 0; q < 4; q++) {
             Pass3 pq = pair_pass<T>(s, c, q);
             pq.in = dphi + (size_t)(c * F + kPair3[q][0]) * N;
@@ -500,6 +570,16 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
         delete s;
         return fail(ctx, e == hipErrorOutOfMemory ? CDFT_ERR_NOMEM : CDFT_ERR_HIP,
                     std::string("device allocation failed: ") + hipGetErrorString(e));
+    }
+    if (d3 && s->ngrid[2] >= 128 && s->ngrid[2] <= 256) {
+        // optional: without them the per-pair z passes are used
+        for (int q = 0; q < 3; q++)
+            if (alloc(&s->B3x[q], (size_t)2 * N) != hipSuccess) {
+                for (int r = 0; r < 3; r++)
+                    if (s->B3x[r]) { (void)hipFree(s->B3x[r]); s->B3x[r] = nullptr; }
+                (void)hipGetLastError();
+                break;
+            }
     }
     if (d->vext) {
         s->has_vext = true;

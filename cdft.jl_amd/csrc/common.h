@@ -66,6 +66,8 @@ struct cdft_sys {
     // 3D grids: tabulated spectra G [(c*8 + f-1)*N + k] and three complex work arrays
     cdft::Grid3 g3;
     double *G3 = nullptr, *A3 = nullptr, *B3 = nullptr, *S3 = nullptr;
+    // three more complex work arrays when z lines fit the all-pairs z pass (128, 256 points)
+    double *B3x[3] = {nullptr, nullptr, nullptr};
     // 3D mixed precision: FP32 spectra, weighted densities and gradients
     float *G3f = nullptr, *n32 = nullptr, *dphi32 = nullptr;
     double *red = nullptr;     // block partials

@@ -113,12 +113,111 @@ __device__ inline cx<T> omega3(const Grid3 &G, int c, int f, int i0, int i1, int
     return {T(0), T(W)};
 }
 
+// Lines per LDS tile of the line-FFT passes.
+constexpr int TB3 = 16;
+
+// Radix-4 Stockham passes (+ a final radix-2 when log2 L is odd) along j over a tile
+// of TB lines in LDS laid out as in k_fft3 (axis AX), in place.  All 256 threads;
+// the tile must be complete before the call (barrier) and is complete after it.
+// EPT = radix-4 butterflies per thread and pass (L TB / 1024, at least 1).
+template <typename T, int AX, int EPT, int TB = TB3>
+__device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const cx<double> *tw, int L, int lg,
+                                                               bool inverse)
+{
+    constexpr int EPR = EPT > 0 ? EPT : 1;
+    const int tid = threadIdx.x;
+    auto lds = [&](int j, int b) -> int { return (AX == 0) ? b * L + j : j * TB + b; };
+    auto split = [&](int e, int len, int &j, int &b) {
+        if (AX == 0) { j = e % len; b = e / len; } else { b = e % TB; j = e / TB; }
+    };
+    const T sg = inverse ? T(1) : T(-1);
+    int Ns = 1;
+    const int nr4 = lg >> 1;
+    for (int pass = 0; pass < nr4; pass++) {
+        const int nb = L >> 2;
+        const int step = L / (Ns * 4);
+        cx<T> v[EPR][4];
+#pragma unroll
+        for (int q = 0; q < EPR; q++) {
+            const int e = tid + q * 256;
+            if (e < nb * TB) {
+                int jj, b;
+                split(e, nb, jj, b);
+                const int k = jj & (Ns - 1);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    cx<T> x = a[lds(jj + r * nb, b)];
+                    if (r) {
+                        const cx<double> wd = tw[(r * k * step) & (L - 1)];
+                        x = cxmul(x, cx<T>{T(wd.x), inverse ? T(-wd.y) : T(wd.y)});
+                    }
+                    v[q][r] = x;
+                }
+                const cx<T> s02 = {v[q][0].x + v[q][2].x, v[q][0].y + v[q][2].y};
+                const cx<T> d02 = {v[q][0].x - v[q][2].x, v[q][0].y - v[q][2].y};
+                const cx<T> s13 = {v[q][1].x + v[q][3].x, v[q][1].y + v[q][3].y};
+                const cx<T> d13 = {v[q][1].x - v[q][3].x, v[q][1].y - v[q][3].y};
+                const cx<T> id13 = {-sg * d13.y, sg * d13.x};
+                v[q][0] = {s02.x + s13.x, s02.y + s13.y};
+                v[q][1] = {d02.x + id13.x, d02.y + id13.y};
+                v[q][2] = {s02.x - s13.x, s02.y - s13.y};
+                v[q][3] = {d02.x - id13.x, d02.y - id13.y};
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < EPR; q++) {
+            const int e = tid + q * 256;
+            if (e < nb * TB) {
+                int jj, b;
+                split(e, nb, jj, b);
+                const int k = jj & (Ns - 1);
+                const int d = (jj / Ns) * Ns * 4 + k;
+#pragma unroll
+                for (int r = 0; r < 4; r++) a[lds(d + r * Ns, b)] = v[q][r];
+            }
+        }
+        __syncthreads();
+        Ns <<= 2;
+    }
+    if (lg & 1) {
+        const int nb = L >> 1;
+        cx<T> v[2 * EPR][2];
+#pragma unroll
+        for (int q = 0; q < 2 * EPR; q++) {
+            const int e = tid + q * 256;
+            if (e < nb * TB) {
+                int jj, b;
+                split(e, nb, jj, b);
+                const int k = jj & (Ns - 1);
+                const cx<double> wd = tw[(k * (L / (Ns * 2))) & (L - 1)];
+                const cx<T> w = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
+                const cx<T> xa = a[lds(jj, b)], xb = cxmul(a[lds(jj + nb, b)], w);
+                v[q][0] = {xa.x + xb.x, xa.y + xb.y};
+                v[q][1] = {xa.x - xb.x, xa.y - xb.y};
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 2 * EPR; q++) {
+            const int e = tid + q * 256;
+            if (e < nb * TB) {
+                int jj, b;
+                split(e, nb, jj, b);
+                const int k = jj & (Ns - 1);
+                const int d = (jj / Ns) * Ns * 2 + k;
+                a[lds(d, b)] = v[q][0];
+                a[lds(d + Ns, b)] = v[q][1];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // One FFT pass along axis AX over a tile of TB lines held in LDS.  Lines along x
 // are contiguous (LDS [b][j], j fastest); along y and z a tile is TB consecutive x
 // values (LDS [j][b], b fastest), so global accesses are coalesced and LDS accesses
 // of consecutive lanes are consecutive.  EPT = radix-4 butterflies per thread.
-constexpr int TB3 = 16;
-
 template <typename T, int AX, int IN, int OUT, int EPT>
 __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
 {
@@ -184,89 +283,7 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
         }
     }
     __syncthreads();
-    // ---- Stockham radix-4 (+ final radix-2) along j for the TB lines
-    const T sg = p.inverse ? T(1) : T(-1);
-    int Ns = 1;
-    const int nr4 = lg >> 1;
-    for (int pass = 0; pass < nr4; pass++) {
-        const int nb = L >> 2;
-        const int step = L / (Ns * 4);
-        cx<T> v[EPR][4];
-#pragma unroll
-        for (int q = 0; q < EPR; q++) {
-            const int e = tid + q * 256;
-            if (e < nb * TB) {
-                int jj, b;
-                split(e, nb, jj, b);
-                const int k = jj & (Ns - 1);
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    cx<T> x = a[lds(jj + r * nb, b)];
-                    if (r) {
-                        const cx<double> wd = p.tw[(r * k * step) & (L - 1)];
-                        x = cxmul(x, cx<T>{T(wd.x), p.inverse ? T(-wd.y) : T(wd.y)});
-                    }
-                    v[q][r] = x;
-                }
-                const cx<T> s02 = {v[q][0].x + v[q][2].x, v[q][0].y + v[q][2].y};
-                const cx<T> d02 = {v[q][0].x - v[q][2].x, v[q][0].y - v[q][2].y};
-                const cx<T> s13 = {v[q][1].x + v[q][3].x, v[q][1].y + v[q][3].y};
-                const cx<T> d13 = {v[q][1].x - v[q][3].x, v[q][1].y - v[q][3].y};
-                const cx<T> id13 = {-sg * d13.y, sg * d13.x};
-                v[q][0] = {s02.x + s13.x, s02.y + s13.y};
-                v[q][1] = {d02.x + id13.x, d02.y + id13.y};
-                v[q][2] = {s02.x - s13.x, s02.y - s13.y};
-                v[q][3] = {d02.x - id13.x, d02.y - id13.y};
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < EPR; q++) {
-            const int e = tid + q * 256;
-            if (e < nb * TB) {
-                int jj, b;
-                split(e, nb, jj, b);
-                const int k = jj & (Ns - 1);
-                const int d = (jj / Ns) * Ns * 4 + k;
-#pragma unroll
-                for (int r = 0; r < 4; r++) a[lds(d + r * Ns, b)] = v[q][r];
-            }
-        }
-        __syncthreads();
-        Ns <<= 2;
-    }
-    if (lg & 1) {
-        const int nb = L >> 1;
-        cx<T> v[2 * EPR][2];
-#pragma unroll
-        for (int q = 0; q < 2 * EPR; q++) {
-            const int e = tid + q * 256;
-            if (e < nb * TB) {
-                int jj, b;
-                split(e, nb, jj, b);
-                const int k = jj & (Ns - 1);
-                const cx<double> wd = p.tw[(k * (L / (Ns * 2))) & (L - 1)];
-                const cx<T> w = {T(wd.x), p.inverse ? T(-wd.y) : T(wd.y)};
-                const cx<T> xa = a[lds(jj, b)], xb = cxmul(a[lds(jj + nb, b)], w);
-                v[q][0] = {xa.x + xb.x, xa.y + xb.y};
-                v[q][1] = {xa.x - xb.x, xa.y - xb.y};
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 2 * EPR; q++) {
-            const int e = tid + q * 256;
-            if (e < nb * TB) {
-                int jj, b;
-                split(e, nb, jj, b);
-                const int k = jj & (Ns - 1);
-                const int d = (jj / Ns) * Ns * 2 + k;
-                a[lds(d, b)] = v[q][0];
-                a[lds(d + Ns, b)] = v[q][1];
-            }
-        }
-        __syncthreads();
-    }
+    tile_fft<T, AX, EPT>(a, p.tw, L, lg, p.inverse != 0);
     // ---- store
 #pragma unroll
     for (int q = 0; q < EL; q++) {
@@ -295,6 +312,90 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
                 S[g] = {sv.x + t.x, sv.y + t.y};
             }
         }
+    }
+}
+
+// The transposed side's z passes of one component in one launch (z lines of 128 or
+// 256 points, two LDS tiles of TBZ lines: 64 KB in FP64 at 256 points, so two
+// workgroups fit a CU): for each field pair q the tile of B_q (gradient pair
+// transformed along x and y) is transformed along z and S += conj(Z_q) FFT_z(B_q)
+// accumulates in LDS; S is then transformed back along z and written out.  This
+// replaces four accumulating k_fft3 passes, each re-reading and re-writing S, and
+// the first inverse pass; arithmetic and order are the same.  (The same fusion on
+// the weighted side, P transformed once and four B_q = IFFT_z(P Z_q) written,
+// measured slower than the four k_fft3 passes and is not used.)
+struct ZPairs3 {
+    void *out;                // S (complex T)
+    const void *b[4];         // B_q
+    const void *g1[4], *g2[4];  // spectra G (type T) of each pair's fields
+    int v1[4], v2[4];         // 1 where the field is a vector weight (H = i G)
+    int np;
+    const cx<double> *tw;     // W_L^t along z
+};
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline cx<T> zpair_z(const ZPairs3 &p, int q, long g)
+{
+    const T a = reinterpret_cast<const T *>(p.g1[q])[g], b = reinterpret_cast<const T *>(p.g2[q])[g];
+    const cx<T> h1 = p.v1[q] ? cx<T>{T(0), a} : cx<T>{a, T(0)};
+    const cx<T> h2 = p.v2[q] ? cx<T>{T(0), b} : cx<T>{b, T(0)};
+    return {h1.x - h2.y, h1.y + h2.x};
+}
+
+constexpr int TBZ = 8;
+
+template <typename T, int EPT>
+__global__ __launch_bounds__(256) void k_fft3z_accum(Grid3 G, ZPairs3 p)
+{
+    static_assert(EPT > 0, "full tiles only");
+    constexpr int TB = TBZ, EL = 4 * EPT;             // L = 128 EPT
+    extern __shared__ __align__(16) unsigned char smem3[];
+    const int tid = threadIdx.x;
+    const int L = G.n[2], lg = G.lg[2];
+    cx<T> *a0 = reinterpret_cast<cx<T> *>(smem3);     // S
+    cx<T> *a1 = a0 + (size_t)L * TB;                    // one pair's line tile
+    const int n0 = G.n[0], tiles_x = n0 / TB;
+    const int x0 = (blockIdx.x % tiles_x) * TB, y0 = blockIdx.x / tiles_x;
+    const long base = x0 + (long)n0 * y0, lstride = (long)n0 * G.n[1];
+    auto at = [&](int e, int &l, long &g) __attribute__((always_inline)) {   // LDS index and global index
+        const int b = e % TB, j = e / TB;
+        l = j * TB + b;
+        g = base + (long)j * lstride + b;
+    };
+    for (int k = 0; k < p.np; k++) {
+#pragma unroll
+        for (int q = 0; q < EL; q++) {
+            int l;
+            long g;
+            at(tid + q * 256, l, g);
+            a1[l] = reinterpret_cast<const cx<T> *>(p.b[k])[g];
+        }
+        __syncthreads();
+        tile_fft<T, 2, EPT, TBZ>(a1, p.tw, L, lg, false);
+#pragma unroll
+        for (int q = 0; q < EL; q++) {
+            int l;
+            long g;
+            at(tid + q * 256, l, g);
+            const cx<T> v = a1[l];
+            const cx<T> z = zpair_z<T>(p, k, g);
+            const cx<T> t = {v.x * z.x + v.y * z.y, v.y * z.x - v.x * z.y};
+            if (k == 0) {
+                a0[l] = t;
+            } else {
+                const cx<T> sv = a0[l];
+                a0[l] = {sv.x + t.x, sv.y + t.y};
+            }
+        }
+        __syncthreads();                                 // a1 is rewritten by the next pair; a0 complete
+    }
+    tile_fft<T, 2, EPT, TBZ>(a0, p.tw, L, lg, true);
+#pragma unroll
+    for (int q = 0; q < EL; q++) {
+        int l;
+        long g;
+        at(tid + q * 256, l, g);
+        reinterpret_cast<cx<T> *>(p.out)[g] = a0[l];
     }
 }
 

@@ -72,6 +72,50 @@ def test_functional_eval_3d_anisotropic():
     assert e_ < 1e-11, e_
 
 
+@pytest.mark.parametrize("ngrid", [(16, 32, 128), (16, 16, 256)])
+def test_functional_eval_3d_all_pairs_z_pass(ngrid):
+    """z lines of 128 or 256 points take the all-pairs z pass on the transposed side
+    (one launch accumulates the four field pairs of a component); checked against the
+    oracle."""
+    say(f"3D functional eval, ternary, {ngrid} grid (all-pairs z pass)")
+    m = PCSAFT(["methane", "ethane", "propane"])
+    T, p = 200.0, 1e7
+    x = np.array([0.6, 0.3, 0.1])
+    v = m.volume(p, T, x, "liquid")
+    L = m.length_scale()
+    s = DFTSystem(m, Uniform3DCart((p, T), x / v, [[0, 4 * L], [0, 5 * L], [0, 9 * L]], ngrid))
+    rho = smooth_profile(s, seed=11).reshape(-1)
+    got, fres, _ = s.device_system.functional_eval(rho)
+    ref, _ = oracle_of(s).dFdrho_res(rho)
+    e_ = rel(got, ref)
+    say(f"  dF/drho rel err {e_:.2e}")
+    assert e_ < 1e-11, e_
+    hv = s.device_system.hvp(rho, smooth_profile(s, seed=12).reshape(-1))
+    assert np.all(np.isfinite(hv))
+
+
+def test_converge_3d_mixed_all_pairs_z_pass():
+    say("3D converge, mixed vs FP64 on a 16 x 16 x 128 grid (all-pairs z pass in both maps)")
+    from cdft_amd import DFTOptions
+    m = PCSAFT(["methane"])
+    T, p = 150.0, 1e7
+    vb = m.volume(p, T, [1.0], "liquid")
+    L = m.length_scale()
+    box = [[0, 4 * L], [0, 4 * L], [0, 12 * L]]
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = DFTSystem(m, Uniform3DCart((p, T), [1 / vb], box, (16, 16, 128)), options=DFTOptions(precision=prec))
+        z = (np.arange(s.N) // 256) / 128.0
+        rho = (1 / vb) * (1.0 + 0.3 * np.cos(2 * np.pi * z))
+        converge_(s, rho, tol=1e-9)
+        out[prec] = (rho.reshape(-1), s.last_result)
+        say(f"  {prec}: errcode {s.last_result['errcode']} evals {s.last_result['nevals']}")
+    assert out["fp64"][1]["errcode"] == 0 and out["mixed"][1]["errcode"] == 0
+    e_ = rel(out["mixed"][0], out["fp64"][0])
+    say(f"  profile rel diff {e_:.2e}")
+    assert e_ < 1e-7, e_
+
+
 def test_bulk_consistency_3d():
     """Uniform profile: delta F_res/delta rho equals the EOS mu_res for every component."""
     say("3D bulk consistency, ternary, 16^3")
