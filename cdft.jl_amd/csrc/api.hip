@@ -156,12 +156,12 @@ void fft3_pass(const cdft_sys *s, Pass3 p, bool inverse)
 {
     p.inverse = inverse ? 1 : 0;
     p.tw = reinterpret_cast<const cx<double> *>(s->ctx->twiddles.at(s->g3.n[AX]));
+    static_assert(TB3 == 8, "EPT = L TB3 / 1024 below");
     switch (s->g3.n[AX]) {
-    case 512: fft3_launch<T, AX, IN, OUT, 8>(s, p); break;
-    case 256: fft3_launch<T, AX, IN, OUT, 4>(s, p); break;
-    case 128: fft3_launch<T, AX, IN, OUT, 2>(s, p); break;
-    case 64: fft3_launch<T, AX, IN, OUT, 1>(s, p); break;
-    default: fft3_launch<T, AX, IN, OUT, 0>(s, p); break;        // 16, 32
+    case 512: fft3_launch<T, AX, IN, OUT, 4>(s, p); break;
+    case 256: fft3_launch<T, AX, IN, OUT, 2>(s, p); break;
+    case 128: fft3_launch<T, AX, IN, OUT, 1>(s, p); break;
+    default: fft3_launch<T, AX, IN, OUT, 0>(s, p); break;        // 16, 32, 64
     }
 }
 

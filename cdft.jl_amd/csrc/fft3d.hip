@@ -113,8 +113,9 @@ __device__ inline cx<T> omega3(const Grid3 &G, int c, int f, int i0, int i1, int
     return {T(0), T(W)};
 }
 
-// Lines per LDS tile of the line-FFT passes.
-constexpr int TB3 = 16;
+// Lines per LDS tile of the line-FFT passes: 32 KB per workgroup in FP64 at 256
+// points, so several workgroups per CU keep loads in flight.
+constexpr int TB3 = 8;
 
 // Radix-4 Stockham passes (+ a final radix-2 when log2 L is odd) along j over a tile
 // of TB lines in LDS laid out as in k_fft3 (axis AX), in place.  All 256 threads;
@@ -257,8 +258,8 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
     };
     const int total = L * TB;
     // ---- load (unrolled: the tile's global reads are all in flight together).
-    // EPT > 0: L = 64 EPT, EL = L TB / 256 = 4 EPT elements per thread, no guard;
-    // EPT = 0: short lines (L = 16, 32) with a guard.
+    // EPT > 0: L TB = 1024 EPT, EL = L TB / 256 = 4 EPT elements per thread, no guard;
+    // EPT = 0: short lines (L TB <= 512) with a guard.
     constexpr bool FULL = EPT > 0;
     constexpr int EL = FULL ? 4 * EPT : 2;
 #pragma unroll
