@@ -16,8 +16,8 @@
 // Either way the kernel spectrum G_f on the ring is real (even kernels) or purely
 // imaginary (odd), so it is stored as one real half-spectrum per field, k <= M/2,
 // and kept in LDS.  Two real fields travel in one complex FFT (u1 + i u2), and the
-// three field pairs of a component share one set of barrier-separated FFT passes:
-// four FFT rounds per component and evaluation.
+// field pairs of a component (three; two for a chain-free system) share one set of
+// barrier-separated FFT passes: four FFT rounds per component and evaluation.
 //
 // Precision: CDFT_FP64 runs the whole map in FP64.  CDFT_MIXED runs the FFTs, the
 // weighted densities and Phi in FP32 (Phi's n3-singular factors in FP64) until the
