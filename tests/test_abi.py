@@ -54,3 +54,13 @@ def test_no_silent_fallback_without_gpu():
         pytest.skip("GPU present")
     with pytest.raises(_lib.CDFTError):
         _lib.Context(0)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU; asking for more exits with a message, before any GPU use."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
