@@ -128,11 +128,13 @@ __device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const c
     constexpr int EPR = EPT > 0 ? EPT : 1;
     const int tid = threadIdx.x;
     auto lds = [&](int j, int b) -> int { return (AX == 0) ? b * L + j : j * TB + b; };
-    auto split = [&](int e, int len, int &j, int &b) {
-        if (AX == 0) { j = e % len; b = e / len; } else { b = e % TB; j = e / TB; }
+    // element e of a (2^llen) x TB tile; lengths are powers of two, so shifts, not
+    // integer division
+    auto split = [&](int e, int llen, int &j, int &b) {
+        if (AX == 0) { j = e & ((1 << llen) - 1); b = e >> llen; } else { b = e % TB; j = e / TB; }
     };
     const T sg = inverse ? T(1) : T(-1);
-    int Ns = 1;
+    int Ns = 1, lNs = 0;
     const int nr4 = lg >> 1;
     for (int pass = 0; pass < nr4; pass++) {
         const int nb = L >> 2;
@@ -143,7 +145,7 @@ __device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const c
             const int e = tid + q * 256;
             if (e < nb * TB) {
                 int jj, b;
-                split(e, nb, jj, b);
+                split(e, lg - 2, jj, b);
                 const int k = jj & (Ns - 1);
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
@@ -171,15 +173,16 @@ __device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const c
             const int e = tid + q * 256;
             if (e < nb * TB) {
                 int jj, b;
-                split(e, nb, jj, b);
+                split(e, lg - 2, jj, b);
                 const int k = jj & (Ns - 1);
-                const int d = (jj / Ns) * Ns * 4 + k;
+                const int d = ((jj >> lNs) << (lNs + 2)) + k;
 #pragma unroll
                 for (int r = 0; r < 4; r++) a[lds(d + r * Ns, b)] = v[q][r];
             }
         }
         __syncthreads();
         Ns <<= 2;
+        lNs += 2;
     }
     if (lg & 1) {
         const int nb = L >> 1;
@@ -189,7 +192,7 @@ __device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const c
             const int e = tid + q * 256;
             if (e < nb * TB) {
                 int jj, b;
-                split(e, nb, jj, b);
+                split(e, lg - 1, jj, b);
                 const int k = jj & (Ns - 1);
                 const cx<double> wd = tw[(k * (L / (Ns * 2))) & (L - 1)];
                 const cx<T> w = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
@@ -204,9 +207,9 @@ __device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const c
             const int e = tid + q * 256;
             if (e < nb * TB) {
                 int jj, b;
-                split(e, nb, jj, b);
+                split(e, lg - 1, jj, b);
                 const int k = jj & (Ns - 1);
-                const int d = (jj / Ns) * Ns * 2 + k;
+                const int d = ((jj >> lNs) << (lNs + 1)) + k;
                 a[lds(d, b)] = v[q][0];
                 a[lds(d + Ns, b)] = v[q][1];
             }
@@ -253,8 +256,8 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
         bstride = 1;
     }
     auto lds = [&](int j, int b) -> int { return (AX == 0) ? b * L + j : j * TB + b; };
-    auto split = [&](int e, int len, int &j, int &b) {     // element e of a len x TB tile
-        if (AX == 0) { j = e % len; b = e / len; } else { b = e % TB; j = e / TB; }
+    auto split = [&](int e, int llen, int &j, int &b) {    // element e of a (2^llen) x TB tile
+        if (AX == 0) { j = e & ((1 << llen) - 1); b = e >> llen; } else { b = e % TB; j = e / TB; }
     };
     const int total = L * TB;
     // ---- load (unrolled: the tile's global reads are all in flight together).
@@ -267,7 +270,7 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
         const int e = tid + q * 256;
         if (FULL || e < total) {
             int j, b;
-            split(e, L, j, b);
+            split(e, lg, j, b);
             const long g = base + (long)j * lstride + (long)b * bstride;
             cx<T> v;
             if (IN == IN_CPLX) {
@@ -291,7 +294,7 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
         const int e = tid + q * 256;
         if (!FULL && e >= total) continue;
         int j, b;
-        split(e, L, j, b);
+        split(e, lg, j, b);
         const long g = base + (long)j * lstride + (long)b * bstride;
         const cx<T> v = a[lds(j, b)];
         if (OUT == OUT_CPLX) {
