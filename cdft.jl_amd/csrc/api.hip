@@ -372,7 +372,7 @@ void launch_transposed(cdft_sys *s)
         return;
     }
     int N = s->N, F = s->F;
-    hipLaunchKernelGGL(k_circ1d<double>, dim3(nblocks(N), s->nc * (F - 1)), dim3(256), 0, s->ctx->stream, s->dphi,
+    hipLaunchKernelGGL(k_circ1d<double>, dim3((N + CIRC_TW - 1) / CIRC_TW, s->nc * (F - 1)), dim3(256), 0, s->ctx->stream, s->dphi,
                        (long)F * N, (long)N, s->h, s->part, (long)(F - 1) * N, (long)N, 1, N, F, 1.0, 1);
 }
 
@@ -384,7 +384,7 @@ void launch_weighted(cdft_sys *s)
         return;
     }
     int N = s->N, F = s->F;
-    hipLaunchKernelGGL(k_circ1d<double>, dim3(nblocks(N), s->nc * (F - 1)), dim3(256), 0, s->ctx->stream, s->rho,
+    hipLaunchKernelGGL(k_circ1d<double>, dim3((N + CIRC_TW - 1) / CIRC_TW, s->nc * (F - 1)), dim3(256), 0, s->ctx->stream, s->rho,
                        (long)N, 0L, s->h, s->n, (long)F * N, (long)N, 0, N, F, s->NA, 0);
 }
 
@@ -416,7 +416,7 @@ void launch_hvp(cdft_sys *s, const double *v)
     if (s->nd == 3) {
         launch_weighted3d<double>(s, v, s->dphi);
     } else {
-        dim3 g(nblocks(N), nc * (F - 1));
+        dim3 g((N + CIRC_TW - 1) / CIRC_TW, nc * (F - 1));
         hipLaunchKernelGGL(k_circ1d<double>, g, dim3(256), 0, st, v, (long)N, 0L, s->h, s->dphi, (long)F * N,
                            (long)N, 0, N, F, s->NA, 0);
     }

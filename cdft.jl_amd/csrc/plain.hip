@@ -68,53 +68,65 @@ __global__ void k_setup_h1d(double *h, int N, int nc, const double *dd, double p
 }
 
 // Circular convolution dst[i] = scale * sum_k h[k] src[(i - k) mod N] for a batch
-// of (src, h, dst) triples selected by blockIdx.y = c*(F-1) + (f-1).
-// The k range is processed in LDS chunks: h chunk (broadcast reads) and the
-// matching window of src (consecutive reads per lane).
+// of (src, h, dst) triples selected by blockIdx.y = c*(F-1) + (f-1).  A workgroup
+// owns CIRC_TW consecutive outputs; each of its four waves sums one quarter of the
+// k range (in LDS chunks: h chunk with broadcast reads, the matching window of src
+// with consecutive reads per lane), and the quarters are added in a fixed order,
+// so the result is reproducible and many workgroups share the N^2 work.
+constexpr int CIRC_TW = 64;
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_circ1d(const T *__restrict__ src, long s_cstride, long s_fstride,
                                                 const T *__restrict__ h, T *__restrict__ dst, long d_cstride,
                                                 long d_fstride, int d_fofs, int N, int F, T scale, int vec_sign)
 {
-    constexpr int TILE = 256, KC = 1024;
-    __shared__ T hs[KC];
-    __shared__ T ws[TILE + KC];
-    int y = blockIdx.y;
-    int c = y / (F - 1), f = 1 + y % (F - 1);
+    constexpr int TW = CIRC_TW, NG = 4, KC = 256;
+    __shared__ T hs[NG][KC];
+    __shared__ T ws[NG][TW + KC];
+    __shared__ T part[NG][TW];
+    const int y = blockIdx.y;
+    const int c = y / (F - 1), f = 1 + y % (F - 1);
     const T *s = src + c * s_cstride + f * s_fstride;
     const T *hk = h + (size_t)(c * F + f) * N;
     T *d = dst + c * d_cstride + (f - d_fofs) * d_fstride;
     T sc = scale;
     if (vec_sign && f == 3) sc = -scale;   // 1D vector slot is f = 3
-    int i0 = blockIdx.x * TILE;
-    int tid = threadIdx.x;
+    const int i0 = blockIdx.x * TW;
+    const int g = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int Kg = (N + NG - 1) / NG;                   // taps per wave
+    const int kb = g * Kg, ke = min(N, kb + Kg);
     T acc = 0;
-    for (int kc = 0; kc < N; kc += KC) {
-        int kcn = min(KC, N - kc);
+    for (int kc0 = 0; kc0 < Kg; kc0 += KC) {            // the same trip count in every wave
+        const int kc = kb + kc0;
+        const int kcn = max(0, min(KC, ke - kc));
         __syncthreads();
-        for (int t = tid; t < kcn; t += TILE) hs[t] = hk[kc + t];
-        int base = i0 - kc - kcn + 1;
-        int wn = TILE + kcn - 1;
-        for (int t = tid; t < wn; t += TILE) {
-            int g = (base + t) % N;
-            if (g < 0) g += N;
-            ws[t] = s[g];
+        for (int t = l; t < kcn; t += TW) hs[g][t] = hk[kc + t];
+        const int base = i0 - kc - kcn + 1;
+        const int wn = kcn > 0 ? TW + kcn - 1 : 0;
+        for (int t = l; t < wn; t += TW) {
+            int gi = (base + t) % N;
+            if (gi < 0) gi += N;
+            ws[g][t] = s[gi];
         }
         __syncthreads();
-        const T *wp = ws + tid + kcn - 1;
-        int kk = 0;
-        T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-        for (; kk + 3 < kcn; kk += 4) {
-            a0 += hs[kk] * wp[-kk];
-            a1 += hs[kk + 1] * wp[-kk - 1];
-            a2 += hs[kk + 2] * wp[-kk - 2];
-            a3 += hs[kk + 3] * wp[-kk - 3];
+        if (kcn > 0) {
+            const T *wp = &ws[g][l + kcn - 1];
+            int kk = 0;
+            T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            for (; kk + 3 < kcn; kk += 4) {
+                a0 += hs[g][kk] * wp[-kk];
+                a1 += hs[g][kk + 1] * wp[-kk - 1];
+                a2 += hs[g][kk + 2] * wp[-kk - 2];
+                a3 += hs[g][kk + 3] * wp[-kk - 3];
+            }
+            for (; kk < kcn; kk++) a0 += hs[g][kk] * wp[-kk];
+            acc += (a0 + a1) + (a2 + a3);
         }
-        for (; kk < kcn; kk++) a0 += hs[kk] * wp[-kk];
-        acc += (a0 + a1) + (a2 + a3);
     }
-    int i = i0 + tid;
-    if (i < N) d[i] = sc * acc;
+    part[g][l] = acc;
+    __syncthreads();
+    const int i = i0 + l;
+    if (g == 0 && i < N) d[i] = sc * ((part[0][l] + part[1][l]) + (part[2][l] + part[3][l]));
 }
 
 // Phi and dPhi/dn at every grid point.
