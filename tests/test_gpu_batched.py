@@ -159,3 +159,31 @@ def test_batched_mixed_maxit(pores):
     rm = [r.copy() for _, r in mixed]
     res = batch([s for s, _ in mixed], rm, tol=1e-8, maxit=2)
     assert all(r["errcode"] == 10 for r in res)
+
+
+# The two-component batched kernel (Anderson state in HBM) faulted the GPU with an
+# illegal memory access on its first run (mixed precision, methane/n-decane interface,
+# 256 points).  A CPU emulation of the launch under AddressSanitizer is clean so far and
+# the cause is not yet shown, so the case stays off the GPU.
+@pytest.mark.skip(reason="two-component batched launch faulted once (illegal memory access); cause not yet shown")
+def test_batched_mixed_binary_interface():
+    """Two components (Anderson state in HBM, k_batched<2, false>), mixed precision,
+    against the plain FP64 path: same termination, profile and surface tension."""
+    say("batched mixed binary interface N=256 vs plain FP64, tol=1e-8")
+    s64, r0 = configs.c1c10_interface(ngrid=256, box=10.0)
+    sm, _ = configs.c1c10_interface(ngrid=256, box=10.0, options=DFTOptions(precision="mixed"))
+    rb = r0.copy()
+    rr = batch([sm], [rb], tol=1e-8)[0]
+    rp = r0.copy()
+    converge_(s64, rp, tol=1e-8)
+    ra = s64.last_result
+    say(f"  mixed errcode {rr['errcode']} evals {rr['nevals']} (fp32 {rr['nevals_fp32']}) | plain errcode "
+        f"{ra['errcode']} evals {ra['nevals']} | rel {rel(rb, rp):.2e}")
+    assert rr["errcode"] == ra["errcode"] == 0
+    assert rr["nevals_fp32"] > 0
+    e_ = rel(rb, rp)
+    assert e_ < 1e-5, e_
+    g_a = surface_tension_system(s64, rp)
+    g_b = surface_tension_system(sm, rb)
+    say(f"  tension rel {abs(g_b - g_a) / abs(g_a):.1e}")
+    assert abs(g_b - g_a) <= 1e-6 * abs(g_a)
