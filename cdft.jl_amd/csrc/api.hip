@@ -1113,6 +1113,11 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         if (!sys[i] || !rho[i]) return fail(ctx, CDFT_ERR_ARG, "cdft_converge_batch: NULL system or profile");
         if (sys[i]->ctx != ctx) return fail(ctx, CDFT_ERR_ARG, "all systems must belong to the context");
         if (sys[i]->nc != nc) return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched systems must share nc");
+        // the multi-component batched kernel faulted on the GPU and is switched off
+        // until that is understood
+        if (sys[i]->nc != 1)
+            return fail(ctx, CDFT_ERR_UNSUPPORTED,
+                        "batched path: one-component systems only for now; use cdft_converge per system");
         if (sys[i]->precision != prec) return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched systems must share precision");
         if (sys[i]->nd != 1) return fail(ctx, CDFT_ERR_UNSUPPORTED, "the batched path takes 1D systems");
         for (int j = 0; j < i; j++)
