@@ -153,7 +153,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=0,
                     help="plain FP64 comparator on this many pores spread over the batch (0: all, one after another)")
     ap.add_argument("--cpu-sample", type=int, default=8)
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed"],
+                    help="the batched path runs mixed precision (FP64 batches are switched off)")
     ap.add_argument("--no-extras", action="store_true", help="skip the 3D and adjoint measurements")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a device copy of this many MiB first (profiler counter scale)")

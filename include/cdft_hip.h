@@ -128,9 +128,9 @@ int cdft_converge(cdft_sys *sys, double *rho_inout, const cdft_solver_opts *opts
  * on the device; systems retire independently.  Each system follows exactly the
  * same algorithm as cdft_converge.  (No batched counterpart in the reference: it
  * replaces a loop of converge! calls, e.g. over adsorption(...) state points,
- * src/methods/adsorption.jl:48-66.)  hist in opts is ignored.  One-component systems
- * only for now: a batch of multi-component systems returns CDFT_ERR_UNSUPPORTED (use
- * cdft_converge per system). */
+ * src/methods/adsorption.jl:48-66.)  hist in opts is ignored.  One-component
+ * CDFT_MIXED systems only for now: a batch of multi-component or CDFT_FP64 systems
+ * returns CDFT_ERR_UNSUPPORTED (use cdft_converge per system). */
 int cdft_converge_batch(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_inout,
                         const cdft_solver_opts *opts, cdft_result *results);
 /* Same with rho_inout as device pointers (inputs resident in HBM). */

@@ -187,3 +187,14 @@ def test_batched_mixed_binary_interface():
     g_b = surface_tension_system(sm, rb)
     say(f"  tension rel {abs(g_b - g_a) / abs(g_a):.1e}")
     assert abs(g_b - g_a) <= 1e-6 * abs(g_a)
+
+
+def test_batched_refuses_fp64_and_multicomponent(pores):
+    """FP64 and multi-component batches are switched off (each faulted the GPU once,
+    cause not shown): the call raises before any launch."""
+    s, r0 = pores[0]                                      # FP64 pore
+    with pytest.raises(_lib.CDFTError):
+        batch([s], [r0.copy()], tol=1e-8)
+    sm, rm = configs.c1c10_interface(ngrid=64, box=6.0, options=DFTOptions(precision="mixed"))
+    with pytest.raises(_lib.CDFTError):
+        batch([sm], [rm.copy()], tol=1e-8)
