@@ -238,8 +238,9 @@ class DeviceSystem:
 
 def converge_batch(ctx, systems, rhos, *, maxit=10000, beta=1e-2, tol=1e-4, anderson_start=1e-1,
                    m=5, device_ptrs=False):
-    """Batched persistent solve of many one-component mixed-precision systems
-    (cdft_converge_batch[_device]); multi-component or FP64 batches raise CDFTError.
+    """Batched persistent solve of many one-component mixed-precision systems of at
+    most 512 points, m <= 5 (cdft_converge_batch[_device]); other batches raise
+    CDFTError.
 
     rhos: list of float64 numpy arrays (in/out), or device pointers (ints) when
     device_ptrs is True."""

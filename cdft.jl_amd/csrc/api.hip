@@ -1125,6 +1125,12 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         if (sys[i]->precision != CDFT_MIXED)
             return fail(ctx, CDFT_ERR_UNSUPPORTED,
                         "batched path: mixed-precision systems only for now; use cdft_converge for FP64");
+        // the kernels with the Anderson state in HBM share the code of the faulting
+        // multi-component kernel and have not run on a GPU: only the register-state
+        // kernel (at most 512 points, m <= 5) is enabled
+        if (sys[i]->N > kRegNPT * BT || o->m > kRegAM)
+            return fail(ctx, CDFT_ERR_UNSUPPORTED,
+                        "batched path: at most 512 grid points and anderson m <= 5 for now; use cdft_converge");
         if (sys[i]->nd != 1) return fail(ctx, CDFT_ERR_UNSUPPORTED, "the batched path takes 1D systems");
         for (int j = 0; j < i; j++)
             if (sys[j] == sys[i]) return fail(ctx, CDFT_ERR_ARG, "a system appears twice in the batch");

@@ -114,7 +114,8 @@ function cDFT.converge!(prob::cDFT.DFTProblem{<:HIPSystem}, method::cDFT.AASol, 
 end
 
 """Batched solve of many HIPNative systems in one persistent launch (one-component,
-mixed-precision systems; others raise, solve them with converge!)."""
+mixed-precision systems of at most 512 points, anderson_m <= 5; others raise, solve them
+with converge!)."""
 function converge_batch!(systems::AbstractVector, rhos::AbstractVector; maxit = 10_000, beta = 1e-2, tol = 1e-4,
                          anderson_start = 1e-1, anderson_m = 5)
     ctx = _ctx(first(systems).options.device.device)

@@ -129,8 +129,8 @@ int cdft_converge(cdft_sys *sys, double *rho_inout, const cdft_solver_opts *opts
  * same algorithm as cdft_converge.  (No batched counterpart in the reference: it
  * replaces a loop of converge! calls, e.g. over adsorption(...) state points,
  * src/methods/adsorption.jl:48-66.)  hist in opts is ignored.  One-component
- * CDFT_MIXED systems only for now: a batch of multi-component or CDFT_FP64 systems
- * returns CDFT_ERR_UNSUPPORTED (use cdft_converge per system). */
+ * CDFT_MIXED systems of at most 512 points with anderson m <= 5 only for now: other
+ * batches return CDFT_ERR_UNSUPPORTED (use cdft_converge per system). */
 int cdft_converge_batch(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_inout,
                         const cdft_solver_opts *opts, cdft_result *results);
 /* Same with rho_inout as device pointers (inputs resident in HBM). */

@@ -191,10 +191,14 @@ def test_batched_mixed_binary_interface():
 
 def test_batched_refuses_fp64_and_multicomponent(pores):
     """FP64 and multi-component batches are switched off (each faulted the GPU once,
-    cause not shown): the call raises before any launch."""
+    cause not shown), and so are the other HBM-state cases: the call raises before
+    any launch."""
     s, r0 = pores[0]                                      # FP64 pore
     with pytest.raises(_lib.CDFTError):
         batch([s], [r0.copy()], tol=1e-8)
     sm, rm = configs.c1c10_interface(ngrid=64, box=6.0, options=DFTOptions(precision="mixed"))
     with pytest.raises(_lib.CDFTError):
         batch([sm], [rm.copy()], tol=1e-8)
+    big = configs.slit_pores(n=1, seed=0, options=DFTOptions(precision="mixed"))[0]
+    with pytest.raises(_lib.CDFTError):                   # anderson m = 6: state in HBM
+        batch([big[0]], [big[1].copy()], tol=1e-8, m=6)
