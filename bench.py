@@ -1,16 +1,22 @@
 """Benchmark: converged profiles per second (residual < 1e-8) of the batched
-mixed-precision path on 512 methane slit pores, with the plain FP64 path timed on
-the same 512 pores one after another and the CPU oracle on a sample of them.
+mixed-precision path on 512 methane slit pores, with the plain FP64 GPU path on the
+same 512 pores one after another and the FP64 CPU oracle on all 512 pores with
+OpenMP beside it.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--npores 512] [--no-extras]
 
 A step is one batched solve of all pores from their seed profiles (inputs resident
-in HBM).  Prints one JSON line.  Besides the headline workload the line carries
-two further measurements of the same library (skipped with --no-extras):
-  matrix_3d   the 3D hard-sphere-matrix ternary on a 256^3 grid: one mixed-precision
-              solve, stream time per map evaluation and the algorithmic-byte rate;
-  adjoint     surface-tension sensitivities of the methane / n-decane interface
-              (2048 points): adjoint solve versus finite differences of full solves.
+in HBM).  Prints one JSON line.  Besides the headline workload the line carries:
+  termination  per-pore errcodes of the batched, plain and oracle solves, the pores
+               where they differ, and the batched and plain results' largest
+               relative differences from the oracle (profile, adsorption, tension);
+  c0, c1       the plain FP64 path on the methane wall (201 points) and on the
+               methane / n-decane interface (2048 points);
+  matrix_3d    the 3D hard-sphere-matrix ternary on a 256^3 grid: one mixed-precision
+               solve, stream time per map evaluation and the algorithmic-byte rate;
+  adjoint      surface-tension sensitivities of the methane / n-decane interface
+               (2048 points): adjoint solve versus finite differences of full solves.
+(the last four are skipped with --no-extras).
 """
 import argparse
 import json
@@ -45,8 +51,21 @@ def measured_traffic():
     return float(m.group(1)) if m else None
 
 
-def run_c3(args, np, torch, DFTOptions, _lib, configs):
-    pores = configs.slit_pores(n=args.npores, seed=0, options=DFTOptions(precision=args.precision))
+def host_threads():
+    """CPU threads this process may use (the GPU box gives one GPU 16 of them)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def relmax(a, b):
+    import numpy as np
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def run_batched(args, np, torch, _lib, pores):
     systems = [s.device_system for s, _ in pores]
     ctx = systems[0].ctx
     sizes = [s.N * s.nc for s in systems]
@@ -80,12 +99,86 @@ def run_c3(args, np, torch, DFTOptions, _lib, configs):
         kern_ms += ctx.last_kernel_ms()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return pores, systems, sizes, offs, work, results, dt, kern_ms
+    return systems, offs, work.cpu().numpy(), results, dt, kern_ms
+
+
+def run_plain(args, pores):
+    """The plain FP64 path on every pore, one after another (set-up outside the timing)."""
+    from cdft_amd import converge_
+    out, res = [], []
+    t = 0.0
+    for s, r0 in pores:
+        s.device_system
+        rho = r0.copy()
+        t1 = time.perf_counter()
+        converge_(s, rho, tol=args.tol)
+        t += time.perf_counter() - t1
+        out.append(rho.reshape(-1))
+        res.append(s.last_result)
+    return out, res, t
+
+
+def run_cpu_baseline(args, np, pores, mus):
+    """cpu_baseline leg: the FP64 oracle (planned 1D evaluation) on all pores with
+    OpenMP over pores, and on one thread over a sample; plus the oracle's adsorption
+    and tension of each converged profile for the accuracy report."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    ors = [po.Oracle(**s.descriptor()) for s, _ in pores]
+    seeds = [r for _, r in pores]
+    nthr = host_threads()
+    outs, res, secs, wall = po.converge_batch(ors, seeds, tol=args.tol, nthreads=nthr)
+    sidx = np.linspace(0, len(pores) - 1, min(args.cpu_1t_sample, len(pores))).round().astype(int)
+    _, res1, _, wall1 = po.converge_batch([ors[i] for i in sidx], [seeds[i] for i in sidx], tol=args.tol,
+                                          nthreads=1)
+    ads, ten = [], []
+    for (s, _), o, x, r, mu in zip(pores, ors, outs, res, mus):
+        if r.errcode == 0:
+            ads.append(o.adsorption(x)[0])
+            ten.append(o.surface_tension_planned(x, mu, s.structure.conditions[0]))
+        else:
+            ads.append(None)
+            ten.append(None)
+    return dict(profiles=outs, errcodes=[r.errcode for r in res], nevals=[r.nevals for r in res], wall=wall,
+                threads=nthr, secs=secs, wall1=wall1, n1=len(sidx), conv1=sum(r.errcode == 0 for r in res1),
+                adsorption=ads, tension=ten)
+
+
+def run_c0(np, configs, converge_, nsolve=20):
+    s, rho0 = configs.methane_wall()
+    s.device_system
+    rho = rho0.copy()
+    converge_(s, rho, tol=1e-8)
+    t = time.perf_counter()
+    for _ in range(nsolve):
+        rho = rho0.copy()
+        converge_(s, rho, tol=1e-8)
+    dt = (time.perf_counter() - t) / nsolve
+    r = s.last_result
+    return {"workload": "methane at 150 K, 1e7 Pa, graphite Steele walls 5 sigma apart, 201 points, tol 1e-8; FP64",
+            "errcode": r["errcode"], "nevals": r["nevals"], "solve_ms": dt * 1e3, "profiles_per_s": 1.0 / dt,
+            "stream_ms": r["device_ms"]}
+
+
+def run_c1(np, configs, converge_):
+    from cdft_amd.methods import bulk_chemical_potential as bulk_mu
+    s, rho0 = configs.c1c10_interface()
+    s.device_system
+    rho = rho0.copy()
+    t = time.perf_counter()
+    converge_(s, rho, tol=1e-8, maxit=20000)
+    dt = time.perf_counter() - t
+    r = s.last_result
+    p = s.structure.conditions[0]
+    gamma = s.device_system.surface_tension(rho, bulk_mu(s), p) / 2
+    return {"workload": "methane/n-decane interface, x=[0.5,0.5] liquid at its bubble point, T=300 K, 2048 points "
+                        "over [-20 L, 20 L], tol 1e-8; FP64",
+            "errcode": r["errcode"], "nevals": r["nevals"], "solve_s": dt, "profiles_per_s": 1.0 / dt,
+            "stream_ms": r["device_ms"], "surface_tension_N_per_m": gamma}
 
 
 def run_matrix_3d(np, DFTOptions, configs, converge_):
     s, rho = configs.matrix_3d(n=256, options=DFTOptions(precision="mixed"))
-    dev = s.device_system
     r = rho.copy()
     t = time.perf_counter()
     converge_(s, r, tol=1e-8, **configs.MATRIX_3D_SOLVER)
@@ -143,6 +236,46 @@ def run_adjoint(np, configs, converge_):
     }
 
 
+def termination_report(np, pores, batched, plain, cpu, mus):
+    """Per-pore errcodes of the three paths, where they differ, and the largest
+    relative differences of the batched and plain results from the oracle's over the
+    pores all of them converge."""
+    bcode = [r["errcode"] for r in batched["results"]]
+    pcode = [r["errcode"] for r in plain["results"]]
+    ocode = cpu["errcodes"]
+
+    def hist(c):
+        h = {}
+        for v in c:
+            h[str(v)] = h.get(str(v), 0) + 1
+        return h
+
+    def diff(a, b):
+        return [i for i in range(len(a)) if a[i] != b[i]]
+
+    err = {k: {"profile": 0.0, "adsorption": 0.0, "surface_tension": 0.0} for k in ("batched_mixed", "plain_fp64")}
+    for i, (s, _) in enumerate(pores):
+        if not (bcode[i] == pcode[i] == ocode[i] == 0):
+            continue
+        xo = cpu["profiles"][i]
+        ao, to = cpu["adsorption"][i], cpu["tension"][i]
+        p = s.structure.conditions[0]
+        for key, x in (("batched_mixed", batched["profiles"][i]), ("plain_fp64", plain["profiles"][i])):
+            e = err[key]
+            e["profile"] = max(e["profile"], relmax(x, xo))
+            a = s.device_system.adsorption(x)[0]
+            e["adsorption"] = max(e["adsorption"], abs(a - ao) / abs(ao))
+            t = s.device_system.surface_tension(x, mus[i], p)
+            e["surface_tension"] = max(e["surface_tension"], abs(t - to) / abs(to))
+    return {
+        "errcodes": {"batched_mixed": hist(bcode), "plain_fp64": hist(pcode), "oracle_fp64": hist(ocode)},
+        "mismatch_batched_vs_oracle": [{"pore": i, "batched": bcode[i], "oracle": ocode[i]} for i in diff(bcode, ocode)],
+        "mismatch_plain_vs_oracle": [{"pore": i, "plain": pcode[i], "oracle": ocode[i]} for i in diff(pcode, ocode)],
+        "mismatch_batched_vs_plain": [{"pore": i, "batched": bcode[i], "plain": pcode[i]} for i in diff(bcode, pcode)],
+        "max_rel_err_vs_oracle": err,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,12 +283,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--npores", type=int, default=512)
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--plain-sample", type=int, default=0,
-                    help="plain FP64 comparator on this many pores spread over the batch (0: all, one after another)")
-    ap.add_argument("--cpu-sample", type=int, default=8)
+    ap.add_argument("--cpu-1t-sample", type=int, default=32,
+                    help="pores (spread over the batch) of the one-thread CPU figure")
     ap.add_argument("--precision", default="mixed", choices=["mixed"],
                     help="the batched path runs mixed precision (FP64 batches are switched off)")
-    ap.add_argument("--no-extras", action="store_true", help="skip the 3D and adjoint measurements")
+    ap.add_argument("--no-extras", action="store_true", help="skip the c0, c1, 3D and adjoint measurements")
+    ap.add_argument("--no-comparators", action="store_true",
+                    help="time the batched path only (no plain, CPU or termination legs; for profiler runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a device copy of this many MiB first (profiler counter scale)")
     args = ap.parse_args()
@@ -168,15 +302,18 @@ def main():
     import torch
 
     from cdft_amd import DFTOptions, _lib, configs, converge_
+    from cdft_amd.methods import bulk_chemical_potential as bulk_mu
 
     # ---------------- headline workload
-    pores, systems, sizes, offs, work, results, dt, kern_ms = run_c3(args, np, torch, DFTOptions, _lib, configs)
-    batched_rho = work.cpu().numpy()
+    pores = configs.slit_pores(n=args.npores, seed=0, options=DFTOptions(precision=args.precision))
+    systems, offs, brho, results, dt, kern_ms = run_batched(args, np, torch, _lib, pores)
     nconv = sum(1 for r in results if r["errcode"] == 0)
     codes = {}
     for r in results:
         codes[str(r["errcode"])] = codes.get(str(r["errcode"]), 0) + 1
     value = nconv * args.steps / dt
+    dev_ms = np.array([r["device_ms"] for r in results])
+    slow = int(np.argmax(dev_ms))
 
     # ---------------- roofline of the batched kernel (algorithmic bytes / kernel time)
     alg_bytes = 0.0
@@ -186,50 +323,6 @@ def main():
         alg_bytes += s.N * s.nc * (n32 * bytes_per_point_eval(4) + n64 * bytes_per_point_eval(8))
     kms = kern_ms / args.steps
     achieved = alg_bytes / (kms * 1e-3) / 1e9
-
-    # ---------------- plain FP64 path on the same pores, one after another (all by default)
-    if args.plain_sample > 0:
-        idx = np.linspace(0, args.npores - 1, args.plain_sample).round().astype(int)
-    else:
-        idx = np.arange(args.npores)
-    plain_pores = configs.slit_pores(n=args.npores, seed=0, options=DFTOptions(precision="fp64"))
-    t_plain = 0.0
-    nplain = 0
-    from cdft_amd.methods import adsorption_system, surface_tension_system
-    err = {"profile": 0.0, "adsorption": 0.0, "surface_tension": 0.0}
-    for i in idx:
-        s, r0 = plain_pores[i]
-        s.device_system  # set-up outside the timed region
-        rho = r0.copy()
-        t1 = time.perf_counter()
-        converge_(s, rho, tol=args.tol)
-        t_plain += time.perf_counter() - t1
-        nplain += s.last_result["errcode"] == 0
-        if s.last_result["errcode"] == 0 and results[i]["errcode"] == 0:
-            # mixed-precision batched result against this FP64 solve of the same pore
-            rb = batched_rho[offs[i]:offs[i + 1]]
-            ra = rho.reshape(-1)
-            err["profile"] = max(err["profile"], float(np.max(np.abs(rb - ra)) / np.max(np.abs(ra))))
-            a, b = adsorption_system(s, ra)[0], adsorption_system(s, rb)[0]
-            err["adsorption"] = max(err["adsorption"], abs(b - a) / abs(a))
-            g, h = surface_tension_system(s, ra), surface_tension_system(s, rb)
-            err["surface_tension"] = max(err["surface_tension"], abs(h - g) / abs(g))
-    plain_rate = nplain / t_plain
-
-    # ---------------- CPU oracle (1 thread, FP64) on a sample
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as po
-    cidx = np.linspace(0, args.npores - 1, args.cpu_sample).round().astype(int)
-    t_cpu = 0.0
-    ncpu = 0
-    for i in cidx:
-        s, r0 = plain_pores[i]
-        o = po.Oracle(**s.descriptor())
-        t1 = time.perf_counter()
-        _, ro, _ = o.converge(r0.reshape(-1), tol=args.tol)
-        t_cpu += time.perf_counter() - t1
-        ncpu += ro.errcode == 0
-    cpu_rate = ncpu / t_cpu
 
     out = {
         "metric": METRIC,
@@ -248,21 +341,48 @@ def main():
             "workload": f"{args.npores} PC-SAFT methane slit pores between Steele graphite walls, T=150 K, "
                         "W~U[2,12] sigma, p~logU[1e3,1e7] Pa (seed 0), N=round(40 W/sigma) points",
             "tol": args.tol, "anderson_m": 5, "beta": 0.01, "precision": args.precision,
-            "grid_points_total": int(sum(sizes)), "errcodes": codes,
+            "grid_points_total": int(sum(s.N * s.nc for s in systems)), "errcodes": codes,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "k_batched", "kernel_ms": kms, "algorithmic_bytes": alg_bytes},
-        "plain_fp64": {"value": plain_rate, "unit": "profiles/s",
-                       "sample": (f"all {len(idx)} pores, one after another" if len(idx) == args.npores
-                                  else f"{len(idx)} pores spread over the batch"),
-                       "seconds": t_plain,
-                       "converged": int(nplain), "speedup_batched_vs_plain": value / plain_rate,
-                       "batched_mixed_max_rel_err": err},
-        "cpu_baseline": {"value": cpu_rate, "unit": "profiles/s", "cores": 1, "kind": "port",
-                         "sample": f"{len(cidx)} pores spread over the batch, FP64 C oracle, 1 thread"},
+                     "kernel": "k_batched", "kernel_ms": kms, "algorithmic_bytes": alg_bytes,
+                     "limiter": "not HBM: per-system latency of LDS FFT passes, barriers and dependent arithmetic; "
+                                "the launch lasts as long as its slowest system"},
+        "batched_device_ms": {"p50": float(np.percentile(dev_ms, 50)), "p90": float(np.percentile(dev_ms, 90)),
+                              "p99": float(np.percentile(dev_ms, 99)), "max": float(dev_ms.max()),
+                              "slowest_pore": slow, "slowest_pore_N": systems[slow].N,
+                              "slowest_pore_nevals": results[slow]["nevals"],
+                              "slowest_pore_errcode": results[slow]["errcode"]},
     }
+    if not args.no_comparators:
+        mus = [bulk_mu(s) for s, _ in pores]
+        # ---------------- plain FP64 path on the same pores, one after another
+        plain_pores = configs.slit_pores(n=args.npores, seed=0, options=DFTOptions(precision="fp64"))
+        pprof, pres, t_plain = run_plain(args, plain_pores)
+        nplain = sum(1 for r in pres if r["errcode"] == 0)
+        plain_rate = nplain / t_plain
+        out["plain_fp64"] = {"value": plain_rate, "unit": "profiles/s",
+                             "sample": f"all {args.npores} pores, one after another", "seconds": t_plain,
+                             "converged": nplain, "speedup_batched_vs_plain": value / plain_rate}
+        # ---------------- CPU oracle, OpenMP over all pores and one thread on a sample
+        cpu = run_cpu_baseline(args, np, plain_pores, mus)
+        ncpu = sum(1 for c in cpu["errcodes"] if c == 0)
+        secs = cpu["secs"]
+        out["cpu_baseline"] = {
+            "value": ncpu / cpu["wall"], "unit": "profiles/s", "cores": cpu["threads"], "kind": "port",
+            "sample": f"all {args.npores} pores, FP64 C oracle (planned 1D evaluation), OpenMP over pores "
+                      f"(dynamic schedule), {cpu['threads']} threads, {cpu['wall']:.2f} s",
+            "one_thread": {"value": cpu["conv1"] / cpu["wall1"], "unit": "profiles/s",
+                           "sample": f"{cpu['n1']} pores spread over the batch, 1 thread, {cpu['wall1']:.2f} s",
+                           "all_pores_cpu_seconds": float(np.sum(secs))},
+            "gpu_over_cpu": value / (ncpu / cpu["wall"]),
+        }
+        batched = {"results": results, "profiles": [brho[offs[i]:offs[i + 1]] for i in range(len(systems))]}
+        plain = {"results": pres, "profiles": pprof}
+        out["termination"] = termination_report(np, plain_pores, batched, plain, cpu, mus)
     if not args.no_extras:
+        out["c0"] = run_c0(np, configs, converge_)
+        out["c1"] = run_c1(np, configs, converge_)
         out["matrix_3d"] = run_matrix_3d(np, DFTOptions, configs, converge_)
         out["adjoint"] = run_adjoint(np, configs, converge_)
     print(json.dumps(out))

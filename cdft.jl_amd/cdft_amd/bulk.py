@@ -37,6 +37,13 @@ DATABASE = {
     "argon": (0.9285, 3.4784, 122.23, 39.948),
 }
 
+# Binary interaction parameters k_ij (Gross & Sadowski 2001), used when the caller
+# gives none, like Clapeyron's unlike-parameter table.  With CO2/methane = 0.065 the
+# reference's stored adsorption answer (test/test_methods.jl:37) is reproduced.
+KIJ_DATABASE = {
+    frozenset(("carbon dioxide", "methane")): 0.065,
+}
+
 # Dispersion constants a_qk, b_qk (G&S 2001 Table 1); identical to PCSAFT.jl:86-103.
 _A = np.array([
     [0.9105631445, -0.3084016918, -0.0906148351],
@@ -82,7 +89,14 @@ class PCSAFT:
         s = rows[:, 1] * 1e-10
         e = rows[:, 2].copy()
         self.Mw = rows[:, 3].copy()
-        k = np.zeros((n, n)) if kij is None else np.asarray(kij, dtype=np.float64)
+        if kij is None:
+            k = np.zeros((n, n))
+            for i in range(n):
+                for j in range(n):
+                    if i != j:
+                        k[i, j] = KIJ_DATABASE.get(frozenset((self.components[i], self.components[j])), 0.0)
+        else:
+            k = np.asarray(kij, dtype=np.float64)
         self.sigma = 0.5 * (s[:, None] + s[None, :])
         self.epsilon = np.sqrt(e[:, None] * e[None, :]) * (1.0 - k)
         self.kij = k

@@ -54,19 +54,23 @@ def slit_pore_params(n=512, seed=0):
     return W, p
 
 
-def slit_pores(n=512, seed=0, options=None, T=150.0):
-    model = PCSAFT(["methane"])
+def _slit_pore(model, W, p, T, options):
     sigma = model.length_scale()
+    width = W * sigma
+    v = model.volume(p, T, [1.0])
+    N = int(round(40 * W))
+    structure = Uniform1DCart((p, T), [1 / v], [0.5 * sigma, width - 0.5 * sigma], N)
+    system = DFTSystem(model, structure, Steele(["graphite"], width), options)
+    return system, initialize_profiles(system)
+
+
+def slit_pores(n=512, seed=0, options=None, T=150.0, indices=None):
+    """The n-pore workload, or only the pores listed in indices (same parameters as
+    in the full set)."""
+    model = PCSAFT(["methane"])
     Ws, ps = slit_pore_params(n, seed)
-    out = []
-    for W, p in zip(Ws, ps):
-        width = W * sigma
-        v = model.volume(p, T, [1.0])
-        N = int(round(40 * W))
-        structure = Uniform1DCart((p, T), [1 / v], [0.5 * sigma, width - 0.5 * sigma], N)
-        system = DFTSystem(model, structure, Steele(["graphite"], width), options)
-        out.append((system, initialize_profiles(system)))
-    return out
+    idx = range(n) if indices is None else indices
+    return [_slit_pore(model, Ws[i], ps[i], T, options) for i in idx]
 
 
 MATRIX_3D_SOLVER = dict(anderson_start=0.5)

@@ -12,16 +12,22 @@ from .structure import TwoPhase1DCart, Uniform1DCart
 from .system import DFTSystem, converge_, free_energy, initialize_profiles, integrate, simpson
 
 
+def bulk_chemical_potential(system):
+    """mu_i (J/mol) of the structure's first bulk phase, as surface_tension(system, rho)
+    takes it (src/methods/surface_tension.jl:45)."""
+    st = system.structure
+    T = st.conditions[1]
+    rl = st.rho_bulk
+    return np.asarray(system.model.VT_chemical_potential(1.0 / rl.sum(), T, rl / rl.sum()), dtype=np.float64)
+
+
 def surface_tension_system(system, rho):
     """F k_B T - sum_i mu_i int rho_i + p int 1 (for the periodic box: two interfaces)."""
     rho = np.asarray(rho, dtype=np.float64).reshape(system.nc, system.N)
     st = system.structure
     p, T = st.conditions
-    rl = st.rho_bulk
-    x = rl / rl.sum()
-    vl = 1.0 / rl.sum()
     F = free_energy(system, rho)
-    mu = system.model.VT_chemical_potential(vl, T, x)
+    mu = bulk_chemical_potential(system)
     dz = st.dz()
     chem = sum(mu[i] * simpson(rho[i], dz) for i in range(system.nc))
     return F * KB * T - chem + p * simpson(np.ones(system.N), dz)

@@ -529,6 +529,7 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
     std::memset(&P, 0, sizeof(P));
     P.nc = nc;
     P.T = s->T;
+    P.psi = s->psi;
     for (int i = 0; i < nc; i++) { P.m[i] = s->m[i]; P.dd[i] = s->d[i] / s->L; }
     for (int i = 0; i < nc; i++)
         for (int j = 0; j < nc; j++) {
@@ -1213,16 +1214,11 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev1, st));
         return CDFT_OK;
     };
-    int rc;
-    // Anderson state in registers when every system fits the register layout
-    bool reg = (nc == 1 && o->m <= kRegAM);
-    for (int i = 0; i < nsys && reg; i++) reg = sys[i]->N <= kRegNPT * BT;
-    switch (nc) {
-    case 1: rc = reg ? launch(k_batched<1, true>) : launch(k_batched<1, false>); break;
-    case 2: rc = launch(k_batched<2, false>); break;
-    case 3: rc = launch(k_batched<3, false>); break;
-    default: rc = launch(k_batched<4, false>); break;
-    }
+    // Only the register-state kernel is built and launched: the checks above admit
+    // one-component systems of at most kRegNPT * BT points with m <= kRegAM.  The
+    // HBM-state variants (k_batched<NC, false>) stay in the source but are not
+    // instantiated while the multi-component fault is unexplained.
+    int rc = launch(k_batched<1, true>);
     if (rc != CDFT_OK) return rc;
     std::vector<BResult> hr(nsys);
     HIPCHECK_CTX(ctx, hipMemcpyAsync(hr.data(), dr, sizeof(BResult) * nsys, hipMemcpyDeviceToHost, st));

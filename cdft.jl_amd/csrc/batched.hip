@@ -544,7 +544,7 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
             const int t = c * N + i;
             if (S.vext) s += S.vext[t];
             double lgv = S.cp[c] - s;
-            lgv = fmin(fmax(lgv, -100.0), 100.0);          // clamp!(ln_Gx, -100, 100)
+            lgv = clamp_lng(lgv);                          // clamp!(ln_Gx, -100, 100)
             const double x = A.sol(jb, t);
             const double gnew = (b == 1.0) ? lgv : (1.0 - b) * x + b * lgv;
             if (mode == 1) {

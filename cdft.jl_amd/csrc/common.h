@@ -14,6 +14,14 @@
 #define CDFT_NQ 16           // reduction slots per vector pass
 
 namespace cdft {
+// clamp!(ln_Gx, -100, 100) (src/methods/converge.jl:150) with Julia's clamp
+// semantics: a NaN stays NaN (fmin/fmax would replace it by a bound), so a
+// blown-up map evaluation ends aasol with a NaN residual as in the reference.
+__host__ __device__ __attribute__((always_inline)) inline double clamp_lng(double v)
+{
+    return v > 100.0 ? 100.0 : (v < -100.0 ? -100.0 : v);
+}
+
 // Per-system constants of the 3D weights (fft3d.hip).
 struct Grid3 {
     int n[3];                 // points per axis (powers of two)

@@ -29,6 +29,7 @@
 struct PhiConst {
     int nc;
     double T;
+    double psi;                           // dispersion width factor (PCSAFT.jl:53,160)
     double m[CDFT_MAXC];
     double dd[CDFT_MAXC];                 // d_i / L
     double cm[CDFT_MAXC * CDFT_MAXC];     // m_i m_j (sigma_ij / L)^3
@@ -199,7 +200,7 @@ __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + N
     }
 
     // ---- dispersion ----
-    const T psi = T(1.3862);
+    const T psi = T(P.psi);
     const T factor = T(3) / (T(4) * psi * psi * psi * pi);
     T rb[NC], S = 0, mt = 0, es = 0;
 #pragma unroll

@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) void k_update(const double *__restrict__ dphi,
             } else {
                 if (vext) s += vext[t];
                 lg = cp[c] - s;
-                lg = fmin(fmax(lg, -100.0), 100.0);
+                lg = clamp_lng(lg);
             }
             double x = sol[t];
             double g = (beta == 1.0) ? lg : (1.0 - beta) * x + beta * lg;

@@ -12,16 +12,25 @@
  * mixed-radix decimation-in-time transform with twiddles taken from an 80-bit
  * table, so any grid size works.
  */
+#define _POSIX_C_SOURCE 200809L
 #include "oracle.h"
 
 #include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #define PI 3.14159265358979323846
 
 int or_nfields(int nd) { return 6 + nd; }
+
+double or_wtime(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
 
 int or_npoints(const or_system *s)
 {
@@ -412,8 +421,9 @@ double or_phi(const or_system *s, const double *nl, double *dphi)
     if (dphi)
         for (int k = 0; k < nc; k++) dphi[k * F + iz] += m[k] / 8.0 * G3 + m[k] / (8.0 * dd[k]) * G2;
 
-    /* ---- dispersion (psi-averaged density) ---- */
-    double psi = 1.3862;
+    /* ---- dispersion (psi-averaged density; the reference fixes psi = 1.3862,
+     * PCSAFT.jl:53,160, and the descriptor carries it) ---- */
+    double psi = s->psi;
     double factor = 3.0 / (4.0 * psi * psi * psi * PI);
     double rb[OR_MAXC], S = 0, mt = 0, es = 0;
     for (int i = 0; i < nc; i++) {
@@ -462,7 +472,6 @@ double or_phi(const or_system *s, const double *nl, double *dphi)
             dphi[k * F + idp] += g * factor / (dd[k] * dd[k] * dd[k]);
         }
     }
-    (void)psi;
     return phi_hs + phi_hc + phi_disp;
 }
 
@@ -1001,4 +1010,252 @@ void or_adsorption(const or_system *s, const double *rho, double *out)
     structure_dz(s, dz);
     for (int a = 0; a < s->nd; a++) V *= (s->ub[a] - s->lb[a]);
     for (int c = 0; c < s->nc; c++) out[c] = or_simpson(rho + (size_t)c * N, s->nd, s->ngrid, dz) / V;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Planned 1D evaluation.  The same operator as or_weighted_densities /      */
+/* or_transposed, with its pieces computed once per system instead of once  */
+/* per call:                                                                */
+/*   h_f = Re IFFT_N(Omega_f)    the real circulant kernel that             */
+/*                               Re IFFT(FFT(u) Omega_f) applies to a real u */
+/*                               (integrals.jl:88-96), from the literal      */
+/*                               Omega and FFT above;                         */
+/*   H_f = FFT_M(h_f unrolled on a power-of-two ring M >= 2N - 1, or M = N  */
+/*         when N is a power of two): lags -(N-1)..N-1 do not alias, so the  */
+/*         length-M circular convolution of the zero-padded u equals the    */
+/*         N-periodic one on 0..N-1.                                        */
+/* Two real fields share one complex transform; the transposed operator of  */
+/* h_f has the spectrum conj(H_f) (the vector slot's h is odd, so this is    */
+/* the reference's subtraction, weighted_densities.jl:381).  Used by the    */
+/* tests for speed (checked there against the literal functions) and by the */
+/* OpenMP batch driver that times the CPU.                                  */
+/* ------------------------------------------------------------------------ */
+
+struct or_plan {
+    const or_system *s;
+    int N, M, lg, nc, F;
+    double *twr, *twi;                 /* W_M^t = exp(-2 pi i t / M), t < M/2 */
+    int *rev;                          /* bit reversal of log2 M bits         */
+    double *Hr, *Hi;                   /* [(c*F + f)*M + k], f >= 1           */
+    double *ar, *ai, *pr, *pi_, *sr, *si;
+    double *n, *df;                    /* fields and gradients, N*F*nc        */
+};
+
+static void fft_pow2(const or_plan *p, double *re, double *im, int inverse)
+{
+    const int M = p->M;
+    for (int i = 0; i < M; i++) {
+        int j = p->rev[i];
+        if (j > i) {
+            double t = re[i]; re[i] = re[j]; re[j] = t;
+            t = im[i]; im[i] = im[j]; im[j] = t;
+        }
+    }
+    for (int len = 2; len <= M; len <<= 1) {
+        const int half = len >> 1, step = M / len;
+        for (int b = 0; b < M; b += len)
+            for (int j = 0; j < half; j++) {
+                const double wr = p->twr[j * step], wi = inverse ? -p->twi[j * step] : p->twi[j * step];
+                const int a = b + j, c = a + half;
+                const double xr = re[c] * wr - im[c] * wi, xi = re[c] * wi + im[c] * wr;
+                re[c] = re[a] - xr;
+                im[c] = im[a] - xi;
+                re[a] += xr;
+                im[a] += xi;
+            }
+    }
+}
+
+void or_plan_destroy(or_plan *p)
+{
+    if (!p) return;
+    free(p->twr); free(p->twi); free(p->rev); free(p->Hr); free(p->Hi);
+    free(p->ar); free(p->ai); free(p->pr); free(p->pi_); free(p->sr); free(p->si);
+    free(p->n); free(p->df);
+    free(p);
+}
+
+or_plan *or_plan_create(const or_system *s)
+{
+    if (s->nd != 1) return NULL;
+    or_plan *p = calloc(1, sizeof(or_plan));
+    const int N = s->ngrid[0], F = 7, nc = s->nc;
+    int M = 1, lg = 0;
+    if ((N & (N - 1)) == 0) {
+        M = N;
+    } else {
+        while (M < 2 * N - 1) M <<= 1;
+    }
+    while ((1 << lg) < M) lg++;
+    p->s = s; p->N = N; p->M = M; p->lg = lg; p->nc = nc; p->F = F;
+    p->twr = malloc(sizeof(double) * M); p->twi = malloc(sizeof(double) * M);
+    for (int t = 0; t < M; t++) {
+        long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)M;
+        p->twr[t] = (double)cosl(a);
+        p->twi[t] = (double)(-sinl(a));
+    }
+    p->rev = malloc(sizeof(int) * M);
+    for (int i = 0; i < M; i++) {
+        int r = 0;
+        for (int b = 0; b < lg; b++) r |= ((i >> b) & 1) << (lg - 1 - b);
+        p->rev[i] = r;
+    }
+    size_t nk = (size_t)nc * F * M;
+    p->Hr = calloc(nk, sizeof(double)); p->Hi = calloc(nk, sizeof(double));
+    double *ore = malloc(sizeof(double) * (size_t)N * F * nc), *oim = malloc(sizeof(double) * (size_t)N * F * nc);
+    double *hr = malloc(sizeof(double) * N), *hi = malloc(sizeof(double) * N);
+    or_omega(s, ore, oim);
+    for (int c = 0; c < nc; c++)
+        for (int f = 1; f < F; f++) {
+            const size_t o = (size_t)(c * F + f) * N;
+            memcpy(hr, ore + o, sizeof(double) * N);
+            memcpy(hi, oim + o, sizeof(double) * N);
+            or_fft(hr, hi, 1, s->ngrid, 1);                    /* h = Re IFFT_N(Omega) */
+            double *Hr = p->Hr + (size_t)(c * F + f) * M, *Hi = p->Hi + (size_t)(c * F + f) * M;
+            for (int k = 0; k < M; k++) Hi[k] = 0.0;
+            if (M == N) {
+                for (int t = 0; t < N; t++) Hr[t] = hr[t];
+            } else {
+                for (int k = 0; k < M; k++) Hr[k] = 0.0;
+                Hr[0] = hr[0];
+                for (int t = 1; t < N; t++) {
+                    Hr[t] = hr[t];
+                    Hr[M - t] = hr[N - t];
+                }
+            }
+            fft_pow2(p, Hr, Hi, 0);
+        }
+    free(ore); free(oim); free(hr); free(hi);
+    p->ar = malloc(sizeof(double) * M); p->ai = malloc(sizeof(double) * M);
+    p->pr = malloc(sizeof(double) * M); p->pi_ = malloc(sizeof(double) * M);
+    p->sr = malloc(sizeof(double) * M); p->si = malloc(sizeof(double) * M);
+    p->n = malloc(sizeof(double) * (size_t)N * F * nc);
+    p->df = malloc(sizeof(double) * (size_t)N * F * nc);
+    return p;
+}
+
+/* dF_res/drho and (optionally) Phi with a plan; same result as or_dFdrho_res up
+ * to rounding.  A plan's buffers serve one call at a time. */
+void or_dFdrho_res_planned(or_plan *p, const double *rho, double *dfdrho, double *fval)
+{
+    const or_system *s = p->s;
+    const int N = p->N, M = p->M, F = p->F, nc = p->nc;
+    const double NA = OR_NA * (s->L * s->L * s->L), invM = 1.0 / (double)M;
+    double nl[OR_MAXC * 9], dl[OR_MAXC * 9];
+    for (int c = 0; c < nc; c++) {
+        const double *rc = rho + (size_t)c * N;
+        for (int k = 0; k < M; k++) { p->pr[k] = k < N ? rc[k] : 0.0; p->pi_[k] = 0.0; }
+        fft_pow2(p, p->pr, p->pi_, 0);
+        for (int j = 0; j < N; j++) p->n[(size_t)(c * F) * N + j] = rc[j] * NA;
+        for (int q = 0; q < 3; q++) {
+            const int fa = 1 + 2 * q, fb = 2 + 2 * q;
+            const double *Har = p->Hr + (size_t)(c * F + fa) * M, *Hai = p->Hi + (size_t)(c * F + fa) * M;
+            const double *Hbr = p->Hr + (size_t)(c * F + fb) * M, *Hbi = p->Hi + (size_t)(c * F + fb) * M;
+            for (int k = 0; k < M; k++) {
+                const double zr = Har[k] - Hbi[k], zi = Hai[k] + Hbr[k];     /* Z = H_a + i H_b */
+                p->ar[k] = p->pr[k] * zr - p->pi_[k] * zi;
+                p->ai[k] = p->pr[k] * zi + p->pi_[k] * zr;
+            }
+            fft_pow2(p, p->ar, p->ai, 1);
+            double *na = p->n + (size_t)(c * F + fa) * N, *nb = p->n + (size_t)(c * F + fb) * N;
+            for (int j = 0; j < N; j++) {
+                na[j] = p->ar[j] * invM * NA;
+                nb[j] = p->ai[j] * invM * NA;
+            }
+        }
+    }
+    for (int j = 0; j < N; j++) {
+        for (int c = 0; c < nc; c++)
+            for (int f = 0; f < F; f++) nl[c * F + f] = p->n[(size_t)(c * F + f) * N + j];
+        double v = or_phi(s, nl, dl);
+        if (fval) fval[j] = v;
+        for (int c = 0; c < nc; c++)
+            for (int f = 0; f < F; f++) p->df[(size_t)(c * F + f) * N + j] = dl[c * F + f];
+    }
+    for (int c = 0; c < nc; c++) {
+        for (int k = 0; k < M; k++) { p->sr[k] = 0.0; p->si[k] = 0.0; }
+        for (int q = 0; q < 3; q++) {
+            const int fa = 1 + 2 * q, fb = 2 + 2 * q;
+            const double *ga = p->df + (size_t)(c * F + fa) * N, *gb = p->df + (size_t)(c * F + fb) * N;
+            for (int k = 0; k < M; k++) {
+                p->ar[k] = k < N ? ga[k] : 0.0;
+                p->ai[k] = k < N ? gb[k] : 0.0;
+            }
+            fft_pow2(p, p->ar, p->ai, 0);
+            const double *Har = p->Hr + (size_t)(c * F + fa) * M, *Hai = p->Hi + (size_t)(c * F + fa) * M;
+            const double *Hbr = p->Hr + (size_t)(c * F + fb) * M, *Hbi = p->Hi + (size_t)(c * F + fb) * M;
+            for (int k = 0; k < M; k++) {
+                const double zr = Har[k] - Hbi[k], zi = Hai[k] + Hbr[k];     /* S += A conj(Z) */
+                p->sr[k] += p->ar[k] * zr + p->ai[k] * zi;
+                p->si[k] += p->ai[k] * zr - p->ar[k] * zi;
+            }
+        }
+        fft_pow2(p, p->sr, p->si, 1);
+        double *oc = dfdrho + (size_t)c * N;
+        const double *g0 = p->df + (size_t)(c * F) * N;
+        for (int j = 0; j < N; j++) oc[j] = g0[j] + p->sr[j] * invM;
+    }
+}
+
+typedef struct {
+    or_plan *p;
+    double *rho;
+} conv_ctx_planned;
+
+static void gfix_planned(double *G, const double *x, void *vctx)
+{
+    conv_ctx_planned *c = (conv_ctx_planned *)vctx;
+    const or_system *s = c->p->s;
+    const int N = c->p->N;
+    const size_t n = (size_t)N * s->nc;
+    for (size_t i = 0; i < n; i++) c->rho[i] = exp(x[i]);
+    or_dFdrho_res_planned(c->p, c->rho, G, NULL);
+    if (s->vext)
+        for (size_t j = 0; j < n; j++) G[j] += s->vext[j];
+    for (int cc = 0; cc < s->nc; cc++) {
+        const double cp = s->mu_res[cc] + log(s->rho_bulk[cc]);
+        for (int j = 0; j < N; j++) {
+            double v = cp - G[(size_t)cc * N + j];
+            if (v < -100.0) v = -100.0;
+            if (v > 100.0) v = 100.0;
+            G[(size_t)cc * N + j] = v;
+        }
+    }
+}
+
+/* or_converge with a plan (1D only; returns -3 for other grids). */
+int or_converge_planned(const or_system *s, double *rho, int maxit, double beta, double tol,
+                        double anderson_start, int m, or_aa_result *res, double *hist, int hist_cap)
+{
+    or_plan *p = or_plan_create(s);
+    if (!p) return -3;
+    size_t n = (size_t)or_npoints(s) * s->nc;
+    double *x0 = malloc(sizeof(double) * n), *sol = malloc(sizeof(double) * n);
+    double *work = malloc(sizeof(double) * n);
+    for (size_t i = 0; i < n; i++) x0[i] = log(rho[i]);
+    or_aa_opts o = {m, maxit, maxit, tol, tol, beta, beta, anderson_start, anderson_start};
+    conv_ctx_planned c = {p, work};
+    int rc = or_aasol(gfix_planned, &c, (int)n, x0, &o, sol, NULL, hist, hist_cap, NULL, NULL, 0, res);
+    for (size_t i = 0; i < n; i++) rho[i] = exp(sol[i]);
+    free(x0); free(sol); free(work);
+    or_plan_destroy(p);
+    return rc;
+}
+
+/* Independent 1D systems solved by or_converge_planned on nthreads OpenMP
+ * threads (dynamic schedule, one system at a time per thread).  rhos[i] is
+ * overwritten like or_converge's rho; secs[i] (optional) receives the wall time
+ * of system i. */
+int or_converge_batch(const or_system *systems, int nsys, double *const *rhos, int maxit, double beta, double tol,
+                      double anderson_start, int m, int nthreads, or_aa_result *res, double *secs)
+{
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+ : bad)
+    for (int i = 0; i < nsys; i++) {
+        double t0 = or_wtime();
+        int rc = or_converge_planned(&systems[i], rhos[i], maxit, beta, tol, anderson_start, m, &res[i], NULL, 0);
+        if (secs) secs[i] = or_wtime() - t0;
+        bad += (rc < 0);
+    }
+    return bad ? -1 : 0;
 }

@@ -93,6 +93,20 @@ void   or_init_two_phase(const or_system *s, const double *rho2, const double *T
 double or_surface_tension(const or_system *s, const double *rho, const double *mu, double p);
 void   or_adsorption(const or_system *s, const double *rho, double *out);
 
+/* Planned 1D evaluation (kernel spectra computed once per system from the
+ * literal Omega and FFT; same operator, different rounding) and an OpenMP
+ * driver over independent systems.  A plan's buffers serve one call at a time. */
+typedef struct or_plan or_plan;
+or_plan *or_plan_create(const or_system *s);            /* NULL unless nd == 1 */
+void   or_plan_destroy(or_plan *p);
+void   or_dFdrho_res_planned(or_plan *p, const double *rho, double *dfdrho, double *fval);
+int    or_converge_planned(const or_system *s, double *rho, int maxit, double beta, double tol,
+                           double anderson_start, int m, or_aa_result *res, double *hist, int hist_cap);
+int    or_converge_batch(const or_system *systems, int nsys, double *const *rhos, int maxit, double beta,
+                         double tol, double anderson_start, int m, int nthreads, or_aa_result *res,
+                         double *secs);
+double or_wtime(void);
+
 #ifdef __cplusplus
 }
 #endif

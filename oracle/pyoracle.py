@@ -79,6 +79,14 @@ def lib():
         L.or_surface_tension.argtypes = [P, _dp, _dp, C.c_double]
         L.or_surface_tension.restype = C.c_double
         L.or_adsorption.argtypes = [P, _dp, _dp]
+        L.or_plan_create.argtypes = [P]
+        L.or_plan_create.restype = C.c_void_p
+        L.or_plan_destroy.argtypes = [C.c_void_p]
+        L.or_dFdrho_res_planned.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.or_converge_planned.argtypes = L.or_converge.argtypes
+        L.or_converge_batch.argtypes = [P, C.c_int, C.POINTER(_dp), C.c_int, C.c_double, C.c_double,
+                                        C.c_double, C.c_int, C.c_int, C.POINTER(OrAAResult), _dp]
+        L.or_wtime.restype = C.c_double
         _lib = L
     return _lib
 
@@ -158,6 +166,33 @@ class Oracle:
         lib().or_dFdrho_res(self.ref, _ptr(rho), _ptr(out), _ptr(fv))
         return out, fv
 
+    def dFdrho_res_planned(self, rho):
+        """The same operator with the kernel spectra computed once (1D only)."""
+        rho = _f64(rho).reshape(-1)
+        out = np.zeros(self.N * self.nc)
+        fv = np.zeros(self.N)
+        p = lib().or_plan_create(self.ref)
+        if not p:
+            raise ValueError("planned evaluation is 1D only")
+        try:
+            lib().or_dFdrho_res_planned(p, _ptr(rho), _ptr(out), _ptr(fv))
+        finally:
+            lib().or_plan_destroy(p)
+        return out, fv
+
+    def surface_tension_planned(self, rho, mu, p):
+        """or_surface_tension with F_res from the planned evaluation (1D only)."""
+        rho = _f64(rho).reshape(-1)
+        _, fv = self.dFdrho_res_planned(rho)
+        ng = (C.c_int * 3)(*self.s.ngrid)
+        dz = np.array([(self.s.ub[a] - self.s.lb[a]) / self.s.ngrid[a] for a in range(3)])
+        simpson = lambda f: lib().or_simpson(_ptr(_f64(f)), self.nd, ng, _ptr(dz))
+        fres = simpson(fv) / self.s.L ** 3
+        F = self.F_ideal(rho) + fres
+        chem = sum(float(mu[c]) * simpson(rho[c * self.N:(c + 1) * self.N]) for c in range(self.nc))
+        KB = 1.380649e-23
+        return F * KB * self.s.T - chem + p * simpson(np.ones(self.N))
+
     def F_res(self, rho):
         return lib().or_F_res(self.ref, _ptr(_f64(rho).reshape(-1)))
 
@@ -186,12 +221,15 @@ class Oracle:
         return out
 
     def converge(self, rho, *, maxit=10000, beta=1e-2, tol=1e-4, anderson_start=1e-1, m=5,
-                 hist_cap=100000):
+                 hist_cap=100000, planned=False):
+        """or_converge (literal operator) or, with planned=True (1D), or_converge_planned."""
         rho = _f64(rho).reshape(-1).copy()
         res = OrAAResult()
         hist = np.zeros(hist_cap)
-        lib().or_converge(self.ref, _ptr(rho), maxit, beta, tol, anderson_start, m,
-                          C.byref(res), _ptr(hist), hist_cap)
+        fn = lib().or_converge_planned if planned else lib().or_converge
+        rc = fn(self.ref, _ptr(rho), maxit, beta, tol, anderson_start, m, C.byref(res), _ptr(hist), hist_cap)
+        if rc == -3:
+            raise ValueError("planned solve is 1D only")
         return rho, res, hist[:min(res.nhist, hist_cap)]
 
     def surface_tension(self, rho, mu, p):
@@ -201,6 +239,24 @@ class Oracle:
         out = np.zeros(self.nc)
         lib().or_adsorption(self.ref, _ptr(_f64(rho).reshape(-1)), _ptr(out))
         return out
+
+
+def converge_batch(oracles, rhos, *, maxit=10000, beta=1e-2, tol=1e-4, anderson_start=1e-1, m=5,
+                   nthreads=1):
+    """or_converge_batch: independent 1D systems, planned solves on nthreads OpenMP
+    threads.  Returns (profiles, results, per-system seconds, wall seconds)."""
+    n = len(oracles)
+    arr = (OrSystem * n)(*[o.s for o in oracles])
+    outs = [_f64(r).reshape(-1).copy() for r in rhos]
+    ptrs = (_dp * n)(*[_ptr(r) for r in outs])
+    res = (OrAAResult * n)()
+    secs = np.zeros(n)
+    t0 = lib().or_wtime()
+    rc = lib().or_converge_batch(arr, n, ptrs, maxit, beta, tol, anderson_start, m, int(nthreads), res, _ptr(secs))
+    wall = lib().or_wtime() - t0
+    if rc != 0:
+        raise ValueError("or_converge_batch failed (only 1D systems are supported)")
+    return outs, list(res), secs, wall
 
 
 def aasol(gfun, x0, *, m, maxit=20, rtol=1e-10, atol=1e-10, beta=1.0, picard_maxit=0,

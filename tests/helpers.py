@@ -21,3 +21,10 @@ def smooth_profile(system, seed=0, amp=0.3):
         mod = 1 + sum(a[k] * np.sin(2 * np.pi * (k + 1) * z + ph[k]) for k in range(4)) / 2
         rho[c] = system.species.bulk_density[c] * mod
     return rho
+
+
+
+def bulk_mu(system):
+    """mu_i of the bulk (J/mol) the way surface_tension(system, rho) takes it."""
+    from cdft_amd.methods import bulk_chemical_potential
+    return bulk_chemical_potential(system)
