@@ -15,6 +15,7 @@ CDFT_OK = 0
 CDFT_FP64 = 0
 CDFT_MIXED = 1
 MAXC = 8
+NOALLOC_MAX_SYS = 1024     # CDFT_NOALLOC_MAX_SYS
 
 _dp = C.POINTER(C.c_double)
 
@@ -22,7 +23,8 @@ _dp = C.POINTER(C.c_double)
 EXPORTS = [
     "cdft_ctx_create", "cdft_ctx_destroy", "cdft_last_error", "cdft_system_create",
     "cdft_system_destroy", "cdft_system_npoints", "cdft_functional_eval", "cdft_converge",
-    "cdft_converge_batch", "cdft_converge_batch_device", "cdft_last_kernel_ms",
+    "cdft_converge_batch", "cdft_converge_batch_device", "cdft_converge_batch_device_noalloc",
+    "cdft_last_kernel_ms",
     "cdft_surface_tension", "cdft_adsorption", "cdft_phi_vjp", "cdft_hvp", "cdft_adjoint_solve",
 ]
 
@@ -83,6 +85,8 @@ def load():
                                       C.POINTER(SolverOpts), C.POINTER(Result)]
     L.cdft_converge_batch_device.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(_dp),
                                              C.POINTER(SolverOpts), C.POINTER(Result)]
+    L.cdft_converge_batch_device_noalloc.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(_dp),
+                                                     C.POINTER(SolverOpts), C.POINTER(Result)]
     L.cdft_last_kernel_ms.argtypes = [vp]
     L.cdft_last_kernel_ms.restype = C.c_double
     L.cdft_surface_tension.argtypes = [vp, _dp, _dp, C.c_double, _dp]
@@ -237,13 +241,16 @@ class DeviceSystem:
 
 
 def converge_batch(ctx, systems, rhos, *, maxit=10000, beta=1e-2, tol=1e-4, anderson_start=1e-1,
-                   m=5, device_ptrs=False):
+                   m=5, device_ptrs=False, noalloc=False):
     """Batched persistent solve of many one-component mixed-precision systems of at
     most 512 points, m <= 5 (cdft_converge_batch[_device]); other batches raise
     CDFTError.
 
     rhos: list of float64 numpy arrays (in/out), or device pointers (ints) when
-    device_ptrs is True."""
+    device_ptrs is True.  noalloc (device pointers only): the allocation-free entry
+    point cdft_converge_batch_device_noalloc."""
+    if noalloc and not device_ptrs:
+        raise ValueError("the allocation-free batched entry point takes device pointers")
     n = len(systems)
     hs = (C.c_void_p * n)(*[s.handle.value for s in systems])
     if device_ptrs:
@@ -254,6 +261,9 @@ def converge_batch(ctx, systems, rhos, *, maxit=10000, beta=1e-2, tol=1e-4, ande
         ps = (_dp * n)(*[ptr(r) for r in rhos])
     o = SolverOpts(int(maxit), float(beta), float(tol), float(anderson_start), int(m), 0, None)
     res = (Result * n)()
-    fn = load().cdft_converge_batch_device if device_ptrs else load().cdft_converge_batch
+    if noalloc:
+        fn = load().cdft_converge_batch_device_noalloc
+    else:
+        fn = load().cdft_converge_batch_device if device_ptrs else load().cdft_converge_batch
     check(fn(ctx.handle, n, hs, ps, C.byref(o), res), ctx.handle)
     return [r.as_dict() for r in res]

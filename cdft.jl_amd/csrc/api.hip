@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,19 @@ using namespace cdft;
 namespace {
 
 thread_local std::string g_err;   // errors raised before a context exists
+
+// Scratch of the allocation-free batched entry point
+// (cdft_converge_batch_device_noalloc): the system descriptors and the results of
+// one launch, in a static device array.  Each GPU holds its own copy for the life
+// of the process; its address on a device comes from hipGetSymbolAddress, taken
+// once per device at the first context created on it.
+constexpr int kNoAllocMaxSys = CDFT_NOALLOC_MAX_SYS;
+constexpr size_t kScratchDescBytes = ((sizeof(BSys) * kNoAllocMaxSys + 255) / 256) * 256;
+constexpr size_t kScratchBytes = kScratchDescBytes + sizeof(BResult) * kNoAllocMaxSys;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
 
 int fail(cdft_ctx *ctx, int code, const std::string &msg)
 {
@@ -443,13 +457,31 @@ int cdft_ctx_create(int device, cdft_ctx **out)
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev == 0) return fail(nullptr, CDFT_ERR_HIP, "no HIP device available");
     if (device < 0 || device >= ndev) return fail(nullptr, CDFT_ERR_ARG, "device index out of range");
+    if (device >= kMaxDevices) return fail(nullptr, CDFT_ERR_UNSUPPORTED, "device index beyond the library's table");
     auto *c = new cdft_ctx;
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-        delete c;
+        cdft_ctx_destroy(c);
         return fail(nullptr, CDFT_ERR_HIP, "stream/event creation failed");
     }
+    {
+        // this device's copy of the batched scratch array (device is current)
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_dev[device]) {
+            void *p = nullptr;
+            hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                cdft_ctx_destroy(c);
+                return fail(nullptr, CDFT_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            }
+            g_scratch_dev[device] = p;
+        }
+        c->scratch = g_scratch_dev[device];
+    }
+    c->bstage.resize(sizeof(BSys) * kNoAllocMaxSys);
+    c->rstage.resize(sizeof(BResult) * kNoAllocMaxSys);
+    c->border.resize(kNoAllocMaxSys);
     *out = c;
     return CDFT_OK;
 }
@@ -469,6 +501,8 @@ int cdft_ctx_destroy(cdft_ctx *ctx)
 double cdft_last_kernel_ms(const cdft_ctx *ctx) { return ctx ? ctx->last_kernel_ms : 0.0; }
 
 int cdft_system_npoints(const cdft_sys *sys) { return sys ? sys->N : 0; }
+
+static int batched_prepare(cdft_ctx *ctx, cdft_sys *s);
 
 int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
 {
@@ -639,6 +673,17 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
         free_sys_buffers(s);
         delete s;
         return fail(ctx, CDFT_ERR_HIP, std::string("kernel set-up failed: ") + hipGetErrorString(e));
+    }
+    // a system the batched kernel can take gets its ring spectra now, so that the
+    // allocation-free batched entry point finds it ready
+    if (!d3 && s->nc == 1 && s->N <= kRegNPT * BT && s->N >= 5) {
+        int rc = batched_prepare(ctx, s);
+        if (rc == CDFT_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = CDFT_ERR_HIP;
+        if (rc != CDFT_OK) {
+            free_sys_buffers(s);
+            delete s;
+            return rc == CDFT_ERR_HIP ? fail(ctx, rc, "batched set-up failed: " + ctx->err) : rc;
+        }
     }
     *out = s;
     return CDFT_OK;
@@ -1100,18 +1145,22 @@ static int batched_prepare(cdft_ctx *ctx, cdft_sys *s)
 }
 
 static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho, const cdft_solver_opts *o,
-                      cdft_result *results, bool device_ptrs)
+                      cdft_result *results, bool device_ptrs, bool noalloc)
 {
+    const char *who = noalloc ? "cdft_converge_batch_device_noalloc" : "cdft_converge_batch";
     if (!ctx || nsys < 0 || (nsys > 0 && (!sys || !rho || !results)) || !o)
-        return fail(ctx, CDFT_ERR_ARG, "cdft_converge_batch: NULL argument");
+        return fail(ctx, CDFT_ERR_ARG, std::string(who) + ": NULL argument");
     if (nsys == 0) return CDFT_OK;
+    if (noalloc && nsys > kNoAllocMaxSys)
+        return fail(ctx, CDFT_ERR_ARG,
+                    std::string(who) + ": at most " + std::to_string(kNoAllocMaxSys) + " systems per call");
     if (o->m < 0 || o->m > CDFT_MAXM - 1) return fail(ctx, CDFT_ERR_ARG, "anderson m must be in 0..7");
     if (!(std::fabs(o->beta) > 0 && std::fabs(o->beta) <= 1)) return fail(ctx, CDFT_ERR_ARG, "abs(beta) must be in (0,1]");
     if (o->maxit < 0) return fail(ctx, CDFT_ERR_ARG, "maxit must be >= 0");
     const int nc = sys[0] ? sys[0]->nc : 0;
     const int prec = sys[0] ? sys[0]->precision : 0;
     for (int i = 0; i < nsys; i++) {
-        if (!sys[i] || !rho[i]) return fail(ctx, CDFT_ERR_ARG, "cdft_converge_batch: NULL system or profile");
+        if (!sys[i] || !rho[i]) return fail(ctx, CDFT_ERR_ARG, std::string(who) + ": NULL system or profile");
         if (sys[i]->ctx != ctx) return fail(ctx, CDFT_ERR_ARG, "all systems must belong to the context");
         if (sys[i]->nc != nc) return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched systems must share nc");
         // the multi-component batched kernel faulted on the GPU and is switched off
@@ -1133,6 +1182,10 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
             return fail(ctx, CDFT_ERR_UNSUPPORTED,
                         "batched path: at most 512 grid points and anderson m <= 5 for now; use cdft_converge");
         if (sys[i]->nd != 1) return fail(ctx, CDFT_ERR_UNSUPPORTED, "the batched path takes 1D systems");
+        // prepared at creation (spectra on the ring, the ring's twiddles); the
+        // allocating entry points prepare a system here if needed
+        if (noalloc && (!sys[i]->H64 || !ctx->twiddles.count(sys[i]->M)))
+            return fail(ctx, CDFT_ERR_ARG, std::string(who) + ": system not prepared for the batched path");
         for (int j = 0; j < i; j++)
             if (sys[j] == sys[i]) return fail(ctx, CDFT_ERR_ARG, "a system appears twice in the batch");
     }
@@ -1142,8 +1195,10 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
     HIPCHECK_CTX(ctx, hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
     size_t lds_g = 0, lds_nog = 0;
     for (int i = 0; i < nsys; i++) {
-        int rc = batched_prepare(ctx, sys[i]);
-        if (rc != CDFT_OK) return rc;
+        if (!noalloc) {
+            int rc = batched_prepare(ctx, sys[i]);
+            if (rc != CDFT_OK) return rc;
+        }
         lds_g = std::max(lds_g, batched_lds_bytes(sys[i], true));
         lds_nog = std::max(lds_nog, batched_lds_bytes(sys[i], false));
     }
@@ -1153,11 +1208,18 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
     const size_t lds = g_lds ? lds_g : lds_nog;
     if (lds > lds_limit)
         return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched path: system too large for one workgroup's LDS; use cdft_converge");
-    // largest systems first: the tail of the launch is then the short ones
-    std::vector<int> order(nsys);
+    // host staging (sized at context creation; grown here only by the allocating entry points)
+    if (ctx->border.size() < (size_t)nsys) {
+        ctx->border.resize(nsys);
+        ctx->bstage.resize(sizeof(BSys) * nsys);
+        ctx->rstage.resize(sizeof(BResult) * nsys);
+    }
+    // largest systems first: the tail of the launch is then the short ones (ties in
+    // input order; std::sort with the index as tie-break needs no buffer)
+    int *order = ctx->border.data();
     for (int i = 0; i < nsys; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sys[a]->N > sys[b]->N; });
-    std::vector<BSys> hb(nsys);
+    std::sort(order, order + nsys, [&](int a, int b) { return sys[a]->N != sys[b]->N ? sys[a]->N > sys[b]->N : a < b; });
+    BSys *hb = reinterpret_cast<BSys *>(ctx->bstage.data());
     for (int q = 0; q < nsys; q++) {
         cdft_sys *s = sys[order[q]];
         BSys &b = hb[q];
@@ -1172,7 +1234,7 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         for (int c = 0; c < s->nc; c++) b.cp[c] = s->cp[c];
         b.vext = s->vext;
         b.G = s->H64;
-        b.tw64 = ctx->twiddles[s->M];
+        b.tw64 = ctx->twiddles.find(s->M)->second;
         long n = (long)s->N * s->nc;
         if (device_ptrs) {
             b.rho = rho[order[q]];
@@ -1189,13 +1251,24 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         b.DG = s->DG;
         b.pc = s->pc;
     }
+    // device copies of the descriptors and the results: this device's static scratch
+    // (allocation-free entry point) or buffers of this call
     BSys *db = nullptr;
     BResult *dr = nullptr;
-    HIPCHECK_CTX(ctx, hipMalloc((void **)&db, sizeof(BSys) * nsys));
-    struct Free { void *p; ~Free() { if (p) (void)hipFree(p); } } f1{db};
-    HIPCHECK_CTX(ctx, hipMalloc((void **)&dr, sizeof(BResult) * nsys));
-    Free f2{dr};
-    HIPCHECK_CTX(ctx, hipMemcpyAsync(db, hb.data(), sizeof(BSys) * nsys, hipMemcpyHostToDevice, st));
+    struct Free {
+        void *p = nullptr;
+        ~Free() { if (p) (void)hipFree(p); }
+    } f1, f2;
+    if (noalloc) {
+        db = reinterpret_cast<BSys *>(ctx->scratch);
+        dr = reinterpret_cast<BResult *>(static_cast<unsigned char *>(ctx->scratch) + kScratchDescBytes);
+    } else {
+        HIPCHECK_CTX(ctx, hipMalloc((void **)&db, sizeof(BSys) * nsys));
+        f1.p = db;
+        HIPCHECK_CTX(ctx, hipMalloc((void **)&dr, sizeof(BResult) * nsys));
+        f2.p = dr;
+    }
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(db, hb, sizeof(BSys) * nsys, hipMemcpyHostToDevice, st));
     BOpts bo;
     bo.maxit = o->maxit;
     bo.m = o->m;
@@ -1206,22 +1279,18 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
     bo.switch_factor = 4.0;
     bo.stall_window = 40;
     bo.g_lds = g_lds ? 1 : 0;
-    auto launch = [&](auto kern) -> int {
-        HIPCHECK_CTX(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev0, st));
-        hipLaunchKernelGGL(kern, dim3(nsys), dim3(BT), lds, st, db, bo, dr);
-        HIPCHECK_CTX(ctx, hipGetLastError());
-        HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev1, st));
-        return CDFT_OK;
-    };
     // Only the register-state kernel is built and launched: the checks above admit
     // one-component systems of at most kRegNPT * BT points with m <= kRegAM.  The
     // HBM-state variants (k_batched<NC, false>) stay in the source but are not
     // instantiated while the multi-component fault is unexplained.
-    int rc = launch(k_batched<1, true>);
-    if (rc != CDFT_OK) return rc;
-    std::vector<BResult> hr(nsys);
-    HIPCHECK_CTX(ctx, hipMemcpyAsync(hr.data(), dr, sizeof(BResult) * nsys, hipMemcpyDeviceToHost, st));
+    auto kern = k_batched<1, true>;
+    HIPCHECK_CTX(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev0, st));
+    hipLaunchKernelGGL(kern, dim3(nsys), dim3(BT), lds, st, db, bo, dr);
+    HIPCHECK_CTX(ctx, hipGetLastError());
+    HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev1, st));
+    BResult *hr = reinterpret_cast<BResult *>(ctx->rstage.data());
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(hr, dr, sizeof(BResult) * nsys, hipMemcpyDeviceToHost, st));
     if (!device_ptrs)
         for (int q = 0; q < nsys; q++) {
             cdft_sys *s = sys[order[q]];
@@ -1253,13 +1322,19 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
 int cdft_converge_batch(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_inout,
                         const cdft_solver_opts *opts, cdft_result *results)
 {
-    return batch_impl(ctx, nsys, sys, rho_inout, opts, results, false);
+    return batch_impl(ctx, nsys, sys, rho_inout, opts, results, false, false);
 }
 
 int cdft_converge_batch_device(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_dev,
                                const cdft_solver_opts *opts, cdft_result *results)
 {
-    return batch_impl(ctx, nsys, sys, rho_dev, opts, results, true);
+    return batch_impl(ctx, nsys, sys, rho_dev, opts, results, true, false);
+}
+
+int cdft_converge_batch_device_noalloc(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_dev,
+                                       const cdft_solver_opts *opts, cdft_result *results)
+{
+    return batch_impl(ctx, nsys, sys, rho_dev, opts, results, true, true);
 }
 
 }  // extern "C"

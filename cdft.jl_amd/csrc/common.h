@@ -6,6 +6,7 @@
 #include <map>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "cdft_hip.h"
 #include "phi.h"
@@ -39,6 +40,12 @@ struct cdft_ctx {
     std::string err;
     double last_kernel_ms = 0.0;
     std::map<int, double *> twiddles;   // FFT ring length M -> W_M^t table (re, im pairs)
+    // batched launches: this device's static scratch (system descriptors and results
+    // of the allocation-free entry point) and host staging sized at creation
+    void *scratch = nullptr;
+    std::vector<unsigned char> bstage;   // host copies of the system descriptors
+    std::vector<unsigned char> rstage;   // host copies of the results
+    std::vector<int> border;             // launch order of the systems
 };
 
 // Per-system state.  Host copies of the descriptor plus device buffers.

@@ -137,6 +137,21 @@ int cdft_converge_batch(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *c
 int cdft_converge_batch_device(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_dev,
                                const cdft_solver_opts *opts, cdft_result *results);
 
+/* Allocation-free variant of cdft_converge_batch_device.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It calls no
+ * hipMalloc/hipFree: the system descriptors and results of the launch live in a
+ * static device array of the library (one per GPU, about 1.4 MB, held until the
+ * process ends), and every system must have been prepared for the batched path when
+ * it was created (cdft_system_create does so for one-component 1D systems of at most
+ * 512 points).  At most CDFT_NOALLOC_MAX_SYS systems per call (CDFT_ERR_ARG beyond).
+ * Same arguments, restrictions and results as cdft_converge_batch_device.  Calls
+ * that overlap on one GPU (from different contexts or threads) share the array and
+ * race; serialise them. */
+#define CDFT_NOALLOC_MAX_SYS 1024
+int cdft_converge_batch_device_noalloc(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_dev,
+                                       const cdft_solver_opts *opts, cdft_result *results);
+
 /* Stream time in ms of the last solve on the context, measured with HIP events
  * on its stream: the persistent kernel of cdft_converge_batch*, or first to last
  * kernel of cdft_converge / cdft_adjoint_solve (their result's device_ms too). */
