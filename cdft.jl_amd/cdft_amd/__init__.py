@@ -7,8 +7,9 @@ from .structure import HardSphereMatrix, Steele, Structure, TwoPhase1DCart, Unif
 from .system import (DFTOptions, DFTSystem, F_ideal, F_res, converge_, dFdrho_res, free_energy,
                      initialize_profiles, integrate, integration_weights, simpson)
 from .methods import adsorption, surface_tension
+from .ddft import ODEProblem, ddft_rhs
 
 __all__ = ["PCSAFT", "Steele", "HardSphereMatrix", "Structure", "Uniform1DCart", "Uniform3DCart", "TwoPhase1DCart",
            "DFTOptions", "DFTSystem", "initialize_profiles", "converge_", "dFdrho_res", "F_res", "F_ideal",
            "free_energy", "simpson", "integrate", "integration_weights",
-           "surface_tension", "adsorption", "NA", "KB", "RGAS"]
+           "surface_tension", "adsorption", "ddft_rhs", "ODEProblem", "NA", "KB", "RGAS"]

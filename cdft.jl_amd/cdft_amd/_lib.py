@@ -26,6 +26,7 @@ EXPORTS = [
     "cdft_converge_batch", "cdft_converge_batch_device", "cdft_converge_batch_device_noalloc",
     "cdft_last_kernel_ms",
     "cdft_surface_tension", "cdft_adsorption", "cdft_phi_vjp", "cdft_hvp", "cdft_adjoint_solve",
+    "cdft_ddft_rhs",
 ]
 
 
@@ -94,6 +95,7 @@ def load():
     L.cdft_phi_vjp.argtypes = [vp, _dp, _dp, _dp]
     L.cdft_hvp.argtypes = [vp, _dp, _dp, _dp]
     L.cdft_adjoint_solve.argtypes = [vp, _dp, _dp, _dp, C.POINTER(SolverOpts), C.POINTER(Result)]
+    L.cdft_ddft_rhs.argtypes = [vp, _dp, _dp]
     _lib = L
     return L
 
@@ -228,6 +230,12 @@ class DeviceSystem:
         rho, v = self._vec(rho), self._vec(v)
         out = np.zeros(self.N * self.nc)
         check(load().cdft_hvp(self.handle, ptr(rho), ptr(v), ptr(out)), self.ctx.handle)
+        return out
+
+    def ddft_rhs(self, eta):
+        eta = self._vec(eta)
+        out = np.zeros(self.N * self.nc)
+        check(load().cdft_ddft_rhs(self.handle, ptr(eta), ptr(out)), self.ctx.handle)
         return out
 
     def adjoint_solve(self, rho, g, lam0=None, *, maxit=10000, beta=1e-2, tol=1e-10, anderson_start=1e-1, m=5):

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -104,7 +105,7 @@ double simpson1d(const double *f, int n, double dz)
 void free_sys_buffers(cdft_sys *s)
 {
     double **bufs[] = {&s->h, &s->vext, &s->cpd, &s->rho, &s->n, &s->dphi, &s->part, &s->fval, &s->out, &s->sol,
-                       &s->gx, &s->df, &s->dg, &s->res, &s->Q, &s->Qalt, &s->DG, &s->red, &s->fin};
+                       &s->gx, &s->df, &s->dg, &s->res, &s->Q, &s->Qalt, &s->DG, &s->red, &s->fin, &s->hddft};
     for (double **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (s->H64) { (void)hipFree(s->H64); s->H64 = nullptr; }
@@ -1067,6 +1068,77 @@ int cdft_adjoint_solve(cdft_sys *s, const double *rho, const double *g, double *
     return aasol_run(s, lambda_inout, o, r, &am);     // s->n keeps W rho* for every product
 }
 
+int cdft_ddft_rhs(cdft_sys *s, const double *eta, double *deta)
+{
+    if (!s || !eta || !deta) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_ddft_rhs: NULL argument");
+    cdft_ctx *ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    const long n = (long)s->N * s->nc;
+    const int N = s->N, nc = s->nc;
+    HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
+    DevBufs tmp;
+    double *etad = nullptr, *detad = nullptr;
+    HIPCHECK_CTX(ctx, tmp.get(&etad, n));
+    HIPCHECK_CTX(ctx, tmp.get(&detad, n));
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(etad, eta, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_ddft_rho, dim3(nblocks(n)), dim3(256), 0, st, etad, s->rho, (int)n);
+    launch_functional(s);
+    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part,
+                       (const double *)nullptr, s->N, s->nc, upd_src(s).F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
+                       (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
+                       (const double *)nullptr, (const double *)nullptr);
+    if (s->nd == 1) {
+        if (!s->hddft) {
+            HIPCHECK_CTX(ctx, hipMalloc((void **)&s->hddft, sizeof(double) * 4 * n));
+            HIPCHECK_CTX(ctx, hipMemsetAsync(s->hddft, 0, sizeof(double) * 4 * n, st));
+            const double mult = ((double)N / (s->ub[0] - s->lb[0])) / (double)N;   // fftfreq spacing
+            hipLaunchKernelGGL(k_setup_hddft, dim3(nblocks(N), nc * 4), dim3(256), 0, st, s->hddft, N, nc, mult, s->L);
+        }
+        double *src = nullptr, *d = nullptr;
+        HIPCHECK_CTX(ctx, tmp.get(&src, 4 * n));
+        HIPCHECK_CTX(ctx, tmp.get(&d, 3 * n));
+        hipLaunchKernelGGL(k_ddft_src, dim3(nblocks(n)), dim3(256), 0, st, etad, s->out, s->vext, N, nc, src,
+                           (double *)nullptr);
+        hipLaunchKernelGGL(k_circ1d<double>, dim3((N + CIRC_TW - 1) / CIRC_TW, nc * 3), dim3(256), 0, st, src,
+                           (long)4 * N, (long)N, s->hddft, d, (long)3 * N, (long)N, 1, N, 4, 1.0, 0);
+        hipLaunchKernelGGL(k_ddft_combine, dim3(nblocks(n)), dim3(256), 0, st, d, N, nc, detad);
+    } else {
+        double *mu = nullptr;
+        HIPCHECK_CTX(ctx, tmp.get(&mu, n));
+        hipLaunchKernelGGL(k_ddft_src, dim3(nblocks(n)), dim3(256), 0, st, etad, s->out, s->vext, N, nc,
+                           (double *)nullptr, mu);
+        for (int c = 0; c < nc; c++) {
+            Pass3 p;
+            std::memset(&p, 0, sizeof(p));
+            p.in = etad + (size_t)c * N;
+            p.in2 = mu + (size_t)c * N;
+            p.out = s->A3;
+            fft3_pass<double, 0, IN_PAIR, OUT_CPLX>(s, p, false);          // FFT(eta + i mu)
+            p.in = s->A3;
+            fft3_pass<double, 1, IN_CPLX, OUT_CPLX>(s, p, false);
+            fft3_pass<double, 2, IN_CPLX, OUT_CPLX>(s, p, false);
+            for (int kind = 0; kind < 4; kind++) {
+                Pass3 q;
+                std::memset(&q, 0, sizeof(q));
+                q.in = s->A3;
+                q.out = s->B3;
+                q.mkind = kind;
+                fft3_pass<double, 2, IN_MULT, OUT_CPLX>(s, q, true);
+                q.in = s->B3;
+                fft3_pass<double, 1, IN_CPLX, OUT_CPLX>(s, q, true);
+                q.out = detad + (size_t)c * N;
+                q.scale = 1.0 / (double)N;
+                if (kind == 0) fft3_pass<double, 0, IN_CPLX, OUT_DDFT0>(s, q, true);
+                else fft3_pass<double, 0, IN_CPLX, OUT_DDFT1>(s, q, true);
+            }
+        }
+    }
+    HIPCHECK_CTX(ctx, hipGetLastError());
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(deta, detad, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+    return CDFT_OK;
+}
+
 int cdft_surface_tension(cdft_sys *s, const double *rho, const double *mu, double p, double *out)
 {
     if (!s || !rho || !mu || !out) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_surface_tension: NULL argument");
@@ -1115,7 +1187,10 @@ static int fft_length(int N)
     return M;
 }
 
-static size_t batched_lds_bytes(const cdft_sys *s, bool g_lds) { return batched_lds_layout(s->N, s->M, s->nc, g_lds); }
+static size_t batched_lds_bytes(const cdft_sys *s, bool g_lds, bool dg_lds)
+{
+    return batched_lds_layout(s->N, s->M, s->nc, g_lds, dg_lds);
+}
 
 static int batched_prepare(cdft_ctx *ctx, cdft_sys *s)
 {
@@ -1193,14 +1268,25 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
     hipStream_t st = ctx->stream;
     int maxlds = 0;
     HIPCHECK_CTX(ctx, hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+    // Workgroup size: 256 threads (two points per thread) by default; 512 threads (one
+    // point per thread, the field-pair FFTs spread over the whole workgroup) with
+    // CDFT_BATCH_THREADS=512.  Both run an evaluation of a 446-point pore in the same
+    // 23.5 us (the solve is bound by the chain of LDS passes and barriers, not by work
+    // per thread), so the smaller kernel stays the default.
+    int bt = BT;
+    if (const char *env = std::getenv("CDFT_BATCH_THREADS")) {
+        const int v = std::atoi(env);
+        if (v == BT || v == 2 * BT) bt = v;
+    }
+    const bool dg_lds = (bt == 2 * BT);               // the 512-thread kernel keeps Delta-G in LDS
     size_t lds_g = 0, lds_nog = 0;
     for (int i = 0; i < nsys; i++) {
         if (!noalloc) {
             int rc = batched_prepare(ctx, sys[i]);
             if (rc != CDFT_OK) return rc;
         }
-        lds_g = std::max(lds_g, batched_lds_bytes(sys[i], true));
-        lds_nog = std::max(lds_nog, batched_lds_bytes(sys[i], false));
+        lds_g = std::max(lds_g, batched_lds_bytes(sys[i], true, dg_lds));
+        lds_nog = std::max(lds_nog, batched_lds_bytes(sys[i], false, dg_lds));
     }
     // spectra in LDS when the largest system leaves room for them, else read from HBM
     const size_t lds_limit = std::max<size_t>((size_t)maxlds, 160 * 1024) - sizeof(AState) - sizeof(PhiConst);
@@ -1283,10 +1369,16 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
     // one-component systems of at most kRegNPT * BT points with m <= kRegAM.  The
     // HBM-state variants (k_batched<NC, false>) stay in the source but are not
     // instantiated while the multi-component fault is unexplained.
-    auto kern = k_batched<1, true>;
+    // chain-free batches (every segment number 1, e.g. methane) use a kernel without
+    // the chain term's code paths, which needs fewer registers
+    bool chain_free = true;
+    for (int i = 0; i < nsys && chain_free; i++)
+        for (int c = 0; c < sys[i]->nc; c++) chain_free = chain_free && sys[i]->m[c] == 1.0;
+    auto kern = (bt == BT) ? (chain_free ? k_batched<1, true, BT, true> : k_batched<1, true, BT, false>)
+                           : (chain_free ? k_batched<1, true, 2 * BT, true> : k_batched<1, true, 2 * BT, false>);
     HIPCHECK_CTX(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev0, st));
-    hipLaunchKernelGGL(kern, dim3(nsys), dim3(BT), lds, st, db, bo, dr);
+    hipLaunchKernelGGL(kern, dim3(nsys), dim3(bt), lds, st, db, bo, dr);
     HIPCHECK_CTX(ctx, hipGetLastError());
     HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev1, st));
     BResult *hr = reinterpret_cast<BResult *>(ctx->rstage.data());

@@ -63,7 +63,7 @@ struct BResult {
     unsigned long long ticks;   // wall_clock64 ticks spent in the workgroup
 };
 
-constexpr int BT = 256;        // threads per workgroup
+constexpr int BT = 256;        // threads per workgroup of the 256-thread kernel (512 is the other)
 
 // ---------------------------------------------------------------- helpers
 template <typename T>
@@ -192,6 +192,104 @@ CDFT_INLINE void fft_fused(c2<T> *W, const c2<double> *tw, int M, int lg, bool i
     }
 }
 
+// The same Stockham passes with the butterflies of the NARR arrays spread over the
+// workgroup: item it = q nb + j is butterfly j of array q, so with 512 threads and
+// two arrays of M = 1024 each thread runs one butterfly per pass (fft_fused gives
+// thread j butterfly j of every array and leaves threads >= M/4 idle).  The first
+// pass reads its inputs through load0(q, idx); the last pass hands its outputs to
+// storeL(q, idx, v) or, with LAST_W, writes them into W like the passes between (a
+// barrier then separates that pass's reads from its writes).  The arithmetic of each
+// butterfly is fft_fused's.  Ends with a barrier.
+template <typename T, int NARR, int BTT, bool LAST_W, class Load0, class StoreL>
+CDFT_INLINE void fft_split(c2<T> *W, const c2<double> *tw, int M, int lg, bool inverse, Load0 &&load0,
+                           StoreL &&storeL)
+{
+    constexpr int IT4 = (NARR * 256 + BTT - 1) / BTT;   // radix-4 items per thread (M/4 <= 256)
+    constexpr int IT2 = (NARR * 512 + BTT - 1) / BTT;   // radix-2 items per thread (M/2 <= 512)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));         // opaque: keeps the index arithmetic inside the solver loop
+    const T sg = inverse ? T(1) : T(-1);
+    const int nr4 = lg >> 1;
+    const bool odd = (lg & 1) != 0;
+    int Ns = 1, lNs = 0;
+    for (int pass = 0; pass < nr4; pass++) {
+        const bool first = (pass == 0), last = (!odd && pass == nr4 - 1);
+        const int lnb = lg - 2, nb = M >> 2, nitems = NARR << lnb;
+        c2<T> v[IT4][4];
+#pragma unroll
+        for (int r = 0; r < IT4; r++) {
+            const int it = tid + r * BTT;
+            if (it < nitems) {
+                const int q = it >> lnb, j = it & (nb - 1);
+                if (first) {
+#pragma unroll
+                    for (int u = 0; u < 4; u++) v[r][u] = load0(q, j + u * nb);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; u++) v[r][u] = W[q * M + j + u * nb];
+                    const c2<double> wd = tw[(j & (Ns - 1)) << (lg - 2 - lNs)];
+                    const c2<T> w1 = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
+                    const c2<T> w2 = cmul(w1, w1), w3 = cmul(w1, w2);
+                    v[r][1] = cmul(v[r][1], w1);
+                    v[r][2] = cmul(v[r][2], w2);
+                    v[r][3] = cmul(v[r][3], w3);
+                }
+                radix4<T>(v[r], sg);
+            }
+        }
+        if (!first && (!last || LAST_W)) __syncthreads();     // in place: every read before any write
+#pragma unroll
+        for (int r = 0; r < IT4; r++) {
+            const int it = tid + r * BTT;
+            if (it < nitems) {
+                const int q = it >> lnb, j = it & (nb - 1);
+                const int d = ((j >> lNs) << (lNs + 2)) + (j & (Ns - 1));
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (last && !LAST_W) storeL(q, d + u * Ns, v[r][u]);
+                    else W[q * M + d + u * Ns] = v[r][u];
+                }
+            }
+        }
+        __syncthreads();
+        Ns <<= 2;
+        lNs += 2;
+    }
+    if (odd) {
+        const int lnb = lg - 1, nb = M >> 1, nitems = NARR << lnb;
+        c2<T> v[IT2][2];
+#pragma unroll
+        for (int r = 0; r < IT2; r++) {
+            const int it = tid + r * BTT;
+            if (it < nitems) {
+                const int q = it >> lnb, j = it & (nb - 1);
+                const c2<double> wd = tw[(j & (Ns - 1)) << (lg - 1 - lNs)];
+                const c2<T> w = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
+                const c2<T> x0 = W[q * M + j], x1 = cmul(W[q * M + j + nb], w);
+                v[r][0] = {x0.x + x1.x, x0.y + x1.y};
+                v[r][1] = {x0.x - x1.x, x0.y - x1.y};
+            }
+        }
+        if (LAST_W) __syncthreads();
+#pragma unroll
+        for (int r = 0; r < IT2; r++) {
+            const int it = tid + r * BTT;
+            if (it < nitems) {
+                const int q = it >> lnb, j = it & (nb - 1);
+                const int d = ((j >> lNs) << (lNs + 1)) + (j & (Ns - 1));
+                if (LAST_W) {
+                    W[q * M + d] = v[r][0];
+                    W[q * M + d + Ns] = v[r][1];
+                } else {
+                    storeL(q, d, v[r][0]);
+                    storeL(q, d + Ns, v[r][1]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Cross-lane moves on FP64 values, as two 32-bit halves: DPP within a row of 16
 // lanes (quad_perm xor 1 = 0xB1, xor 2 = 0x4E, row_half_mirror = 0x141,
 // row_mirror = 0x140) and reads of one lane (uniform lane index).
@@ -228,10 +326,11 @@ CDFT_INLINE double sum8(double x)
 // Consecutive calls alternate between two halves of red (2 x 64 doubles), so one
 // barrier per call suffices: a half is rewritten two calls later, after the
 // intervening call's barrier, when every thread has read it.
-template <int NQ>
+template <int NQ, int BTT = BT>
 CDFT_INLINE void wg_sum(double (&v)[NQ], double *red2, int &par)
 {
-    static_assert(NQ * 4 <= 64, "reduction slots");
+    constexpr int NW = BTT / 64;                  // waves of the workgroup (4 or 8)
+    static_assert(NQ * NW <= 64, "reduction slots");
     double *red = red2 + (par ? 64 : 0);
     par ^= 1;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -243,10 +342,16 @@ CDFT_INLINE void wg_sum(double (&v)[NQ], double *red2, int &par)
     }
     if (lane == 0)
 #pragma unroll
-        for (int q = 0; q < NQ; q++) red[q * 4 + wid] = v[q];
+        for (int q = 0; q < NQ; q++) red[q * NW + wid] = v[q];
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < NQ; q++) v[q] = (red[q * 4 + 0] + red[q * 4 + 1]) + (red[q * 4 + 2] + red[q * 4 + 3]);
+    for (int q = 0; q < NQ; q++) {
+        const double *r = red + q * NW;
+        double t;
+        if constexpr (NW == 4) t = (r[0] + r[1]) + (r[2] + r[3]);
+        else t = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+        v[q] = readlane_f64(t, 0);            // the same bits in every lane: keep it scalar
+    }
 }
 
 // One system's data, read once from its descriptor into registers.
@@ -261,18 +366,24 @@ struct SysRegs {
 
 // Per-point Anderson state of one system (anderson.jl:665-703: sol, gx, res, df,
 // dg and the Q / Delta-G history columns).  REG: in registers of the thread that
-// owns the points i = tid + b BT (b < NPT, one component, at most AM history
-// columns), so an Anderson step touches no memory; otherwise in HBM ([i] and
-// [col n + i]).  Column indices that are not compile-time constants after
-// unrolling go through the set/get helpers, which select among the registers.
-constexpr int kRegNPT = 2;     // points per thread of the register-resident state
+// owns the points i = tid + b BTT (b < NPT, one component, at most AM history
+// columns), so an Anderson step touches no HBM; otherwise in HBM ([i] and
+// [col n + i]).  With one point per thread (512 threads) the Delta-G columns sit in
+// LDS instead ([col kRegPoints + i], each thread touching only its own points), which
+// keeps the kernel within the 256 registers two waves per SIMD allow.  Column
+// indices that are not compile-time constants after unrolling go through the set/get
+// helpers, which select among the registers.
+constexpr int kRegNPT = 2;     // points per thread of the register-resident state (256 threads)
 constexpr int kRegAM = 5;      // history columns it holds (Anderson m <= 5)
+constexpr int kRegPoints = kRegNPT * BT;   // points of one system the register state holds
 
-template <bool REG>
+template <bool REG, int NPTT = kRegNPT>
 struct AStore {
-    static constexpr int NPT = REG ? kRegNPT : 1, AM = REG ? kRegAM : 1;
-    double sol_[NPT], gx_[NPT], res_[NPT], df_[NPT], dg_[NPT], Q_[NPT][AM], DG_[NPT][AM];
+    static constexpr int NPT = REG ? NPTT : 1, AM = REG ? kRegAM : 1;
+    static constexpr bool DGL = REG && NPTT == 1;          // Delta-G columns in LDS
+    double sol_[NPT], gx_[NPT], res_[NPT], df_[NPT], dg_[NPT], Q_[NPT][AM], DG_[NPT][DGL ? 1 : AM];
     double *gsol, *ggx, *gres, *gdf, *gdg, *gQ, *gDG;
+    double *dgl;                                            // LDS Delta-G columns (DGL)
     long n;
 
     CDFT_INLINE double &sol(int b, int i) { if constexpr (REG) return sol_[b]; else return gsol[i]; }
@@ -294,9 +405,17 @@ struct AStore {
         if constexpr (REG) Q_[b][j] = c ? v : Q_[b][j];
         else if (c) gQ[j * n + i] = v;
     }
+    // Delta-G column s (compile-time s < AM) at this thread's point b
+    CDFT_INLINE double DGcol(int b, int i, int s)
+    {
+        if constexpr (DGL) return dgl[s * kRegPoints + i];
+        else return DG_[b][s];
+    }
     CDFT_INLINE double getDG(int b, int i, int col)
     {
-        if constexpr (REG) {
+        if constexpr (DGL) {
+            return dgl[col * kRegPoints + i];
+        } else if constexpr (REG) {
             double v = 0.0;
 #pragma unroll
             for (int s = 0; s < AM; s++) v = (s == col) ? DG_[b][s] : v;
@@ -307,7 +426,9 @@ struct AStore {
     }
     CDFT_INLINE void setDG(int b, int i, int col, double v)
     {
-        if constexpr (REG) {
+        if constexpr (DGL) {
+            dgl[col * kRegPoints + i] = v;
+        } else if constexpr (REG) {
 #pragma unroll
             for (int s = 0; s < AM; s++) DG_[b][s] = (s == col) ? v : DG_[b][s];   // no dynamic register index
         } else {
@@ -329,18 +450,18 @@ struct AStore {
 // history columns stay one basic block (same rounding as the guarded a += x y).
 CDFT_INLINE double fma_if(bool c, double a, double x, double y) { return c ? a + x * y : a; }
 
-// Calls fn(b, i) for the points i < n of this thread: i = tid + b BT.
-template <bool REG, typename Fn>
+// Calls fn(b, i) for the points i < n of this thread: i = tid + b BTT.
+template <bool REG, int BTT = BT, typename Fn>
 CDFT_INLINE void for_points(int n, Fn &&fn)
 {
     if constexpr (REG) {
 #pragma unroll
-        for (int b = 0; b < kRegNPT; b++) {
-            const int i = (int)threadIdx.x + b * BT;
+        for (int b = 0; b < kRegPoints / BTT; b++) {
+            const int i = (int)threadIdx.x + b * BTT;
             if (i < n) fn(b, i);
         }
     } else {
-        for (int i = threadIdx.x; i < n; i += BT) fn(0, i);
+        for (int i = threadIdx.x; i < n; i += BTT) fn(0, i);
     }
 }
 
@@ -375,12 +496,24 @@ CDFT_INLINE void pair_multipliers(const double *Gc, int G1, int M, int k, c2<T> 
     }
 }
 
+// Z_q alone (the split FFT handles one pair per butterfly).
+template <typename T, int NP>
+CDFT_INLINE c2<T> pair_multiplier(const double *Gc, int G1, int M, int k, int q)
+{
+    const bool up = k > (M >> 1);
+    const int kk = up ? M - k : k;
+    if (q == 0) return {T(Gc[kk]), T(Gc[G1 + kk])};
+    if (NP == 3 && q == 2) return {T(Gc[4 * G1 + kk]), T(Gc[5 * G1 + kk])};
+    const double g3 = up ? -Gc[2 * G1 + kk] : Gc[2 * G1 + kk];
+    return {T(0), T(g3 + Gc[(NP == 3 ? 3 : 5) * G1 + kk])};
+}
+
 // Phi and dPhi/dn at the points tid, tid + nthr, tid + 2 nthr, ... (threads
 // tid < nthr; nthr a multiple of 64), in place over the field slots.  A wave whose
 // first lane has a second point evaluates two points per lane side by side (a lane
 // without one repeats its first and discards it): with CHAIN = false the two
 // inlined bodies are branch free and interleave.
-template <typename T, int NC, bool CHAIN>
+template <typename T, int NC, bool CHAIN, int BTT = BT>
 CDFT_INLINE void phi_points(const PhiConst &P, T *fld, int N, int nthr)
 {
     constexpr int F = 7;
@@ -397,6 +530,15 @@ CDFT_INLINE void phi_points(const PhiConst &P, T *fld, int N, int nthr)
 #pragma unroll
             for (int f = 0; f < F; f++) fld[(c * F + f) * N + i] = g[c][f];
     };
+    if constexpr (BTT > BT) {                    // one point per lane: enough waves to overlap
+        for (int i = tid; i < N; i += nthr) {
+            T nl[NC][F], g[NC][F];
+            load(nl, i);
+            phi_grad<T, double, NC, 1, CHAIN>(P, nl, g);
+            store(g, i);
+        }
+        return;
+    }
     for (int i = tid; i < N; i += 2 * nthr) {
         T nl[NC][F], g[NC][F];
         load(nl, i);
@@ -424,19 +566,20 @@ CDFT_INLINE void phi_points(const PhiConst &P, T *fld, int N, int nthr)
 // LDS: tw (M, FP64), spectra G (6 NC (M/2+1) FP64, or global), three work arrays W
 // and the accumulator P (M complex T each, at FP64 stride), fields fld (F*N*NC of T
 // at FP64 stride), red (2 x 64 doubles, used alternately).
-template <typename T, int NC, bool REG, class Side>
-CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, const c2<double> *tw, const double *G,
-                       c2<T> *W, c2<T> *Pc, T *fld, double *red, int &rpar, int mode, double b,
-                       double (&out)[3], bool side_on, Side &&side)
+template <typename T, int NC, bool REG, int BTT, bool CF, class Side>
+CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG, kRegPoints / BTT> &A, const PhiConst &P,
+                       const c2<double> *tw, const double *G, c2<T> *W, c2<T> *Pc, T *fld, double *red, int &rpar,
+                       int mode, double b, double (&out)[3], bool side_on, Side &&side)
 {
+    using AS = AStore<REG, kRegPoints / BTT>;
     constexpr int F = 7;
     const int tid = threadIdx.x;
     const int N = S.N, M = S.M, lg = S.lg, G1 = (S.M >> 1) + 1;
     const T invM = T(1) / T(M);
     const T sc = T(S.NA) * invM;
-    bool chain = false;
+    bool chain = false;                 // CF: every segment number is 1 (known at launch)
 #pragma unroll
-    for (int c = 0; c < NC; c++) chain = chain || (P.m[c] != 1.0);
+    for (int c = 0; c < NC; c++) chain = chain || (!CF && P.m[c] != 1.0);
     using NP3 = std::integral_constant<int, 3>;
     using NP2 = std::integral_constant<int, 2>;
 
@@ -444,11 +587,11 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
         const double *Gc = G + (size_t)c * (F - 1) * G1;
         // rho = exp(sol): slot 0 of the fields and the zero-padded FFT input
 #pragma unroll
-        for (int j = 0; j < 1024 / BT; j++) {
-            const int k = tid + j * BT;
+        for (int j = 0; j < 1024 / BTT; j++) {
+            const int k = tid + j * BTT;
             if (k < M) {
                 if (k < N) {
-                    const int jb = j < AStore<REG>::NPT ? j : AStore<REG>::NPT - 1;
+                    const int jb = j < AS::NPT ? j : AS::NPT - 1;
                     const double r = exp(A.sol(jb, c * N + k));
                     fld[(c * F) * N + k] = T(S.NA * r);
                     Pc[k] = {T(r), T(0)};
@@ -467,6 +610,20 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
         // pass, the last pass writes the weighted densities
         auto weighted = [&](auto np) __attribute__((always_inline)) {
             constexpr int NP = decltype(np)::value;
+            if constexpr (BTT > BT) {
+                fft_split<T, NP, BTT, false>(
+                    W, tw, M, lg, true,
+                    [&](int q, int idx) __attribute__((always_inline)) {
+                        return cmul(Pc[idx], pair_multiplier<T, NP>(Gc, G1, M, idx, q));
+                    },
+                    [&](int q, int idx, const c2<T> &v) __attribute__((always_inline)) {
+                        if (idx < N) {
+                            fld[(c * F + pair_slot<NP>(q, 0)) * N + idx] = sc * v.x;
+                            fld[(c * F + pair_slot<NP>(q, 1)) * N + idx] = sc * v.y;
+                        }
+                    });
+                return;
+            }
             fft_fused<T, NP>(
                 W, tw, M, lg, true,
                 [&](int idx, c2<T>(&v)[NP]) __attribute__((always_inline)) {
@@ -491,14 +648,16 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
     }
     // Phi and dPhi/dn, in place over the field slots
     {
-        const int r_all = (N + 2 * BT - 1) / (2 * BT), r_part = (N + 2 * (BT - 64) - 1) / (2 * (BT - 64));
-        const int nthr = (side_on && r_part <= r_all) ? BT - 64 : BT;
+        constexpr int PPL = BTT > BT ? 1 : 2;         // points per lane and round
+        const int r_all = (N + PPL * BTT - 1) / (PPL * BTT);
+        const int r_part = (N + PPL * (BTT - 64) - 1) / (PPL * (BTT - 64));
+        const int nthr = (side_on && r_part <= r_all) ? BTT - 64 : BTT;
         if (tid < nthr) {
-            if (chain) phi_points<T, NC, true>(P, fld, N, nthr);
-            else phi_points<T, NC, false>(P, fld, N, nthr);
+            if (chain) phi_points<T, NC, true, BTT>(P, fld, N, nthr);
+            else phi_points<T, NC, false, BTT>(P, fld, N, nthr);
         }
     }
-    if ((tid >> 6) == BT / 64 - 1) side();          // the last wave has the fewest Phi points
+    if ((tid >> 6) == BTT / 64 - 1) side();          // the last wave has the fewest Phi points
     __syncthreads();
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
     for (int c = 0; c < NC; c++) {
@@ -508,6 +667,32 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
         // spectrum
         auto transposed = [&](auto np) __attribute__((always_inline)) {
             constexpr int NP = decltype(np)::value;
+            if constexpr (BTT > BT) {
+                // the pairs' spectra stay in W; their sum with conj(Z_q) is one more
+                // pointwise pass
+                fft_split<T, NP, BTT, true>(
+                    W, tw, M, lg, false,
+                    [&](int q, int idx) __attribute__((always_inline)) {
+                        return (idx < N) ? c2<T>{fld[(c * F + pair_slot<NP>(q, 0)) * N + idx],
+                                                 fld[(c * F + pair_slot<NP>(q, 1)) * N + idx]}
+                                         : c2<T>{T(0), T(0)};
+                    },
+                    [&](int, int, const c2<T> &) __attribute__((always_inline)) {});
+                for (int idx = tid; idx < M; idx += BTT) {
+                    c2<T> Z[NP];
+                    pair_multipliers<T, NP>(Gc, G1, M, idx, Z);
+                    c2<T> acc = {T(0), T(0)};
+#pragma unroll
+                    for (int q = 0; q < NP; q++) {
+                        const c2<T> v = W[q * M + idx];
+                        acc.x += v.x * Z[q].x + v.y * Z[q].y;      // v * conj(Z)
+                        acc.y += v.y * Z[q].x - v.x * Z[q].y;
+                    }
+                    Pc[idx] = acc;
+                }
+                __syncthreads();
+                return;
+            }
             fft_fused<T, NP>(
                 W, tw, M, lg, false,
                 [&](int idx, c2<T>(&v)[NP]) __attribute__((always_inline)) {
@@ -536,10 +721,10 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
             [&](int idx, c2<T>(&v)[1]) __attribute__((always_inline)) { v[0] = Pc[idx]; },
             [&](int idx, const c2<T>(&v)[1]) __attribute__((always_inline)) { Pc[idx] = v[0]; });
 #pragma unroll
-        for (int j = 0; j < 1024 / BT; j++) {
-            const int i = tid + j * BT;
+        for (int j = 0; j < 1024 / BTT; j++) {
+            const int i = tid + j * BTT;
             if (i >= N) continue;
-            const int jb = j < AStore<REG>::NPT ? j : AStore<REG>::NPT - 1;
+            const int jb = j < AS::NPT ? j : AS::NPT - 1;
             double s = double(fld[(c * F) * N + i]) + double(Pc[i].x * invM);
             const int t = c * N + i;
             if (S.vext) s += S.vext[t];
@@ -568,7 +753,7 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
         __syncthreads();
     }
     double v[3] = {v0, v1, v2};
-    wg_sum<3>(v, red, rpar);
+    wg_sum<3, BTT>(v, red, rpar);
     out[0] = v[0];
     out[1] = v[1];
     out[2] = v[2];
@@ -582,14 +767,15 @@ CDFT_INLINE void evalG(const SysRegs<NC> &S, AStore<REG> &A, const PhiConst &P, 
 // LDS.  Run by one whole wave, which holds the matrix one element per lane, row
 // i = lane & 7 and column j = lane >> 3, and reduces over the 8 lanes of a column
 // with DPP steps.
+template <int MX = CDFT_MAXM>
 CDFT_INLINE void wave_downdate(double *R, int m, double *Qx)
 {
     const int lane = threadIdx.x & 63, i = lane & 7, j = lane >> 3;
     const int nr = m, ncol = m - 1;
     double a = (i < nr && j < ncol) ? R[(j + 1) * m + i] : 0.0;
-    double tau[CDFT_MAXM];
+    double tau[MX];
 #pragma unroll
-    for (int h = 0; h < CDFT_MAXM; h++) {
+    for (int h = 0; h < MX; h++) {
         tau[h] = 0.0;
         if (h >= ncol) continue;
         const double x = __shfl(a, h * 8 + i, 64);                 // A[i][h]
@@ -609,7 +795,7 @@ CDFT_INLINE void wave_downdate(double *R, int m, double *Qx)
     // Qx = H_0 H_1 ... H_{ncol-1} I[:, 0:ncol]
     double q = (i == j) ? 1.0 : 0.0;
 #pragma unroll
-    for (int h = CDFT_MAXM - 1; h >= 0; h--) {
+    for (int h = MX - 1; h >= 0; h--) {
         if (h >= ncol || tau[h] == 0.0) continue;
         const double vh = __shfl(a, h * 8 + i, 64);
         const double v = (i == h) ? 1.0 : ((i > h && i < nr) ? vh : 0.0);
@@ -625,19 +811,20 @@ CDFT_INLINE void wave_downdate(double *R, int m, double *Qx)
 // theta = R[0:mk, 0:mk] \ tr by column-oriented back substitution.  The last column
 // (mk - 1) and tr come in registers (uniform over the workgroup), the other columns
 // from R in LDS; lane l of every wave ends holding theta_l (l < mk).
-CDFT_INLINE double dev_triangular_solve(const double *R, const double (&rc)[CDFT_MAXM], const double (&tr)[CDFT_MAXM],
+template <int MX = CDFT_MAXM>
+CDFT_INLINE double dev_triangular_solve(const double *R, const double (&rc)[MX], const double (&tr)[MX],
                                         int m, int mk)
 {
     const int lane = threadIdx.x & 63;
     double x = 0.0, last = 0.0;
 #pragma unroll
-    for (int l = 0; l < CDFT_MAXM; l++) {
+    for (int l = 0; l < MX; l++) {
         x = (lane == l && l < mk) ? tr[l] : x;
         last = (lane == l) ? rc[l] : last;
     }
     double th = 0.0;
 #pragma unroll
-    for (int j = CDFT_MAXM - 1; j >= 0; j--) {
+    for (int j = MX - 1; j >= 0; j--) {
         if (j >= mk) continue;
         const bool lastc = (j == mk - 1);
         const double rjj = lastc ? rc[j] : R[j * m + j];
@@ -656,20 +843,23 @@ struct AState {
 
 // Dynamic LDS bytes of one system (FP64 layout; the FP32 map uses the same slots):
 // twiddles, three work arrays and the accumulator (M complex each), the spectra when
-// they live in LDS, the fields and the reduction slots.
-__host__ __device__ inline size_t batched_lds_layout(int N, int M, int nc, bool g_lds)
+// they live in LDS, the fields, the reduction slots and, for the 512-thread kernel,
+// the Delta-G history columns.
+__host__ __device__ inline size_t batched_lds_layout(int N, int M, int nc, bool g_lds, bool dg_lds = false)
 {
-    return (size_t)5 * M * 16 + (g_lds ? (size_t)6 * nc * ((M >> 1) + 1) * 8 : 0) + (size_t)7 * N * nc * 8 + 128 * 8;
+    return (size_t)5 * M * 16 + (g_lds ? (size_t)6 * nc * ((M >> 1) + 1) * 8 : 0) + (size_t)7 * N * nc * 8 + 128 * 8 +
+           (dg_lds ? (size_t)kRegAM * kRegPoints * 8 : 0);
 }
 
-template <int NC, bool REG>
-__global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems, BOpts o, BResult *results)
+template <int NC, bool REG, int BTT = BT, bool CF = false>
+__global__ __launch_bounds__(BTT) void k_batched(const BSys *__restrict__ systems, BOpts o, BResult *results)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ AState st;
     __shared__ PhiConst pcs;
     constexpr int F = 7;
     constexpr int AM = REG ? kRegAM : CDFT_MAXM;     // history columns addressed
+    constexpr int MX = AM;                             // largest Anderson depth of this kernel
     const int tid = threadIdx.x;
     const unsigned long long t_start = wall_clock64();
 
@@ -685,7 +875,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
     for (int c = 0; c < NC; c++) S.cp[c] = D->cp[c];
     S.vext = D->vext;
     S.rho = D->rho;
-    AStore<REG> A;
+    AStore<REG, kRegPoints / BTT> A;
     A.gsol = D->sol;
     A.ggx = D->gx;
     A.gdf = D->df;
@@ -698,7 +888,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
     {
         const double *src = reinterpret_cast<const double *>(&D->pc);
         double *dst = reinterpret_cast<double *>(&pcs);
-        for (int i = tid; i < (int)(sizeof(PhiConst) / sizeof(double)); i += BT) dst[i] = src[i];
+        for (int i = tid; i < (int)(sizeof(PhiConst) / sizeof(double)); i += BTT) dst[i] = src[i];
     }
     const int N = S.N, M = S.M, n = S.n;
     const int ng = (F - 1) * NC * ((M >> 1) + 1);
@@ -708,13 +898,14 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
     double *Gl = reinterpret_cast<double *>(P64 + M);
     double *fld64 = Gl + (o.g_lds ? ng : 0);
     double *red = fld64 + (size_t)F * N * NC;
+    A.dgl = red + 128;                                // Delta-G columns when they live in LDS
     c2<float> *W32 = reinterpret_cast<c2<float> *>(W64);
     c2<float> *P32 = reinterpret_cast<c2<float> *>(P64);
     float *fld32 = reinterpret_cast<float *>(fld64);
-    for (int t = tid; t < M; t += BT) tw[t] = {twg[2 * t], twg[2 * t + 1]};
+    for (int t = tid; t < M; t += BTT) tw[t] = {twg[2 * t], twg[2 * t + 1]};
     const double *G = D->G;
     if (o.g_lds) {
-        for (int t = tid; t < ng; t += BT) Gl[t] = G[t];
+        for (int t = tid; t < ng; t += BTT) Gl[t] = G[t];
         G = Gl;
     }
 
@@ -728,7 +919,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
     int rpar = 0;                                          // which half of red the next sum uses
     double fo[3];
 
-    for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = log(S.rho[i]); });   // x0 = log(rho)
+    for_points<REG, BTT>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = log(S.rho[i]); });   // x0 = log(rho)
     __syncthreads();
 
     // aasol (anderson.jl:301-443) as a phase machine with a single map-evaluation
@@ -748,7 +939,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
         } else if (phase == PH_RESTART) {
             if (tid == 0)
                 for (int j = 0; j < CDFT_MAXM * CDFT_MAXM; j++) st.R[j] = 0.0;
-            for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });   // copy!(sol, gx)
+            for_points<REG, BTT>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });   // copy!(sol, gx)
             __syncthreads();
             k += 1;
             ka = 1;
@@ -757,7 +948,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
             fp32 = false;                                           // FP32 noise floor: FP64 map from here
         } else if (phase == PH_STEP) {
             if (m == 0) {
-                for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });
+                for_points<REG, BTT>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });
                 __syncthreads();
             } else {
                 // aa_point! (anderson.jl) with the Delta-G history as a ring: the
@@ -775,14 +966,14 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                 const int kqq = kq < m - 1 ? kq : m - 1;            // the new Q / R column
                 const bool down = (kq > m - 1);
                 double nrm;
-                double rc[CDFT_MAXM], tr[CDFT_MAXM];                // new R column, Q^T res
+                double rc[MX], tr[MX];                              // new R column, Q^T res
 #pragma unroll
-                for (int j = 0; j < CDFT_MAXM; j++) {
+                for (int j = 0; j < MX; j++) {
                     rc[j] = 0.0;
                     tr[j] = 0.0;
                 }
                 if (kq == 0) {                                      // aa_qr_update! first column
-                    for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.setDG(b, i, slot, A.dg(b, i)); });
+                    for_points<REG, BTT>(n, [&](int b, int i) __attribute__((always_inline)) { A.setDG(b, i, slot, A.dg(b, i)); });
                     nrm = sqrt(dfn2);
                     rc[0] = nrm;
                     tr[0] = dfres / nrm;
@@ -792,7 +983,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                     double v1[2 * (AM - 1)];
 #pragma unroll
                     for (int j = 0; j < 2 * (AM - 1); j++) v1[j] = 0.0;
-                    for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) {
+                    for_points<REG, BTT>(n, [&](int b, int i) __attribute__((always_inline)) {
                         A.setDG(b, i, slot, A.dg(b, i));
                         double qv[AM];
                         if (down) {
@@ -819,12 +1010,12 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                             v1[(AM - 1) + j] = fma_if(j < kqq, v1[(AM - 1) + j], qv[j], rv);
                         }
                     });
-                    wg_sum<2 * (AM - 1)>(v1, red, rpar);
+                    wg_sum<2 * (AM - 1), BTT>(v1, red, rpar);
                     // pass 2: df -= Q h1; h2 = Q^T df
                     double v2[AM - 1];
 #pragma unroll
                     for (int j = 0; j < AM - 1; j++) v2[j] = 0.0;
-                    for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) {
+                    for_points<REG, BTT>(n, [&](int b, int i) __attribute__((always_inline)) {
                         double qv[AM], a = 0.0;
 #pragma unroll
                         for (int j = 0; j < AM - 1; j++) {
@@ -836,10 +1027,10 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
 #pragma unroll
                         for (int j = 0; j < AM - 1; j++) v2[j] = fma_if(j < kqq, v2[j], qv[j], d);
                     });
-                    wg_sum<AM - 1>(v2, red, rpar);
+                    wg_sum<AM - 1, BTT>(v2, red, rpar);
                     // pass 3: df -= Q h2; |df|^2, df.res
                     double v3[2] = {0.0, 0.0};
-                    for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) {
+                    for_points<REG, BTT>(n, [&](int b, int i) __attribute__((always_inline)) {
                         double a = 0.0;
 #pragma unroll
                         for (int j = 0; j < AM - 1; j++) a = fma_if(j < kqq, a, A.Qif(b, i, j, j < kqq), v2[j]);
@@ -848,7 +1039,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                         v3[0] += d * d;
                         v3[1] += d * A.res(b, i);
                     });
-                    wg_sum<2>(v3, red, rpar);
+                    wg_sum<2, BTT>(v3, red, rpar);
                     nrm = sqrt(v3[0]);
 #pragma unroll
                     for (int j = 0; j < AM - 1; j++)
@@ -864,19 +1055,19 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                         }
                 }
                 // R's new column for the next downdate; nothing reads it in this step
-                if (tid < CDFT_MAXM && tid <= kqq) {
+                if (tid < MX && tid <= kqq) {
                     double r = 0.0;
 #pragma unroll
-                    for (int j = 0; j < CDFT_MAXM; j++) r = (j == tid) ? rc[j] : r;
+                    for (int j = 0; j < MX; j++) r = (j == tid) ? rc[j] : r;
                     st.R[kqq * m + tid] = r;
                 }
-                const double thl = dev_triangular_solve(st.R, rc, tr, m, mk);
+                const double thl = dev_triangular_solve<MX>(st.R, rc, tr, m, mk);
                 // coefficient of each physical Delta-G column: theta of its ring position
-                double cf[AM], th[CDFT_MAXM];
+                double cf[AM], th[MX];
 #pragma unroll
                 for (int s2 = 0; s2 < AM; s2++) cf[s2] = 0.0;
 #pragma unroll
-                for (int j = 0; j < CDFT_MAXM; j++) {
+                for (int j = 0; j < MX; j++) {
                     const double thj = readlane_f64(thl, j);
                     th[j] = thj;
                     const int col = (j < mk) ? rot + j - (rot + j >= m ? m : 0) : -1;
@@ -884,16 +1075,19 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
                     for (int s2 = 0; s2 < AM; s2++) cf[s2] = (s2 == col) ? thj : cf[s2];
                 }
                 // new Q column; sol = gx - DG theta
-                for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) {
+                for_points<REG, BTT>(n, [&](int b, int i) __attribute__((always_inline)) {
                     const double dv = A.df(b, i);
                     A.setQ(b, i, kqq, dv / nrm);
                     double a = 0.0;
                     if constexpr (REG) {
+                        // physical columns 0 .. mk-1 hold the ring's mk live columns; the
+                        // others were never written (leftover register or LDS bits, possibly
+                        // Inf/NaN), so they are selected out rather than multiplied by 0
 #pragma unroll
-                        for (int s2 = 0; s2 < AM; s2++) a += A.DG_[b][s2] * cf[s2];
+                        for (int s2 = 0; s2 < AM; s2++) a = fma_if(s2 < mk, a, A.DGcol(b, i, s2), cf[s2]);
                     } else {
 #pragma unroll
-                        for (int j = 0; j < CDFT_MAXM; j++)
+                        for (int j = 0; j < MX; j++)
                             if (j < mk) a += A.getDG(b, i, rot + j - (rot + j >= m ? m : 0)) * th[j];
                     }
                     A.sol(b, i) = A.gx(b, i) - a;
@@ -910,13 +1104,13 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
         // when ka > m) depends only on R, so the last wave does it meanwhile
         const bool dd = (phase == PH_STEP || phase == PH_RESTART) && m >= 1 && ka > m;
         auto side = [&]() __attribute__((always_inline)) {
-            if (dd) wave_downdate(st.R, m, st.Qx);
+            if (dd) wave_downdate<MX>(st.R, m, st.Qx);
         };
         if (fp32) {
-            evalG<float, NC, REG>(S, A, pcs, tw, G, W32, P32, fld32, red, rpar, mode, bb, fo, dd, side);
+            evalG<float, NC, REG, BTT, CF>(S, A, pcs, tw, G, W32, P32, fld32, red, rpar, mode, bb, fo, dd, side);
             nevals32++;
         } else {
-            evalG<double, NC, REG>(S, A, pcs, tw, G, W64, P64, fld64, red, rpar, mode, bb, fo, dd, side);
+            evalG<double, NC, REG, BTT, CF>(S, A, pcs, tw, G, W64, P64, fld64, red, rpar, mode, bb, fo, dd, side);
         }
         nevals++;
         resnorm = sqrt(fo[0]);
@@ -931,7 +1125,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
             if (toosoon) {
                 phase = PH_DONE;
             } else if (picard_maxit > 0) {
-                for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });
+                for_points<REG, BTT>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });
                 __syncthreads();
                 pit = 0;
                 phase = (picard_maxit - 1 > 0) ? PH_PICARD : PH_RESTART;
@@ -942,7 +1136,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
             pit++;
             const bool brk = (resnorm <= picard_tol || resnorm >= up_bd);
             if (!brk) {
-                for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });
+                for_points<REG, BTT>(n, [&](int b, int i) __attribute__((always_inline)) { A.sol(b, i) = A.gx(b, i); });
                 __syncthreads();
             }
             if (brk || pit >= picard_maxit - 1) {
@@ -985,7 +1179,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BSys *__restrict__ systems
     else if (resnorm >= up_bd) { idid = 0; errcode = -2; }
     else if (resnorm <= tol && !fp32) { idid = 1; errcode = 0; }
     else { idid = 0; errcode = 10; }
-    for_points<REG>(n, [&](int b, int i) __attribute__((always_inline)) { S.rho[i] = exp(A.sol(b, i)); });
+    for_points<REG, BTT>(n, [&](int b, int i) __attribute__((always_inline)) { S.rho[i] = exp(A.sol(b, i)); });
     if (tid == 0) {
         BResult r;
         r.idid = idid;

@@ -35,8 +35,8 @@ __device__ __attribute__((always_inline)) inline cx<T> cxmul(cx<T> a, cx<T> b)
 }
 
 
-enum { IN_CPLX = 0, IN_RHO = 1, IN_PAIR = 2, IN_SPEC = 3 };
-enum { OUT_CPLX = 0, OUT_FIELDS = 1, OUT_ACCUM = 2, OUT_REAL = 3 };
+enum { IN_CPLX = 0, IN_RHO = 1, IN_PAIR = 2, IN_SPEC = 3, IN_MULT = 4 };
+enum { OUT_CPLX = 0, OUT_FIELDS = 1, OUT_ACCUM = 2, OUT_REAL = 3, OUT_DDFT0 = 4, OUT_DDFT1 = 5 };
 
 struct Pass3 {
     const void *in;           // IN_CPLX / IN_SPEC: complex T; IN_RHO: double rho; IN_PAIR: T fields
@@ -49,8 +49,34 @@ struct Pass3 {
     int v1, v2;               // 1 where the field is a vector weight (H = i G)
     int first;                // OUT_ACCUM: overwrite instead of accumulate
     int inverse;
-    double scale;             // OUT_FIELDS / OUT_REAL
+    double scale;             // OUT_FIELDS / OUT_REAL / OUT_DDFT*
+    int mkind;                // IN_MULT: 0 the Laplacian map, 1 + a the gradient map along axis a
 };
+
+// The DDFT derivative maps at grid index g (ext/SciMLBasecDFTExt.jl:16-19): the
+// Laplacian -(2 pi)^2 |k|^2 (real) or the gradient 2 pi i k_a, k the reduced
+// frequencies; the gradient's Nyquist term along a is not Hermitian and real()
+// drops it, so it is 0 here.
+__device__ __attribute__((always_inline)) inline cx<double> ddft_map(const Grid3 &G, long g, int kind)
+{
+    const int n0 = G.n[0], n1 = G.n[1];
+    const int ii[3] = {(int)(g % n0), (int)((g / n0) % n1), (int)(g / ((long)n0 * n1))};
+    const double twopi = 2.0 * kPi;
+    if (kind == 0) {
+        double k2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const int ks = (ii[a] < (G.n[a] + 1) / 2) ? ii[a] : ii[a] - G.n[a];
+            const double w = (double)ks * G.mult[a];
+            k2 += w * w;
+        }
+        return {-twopi * twopi * k2, 0.0};
+    }
+    const int a = kind - 1;
+    if (2 * ii[a] == G.n[a]) return {0.0, 0.0};
+    const int ks = (ii[a] < (G.n[a] + 1) / 2) ? ii[a] : ii[a] - G.n[a];
+    return {0.0, twopi * (double)ks * G.mult[a]};
+}
 
 // Z = H1 + i H2 at one frequency from the tabulated G (H = G or i G).
 template <typename T>
@@ -279,6 +305,10 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
                 v = {T(reinterpret_cast<const double *>(p.in)[g]), T(0)};
             } else if (IN == IN_PAIR) {
                 v = {reinterpret_cast<const T *>(p.in)[g], reinterpret_cast<const T *>(p.in2)[g]};
+            } else if (IN == IN_MULT) {
+                const cx<T> sv = reinterpret_cast<const cx<T> *>(p.in)[g];
+                const cx<double> m = ddft_map(G, g, p.mkind);
+                v = cxmul(sv, cx<T>{T(m.x), T(m.y)});
             } else {  // IN_SPEC: P Z
                 const cx<T> s = reinterpret_cast<const cx<T> *>(p.in)[g];
                 v = cxmul(s, pair_z<T>(p, g));
@@ -305,6 +335,11 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
         } else if (OUT == OUT_REAL) {
             reinterpret_cast<double *>(p.out)[g] =
                 p.scale * double(v.x) + (p.base ? double(reinterpret_cast<const T *>(p.base)[g]) : 0.0);
+        } else if (OUT == OUT_DDFT0) {          // lap(eta) + lap(mu)
+            reinterpret_cast<double *>(p.out)[g] = p.scale * double(v.x) + p.scale * double(v.y);
+        } else if (OUT == OUT_DDFT1) {          // += grad(eta)^2 + grad(mu) grad(eta)
+            const double ge = p.scale * double(v.x), gm = p.scale * double(v.y);
+            reinterpret_cast<double *>(p.out)[g] += ge * ge + gm * ge;
         } else {  // OUT_ACCUM: S += conj(Z) A
             const cx<T> z = pair_z<T>(p, g);
             const cx<T> t = {v.x * z.x + v.y * z.y, v.y * z.x - v.x * z.y};

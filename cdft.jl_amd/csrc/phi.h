@@ -38,29 +38,34 @@ struct PhiConst {
 
 namespace cdft {
 
-constexpr double kCorr1[7][3] = {
-    {0.9105631445, -0.3084016918, -0.0906148351},
-    {0.6361281449, 0.1860531159, 0.4527842806},
-    {2.6861347891, -2.5030047259, 0.5962700728},
-    {-26.547362491, 21.419793629, -1.7241829131},
-    {97.759208784, -65.25588533, -4.1302112531},
-    {-159.59154087, 83.318680481, 13.77663187},
-    {91.297774084, -33.74692293, -8.6728470368}};
-constexpr double kCorr2[7][3] = {
-    {0.7240946941, -0.5755498075, 0.0976883116},
-    {2.2382791861, 0.6995095521, -0.2557574982},
-    {-4.0025849485, 3.892567339, -9.155856153},
-    {-21.003576815, -17.215471648, 20.642075974},
-    {26.855641363, 192.67226447, -38.804430052},
-    {206.55133841, -161.82646165, 93.626774077},
-    {-355.60235612, -165.20769346, -29.666905585}};
-
 constexpr double kPi = 3.14159265358979323846;
 
-// I(mbar, eta) of PCSAFT.jl:199-211 and its partials in mbar and eta.
+// The a_qk / b_qk tables of PCSAFT_CORR1/2 (PCSAFT.jl:86-103, Gross & Sadowski
+// 2001), in constant memory and read through an opaque pointer in i_lite, so the
+// compiler re-reads them at each evaluation instead of holding 42 constants in
+// registers across a whole solve.
+__device__ __constant__ double kCorrDev[2][7][3] = {
+    {{0.9105631445, -0.3084016918, -0.0906148351},
+     {0.6361281449, 0.1860531159, 0.4527842806},
+     {2.6861347891, -2.5030047259, 0.5962700728},
+     {-26.547362491, 21.419793629, -1.7241829131},
+     {97.759208784, -65.25588533, -4.1302112531},
+     {-159.59154087, 83.318680481, 13.77663187},
+     {91.297774084, -33.74692293, -8.6728470368}},
+    {{0.7240946941, -0.5755498075, 0.0976883116},
+     {2.2382791861, 0.6995095521, -0.2557574982},
+     {-4.0025849485, 3.892567339, -9.155856153},
+     {-21.003576815, -17.215471648, 20.642075974},
+     {26.855641363, 192.67226447, -38.804430052},
+     {206.55133841, -161.82646165, 93.626774077},
+     {-355.60235612, -165.20769346, -29.666905585}}};
+
+// I(mbar, eta) of PCSAFT.jl:199-211 and its partials in mbar and eta (table 0 or 1).
 template <typename T>
-__device__ __forceinline__ void i_lite(const double (&corr)[7][3], T mb, T eta, T &I, T &Im, T &Ie)
+__device__ __forceinline__ void i_lite(int which, T mb, T eta, T &I, T &Im, T &Ie)
 {
+    const double *tab = &kCorrDev[which][0][0];
+    asm volatile("" : "+s"(tab));
     T m2 = (mb - T(1)) / mb;
     T m3 = m2 * (mb - T(2)) / mb;
     T inv = T(1) / mb;
@@ -69,7 +74,7 @@ __device__ __forceinline__ void i_lite(const double (&corr)[7][3], T mb, T eta, 
     T v = 0, vm = 0, ve = 0, ep = 1, epm1 = 0;
 #pragma unroll
     for (int q = 0; q < 7; q++) {
-        T c0 = T(corr[q][0]), c1 = T(corr[q][1]), c2 = T(corr[q][2]);
+        T c0 = T(tab[3 * q]), c1 = T(tab[3 * q + 1]), c2 = T(tab[3 * q + 2]);
         T a = c0 + m2 * c1 + m3 * c2;
         T am = dm2 * c1 + dm3 * c2;
         v += a * ep;
@@ -236,8 +241,8 @@ __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + N
     T q12 = i1e * i1e * i2e * i2e;
     T C1 = T(1) + mb * Pn / om4 + (T(1) - mb) * Qn * q12;
     T I1, I1m, I1e, I2, I2m, I2e;
-    i_lite<T>(kCorr1, mb, eta, I1, I1m, I1e);
-    i_lite<T>(kCorr2, mb, eta, I2, I2m, I2e);
+    i_lite<T>(0, mb, eta, I1, I1m, I1e);
+    i_lite<T>(1, mb, eta, I2, I2m, I2e);
     T iC1 = T(1) / C1;
     phi += -T(2) * (pi * I1 * A1) - pi * mb * I2 * A2 * iC1;
     {

@@ -67,6 +67,76 @@ __global__ void k_setup_h1d(double *h, int N, int nc, const double *dd, double p
     h[(size_t)cf * N + j] = acc / (double)N;
 }
 
+// Real-space kernels of the DDFT derivative maps on a 1D grid (set-up, FP64):
+// h[(c F + 1) N + j] for the Laplacian map -(2 pi)^2 k^2 and h[(c F + f) N + j],
+// f = 2, 3, for the gradient map 2 pi i k (F = 4; the same for every component),
+// h = Re IDFT_N(map) with k the reduced frequencies omega L; the gradient's Nyquist
+// term is imaginary and real() drops it (ext/SciMLBasecDFTExt.jl:16-19,
+// integrals.jl:88-96).
+__global__ void k_setup_hddft(double *h, int N, int nc, double mult, double L)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cf = blockIdx.y;                 // c*4 + f
+    const int f = cf % 4;
+    if (j >= N || f == 0) return;
+    const double twopi = 2.0 * kPi;
+    const int nnonneg = (N + 1) / 2;
+    double acc = 0.0;
+    for (int k = 0; k < N; k++) {
+        const int ks = (k < nnonneg) ? k : k - N;
+        const double w = ((double)ks * mult) * L;
+        const long r = ((long)k * (long)j) % N;
+        double sn, cs;
+        sincospi(2.0 * (double)r / (double)N, &sn, &cs);
+        if (f == 1) {
+            acc += -twopi * twopi * w * w * cs;
+        } else {
+            if (2 * k == N) continue;
+            acc -= twopi * w * sn;                // Re(i 2 pi w e^{i theta}) = -2 pi w sin(theta)
+        }
+    }
+    h[(size_t)cf * N + j] = acc / (double)N;
+}
+
+// rho = exp(clamp(eta, -50, 30)) with Julia's clamp (NaN stays NaN)
+// (ext/SciMLBasecDFTExt.jl:27).
+__global__ void k_ddft_rho(const double *eta, double *rho, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double e = eta[i];
+    rho[i] = exp(e > 30.0 ? 30.0 : (e < -50.0 ? -50.0 : e));
+}
+
+// Sources of the DDFT derivatives: mu = dF_res/drho + beta V_ext; per component the
+// slots 1 (eta + mu, for the Laplacian), 2 (eta) and 3 (mu) of src[(c 4 + f) N + i].
+__global__ void k_ddft_src(const double *eta, const double *dfdrho, const double *vext, int N, int nc, double *src,
+                           double *mu)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * nc) return;
+    const int c = t / N, i = t - c * N;
+    const double m = dfdrho[t] + (vext ? vext[t] : 0.0);
+    const double e = eta[t];
+    if (mu) mu[t] = m;
+    if (src) {
+        src[(size_t)(c * 4 + 1) * N + i] = e + m;
+        src[(size_t)(c * 4 + 2) * N + i] = e;
+        src[(size_t)(c * 4 + 3) * N + i] = m;
+    }
+}
+
+// deta = lap(eta + mu) + grad(eta)^2 + grad(mu) grad(eta) from the convolved slots
+// d[(c 3 + f - 1) N + i] (1D).
+__global__ void k_ddft_combine(const double *d, int N, int nc, double *deta)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * nc) return;
+    const int c = t / N, i = t - c * N;
+    const double lap = d[(size_t)(c * 3) * N + i], ge = d[(size_t)(c * 3 + 1) * N + i], gm = d[(size_t)(c * 3 + 2) * N + i];
+    deta[t] = lap + ge * ge + gm * ge;
+}
+
 // Circular convolution dst[i] = scale * sum_k h[k] src[(i - k) mod N] for a batch
 // of (src, h, dst) triples selected by blockIdx.y = c*(F-1) + (f-1).  A workgroup
 // owns CIRC_TW consecutive outputs; each of its four waves sums one quarter of the

@@ -185,6 +185,15 @@ int cdft_hvp(cdft_sys *sys, const double *rho, const double *v, double *out);
 int cdft_adjoint_solve(cdft_sys *sys, const double *rho, const double *g, double *lambda_inout,
                        const cdft_solver_opts *opts, cdft_result *result);
 
+/* Dynamic DFT: the right-hand side d(eta)/dt of the log-density form, eta = ln rho
+ * (nc*N values, same layout as rho): rho = exp(clamp(eta, -50, 30)),
+ * mu = dF_res/drho + beta V_ext, and per component
+ *   deta = lap(eta) + lap(mu) + sum_d grad_d(eta)^2 + grad_d(mu) . grad_d(eta),
+ * the derivatives spectral over the periodic box in reduced lengths (z / L).
+ * Replaces ddft_rhs_log! of ODEProblem(system, rho, tspan)
+ * (ext/SciMLBasecDFTExt.jl:10-61); an ODE integrator calls it per stage. */
+int cdft_ddft_rhs(cdft_sys *sys, const double *eta, double *deta);
+
 #ifdef __cplusplus
 }
 #endif

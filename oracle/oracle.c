@@ -1259,3 +1259,77 @@ int or_converge_batch(const or_system *systems, int nsys, double *const *rhos, i
     }
     return bad ? -1 : 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* DDFT right-hand side in the log-density form (ext/SciMLBasecDFTExt.jl:   */
+/* 10-61): rho = exp(clamp(eta, -50, 30)), mu = dF_res/drho + beta V_ext     */
+/* (the ideal propagator adds nothing, propagator/ideal.jl:1-3), then per    */
+/* component                                                                 */
+/*   deta = lap(eta) + lap(mu) + sum_d grad_d(eta)^2 + grad_d(mu) grad_d(eta) */
+/* with every derivative a convolve! (integrals.jl:88-96): real(IFFT(FFT(u)  */
+/* .* map)), map_lapl = -(2 pi)^2 |k|^2 and map_grad_d = 2 pi i k_d, k the   */
+/* reduced frequencies omega L (structure_omega, structure.jl:153-185).      */
+/* ------------------------------------------------------------------------ */
+
+static void ddft_convolve(const or_system *s, const double *src, int kind, double *dst, double *br, double *bi)
+{
+    /* kind 0: Laplacian; 1 + d: gradient along axis d */
+    int N = or_npoints(s), nd = s->nd;
+    int n0 = s->ngrid[0], n1 = nd > 1 ? s->ngrid[1] : 1;
+    for (int j = 0; j < N; j++) { br[j] = src[j]; bi[j] = 0.0; }
+    or_fft(br, bi, nd, s->ngrid, 0);
+    const double twopi = 2.0 * PI;
+    for (int idx = 0; idx < N; idx++) {
+        int ii[3] = {idx % n0, (idx / n0) % n1, idx / (n0 * n1)};
+        double mr, mi;
+        if (kind == 0) {
+            double k2 = 0.0;
+            for (int a = 0; a < nd; a++) {
+                double k = omega_axis(s, a, ii[a]);
+                k2 += k * k;
+            }
+            mr = -twopi * twopi * k2;
+            mi = 0.0;
+        } else {
+            mr = 0.0;
+            mi = twopi * omega_axis(s, kind - 1, ii[kind - 1]);
+        }
+        double r = br[idx] * mr - bi[idx] * mi, im = br[idx] * mi + bi[idx] * mr;
+        br[idx] = r;
+        bi[idx] = im;
+    }
+    or_fft(br, bi, nd, s->ngrid, 1);
+    for (int j = 0; j < N; j++) dst[j] = br[j];
+}
+
+void or_ddft_rhs(const or_system *s, const double *eta, double *deta)
+{
+    int N = or_npoints(s), nc = s->nc, nd = s->nd;
+    size_t n = (size_t)N * nc;
+    double *rho = malloc(sizeof(double) * n), *mu = malloc(sizeof(double) * n);
+    double *br = malloc(sizeof(double) * N), *bi = malloc(sizeof(double) * N);
+    double *tmp = malloc(sizeof(double) * N), *g = malloc(sizeof(double) * N);
+    for (size_t i = 0; i < n; i++) {
+        double e = eta[i];
+        if (e < -50.0) e = -50.0;
+        if (e > 30.0) e = 30.0;
+        rho[i] = exp(e);
+    }
+    or_dFdrho_res(s, rho, mu, NULL);
+    if (s->vext)
+        for (size_t i = 0; i < n; i++) mu[i] += s->vext[i];
+    for (int c = 0; c < nc; c++) {
+        const double *ec = eta + (size_t)c * N, *mc = mu + (size_t)c * N;
+        double *dc = deta + (size_t)c * N;
+        ddft_convolve(s, ec, 0, dc, br, bi);
+        ddft_convolve(s, mc, 0, tmp, br, bi);
+        for (int j = 0; j < N; j++) dc[j] += tmp[j];
+        for (int d = 0; d < nd; d++) {
+            ddft_convolve(s, ec, 1 + d, tmp, br, bi);
+            for (int j = 0; j < N; j++) dc[j] += tmp[j] * tmp[j];
+            ddft_convolve(s, mc, 1 + d, g, br, bi);
+            for (int j = 0; j < N; j++) dc[j] += g[j] * tmp[j];
+        }
+    }
+    free(rho); free(mu); free(br); free(bi); free(tmp); free(g);
+}

@@ -107,6 +107,10 @@ int    or_converge_batch(const or_system *systems, int nsys, double *const *rhos
                          double *secs);
 double or_wtime(void);
 
+/* DDFT right-hand side d(eta)/dt of the log-density form, eta = ln rho
+ * (ext/SciMLBasecDFTExt.jl:10-61). */
+void   or_ddft_rhs(const or_system *s, const double *eta, double *deta);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,7 @@ def lib():
         L.or_converge_batch.argtypes = [P, C.c_int, C.POINTER(_dp), C.c_int, C.c_double, C.c_double,
                                         C.c_double, C.c_int, C.c_int, C.POINTER(OrAAResult), _dp]
         L.or_wtime.restype = C.c_double
+        L.or_ddft_rhs.argtypes = [P, _dp, _dp]
         _lib = L
     return _lib
 
@@ -192,6 +193,13 @@ class Oracle:
         chem = sum(float(mu[c]) * simpson(rho[c * self.N:(c + 1) * self.N]) for c in range(self.nc))
         KB = 1.380649e-23
         return F * KB * self.s.T - chem + p * simpson(np.ones(self.N))
+
+    def ddft_rhs(self, eta):
+        """d(eta)/dt of the DDFT log-density form (ext/SciMLBasecDFTExt.jl:10-61)."""
+        eta = _f64(eta).reshape(-1)
+        out = np.zeros(self.N * self.nc)
+        lib().or_ddft_rhs(self.ref, _ptr(eta), _ptr(out))
+        return out
 
     def F_res(self, rho):
         return lib().or_F_res(self.ref, _ptr(_f64(rho).reshape(-1)))
