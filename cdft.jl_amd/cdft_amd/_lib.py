@@ -37,6 +37,9 @@ class SystemDesc(C.Structure):
         ("T", C.c_double), ("p", C.c_double), ("L", C.c_double), ("psi", C.c_double),
         ("m", _dp), ("d", _dp), ("sigma", _dp), ("eps", _dp),
         ("rho_bulk", _dp), ("mu_res", _dp), ("vext", _dp), ("precision", C.c_int),
+        ("assoc_nsite", C.c_int), ("assoc_npair", C.c_int),
+        ("assoc_site_comp", C.POINTER(C.c_int)), ("assoc_site_count", _dp),
+        ("assoc_pair_site", C.POINTER(C.c_int)), ("assoc_pair_eps", _dp), ("assoc_pair_kappa", _dp),
     ]
 
 
@@ -145,7 +148,8 @@ class DeviceSystem:
     """A cdft_sys handle plus the host arrays its descriptor pointed to."""
 
     def __init__(self, ctx, *, nc, ndim, ngrid, lb, ub, T, p, L, psi, m, d, sigma, eps,
-                 rho_bulk, mu_res, vext, precision):
+                 rho_bulk, mu_res, vext, precision, assoc_site_comp=(), assoc_site_count=(),
+                 assoc_pairs=(), assoc_eps=(), assoc_kappa=()):
         self.ctx = ctx
         keep = {k: np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
                 for k, v in dict(m=m, d=d, sigma=sigma, eps=eps, rho_bulk=rho_bulk,
@@ -163,6 +167,19 @@ class DeviceSystem:
             keep["vext"] = np.ascontiguousarray(vext, dtype=np.float64).reshape(-1)
             desc.vext = ptr(keep["vext"])
         desc.precision = int(precision)
+        if len(assoc_pairs):
+            keep["site_comp"] = np.ascontiguousarray(assoc_site_comp, dtype=np.int32)
+            keep["site_count"] = np.ascontiguousarray(assoc_site_count, dtype=np.float64)
+            keep["pair_site"] = np.ascontiguousarray(assoc_pairs, dtype=np.int32).reshape(-1)
+            keep["pair_eps"] = np.ascontiguousarray(assoc_eps, dtype=np.float64)
+            keep["pair_kappa"] = np.ascontiguousarray(assoc_kappa, dtype=np.float64)
+            ip = C.POINTER(C.c_int)
+            desc.assoc_nsite, desc.assoc_npair = len(assoc_site_comp), len(assoc_pairs)
+            desc.assoc_site_comp = keep["site_comp"].ctypes.data_as(ip)
+            desc.assoc_site_count = ptr(keep["site_count"])
+            desc.assoc_pair_site = keep["pair_site"].ctypes.data_as(ip)
+            desc.assoc_pair_eps = ptr(keep["pair_eps"])
+            desc.assoc_pair_kappa = ptr(keep["pair_kappa"])
         h = C.c_void_p()
         check(load().cdft_system_create(ctx.handle, C.byref(desc), C.byref(h)), ctx.handle)
         self.handle = h

@@ -1,4 +1,5 @@
-"""Bulk PC-SAFT equation of state (Gross & Sadowski, Ind. Eng. Chem. Res. 40 (2001) 1244).
+"""Bulk PC-SAFT equation of state (Gross & Sadowski, Ind. Eng. Chem. Res. 40 (2001) 1244),
+with the Wertheim association term of Gross & Sadowski, Ind. Eng. Chem. Res. 41 (2002) 5510.
 
 Host-side replacement for the Clapeyron.jl calls the reference makes during set-up
 and post-processing (Clapeyron is not vendored in the reference):
@@ -35,6 +36,22 @@ DATABASE = {
     "carbon dioxide": (2.0729, 2.7852, 169.21, 44.010),
     "nitrogen": (1.2053, 3.3130, 90.96, 28.014),
     "argon": (0.9285, 3.4784, 122.23, 39.948),
+    "hexane": (3.0576, 3.7983, 236.77, 86.177),
+    # associating components (Gross & Sadowski 2002, Table 1; 2B scheme)
+    "water": (1.0656, 3.0007, 366.51, 18.015),
+    "methanol": (1.5255, 3.2300, 188.90, 32.042),
+    "ethanol": (2.3827, 3.1771, 198.24, 46.069),
+    "1-propanol": (2.9997, 3.2522, 233.40, 60.096),
+}
+
+# Association parameters (Gross & Sadowski 2002): eps_AB/k (K), bonding volume kappa_AB,
+# and the site types with their multiplicities.  The 2B scheme has one electron-donor
+# site "e" and one proton site "H", which bond to each other only.
+ASSOC_DATABASE = {
+    "water": (2500.7, 0.034868, (("e", 1), ("H", 1))),
+    "methanol": (2899.5, 0.035176, (("e", 1), ("H", 1))),
+    "ethanol": (2653.4, 0.032384, (("e", 1), ("H", 1))),
+    "1-propanol": (2276.8, 0.015268, (("e", 1), ("H", 1))),
 }
 
 # Binary interaction parameters k_ij (Gross & Sadowski 2001), used when the caller
@@ -66,14 +83,23 @@ _H = 1e-30  # complex-step size (relative)
 
 
 class PCSAFT:
-    """PC-SAFT model for non-associating components (one bead per component).
+    """PC-SAFT model, one bead per component, with optional Wertheim association.
 
     ``sigma`` and ``epsilon`` are stored as full matrices with the Clapeyron
     combining rules: sigma_ij = (sigma_i + sigma_j)/2 and
     eps_ij = (1 - k_ij) sqrt(eps_i eps_j).
+
+    Association follows Clapeyron's sparse layout, which the reference packs in
+    ``pack_assoc_params`` (src/models/DFT/assoc.jl:18-57): a list of bonding pairs
+    (i, a, j, b) between site a of component i and site b of component j, each with
+    eps_AB (K) and kappa_AB.  Like sites bond as the database lists them (e with H for
+    the 2B scheme).  Unlike pairs between two associating components exist only with
+    ``combining="cr1"``: eps = (eps_i + eps_j)/2 and
+    kappa = sqrt(kappa_i kappa_j) (sqrt(sigma_i sigma_j) / sigma_ij)^3; the default,
+    ``"nocombining"``, adds none (Clapeyron's AssocOptions default).
     """
 
-    def __init__(self, components, kij=None, params=None):
+    def __init__(self, components, kij=None, params=None, assoc=None, combining="nocombining"):
         self.components = list(components)
         n = len(self.components)
         rows = []
@@ -100,6 +126,63 @@ class PCSAFT:
         self.sigma = 0.5 * (s[:, None] + s[None, :])
         self.epsilon = np.sqrt(e[:, None] * e[None, :]) * (1.0 - k)
         self.kij = k
+        if combining not in ("nocombining", "cr1"):
+            raise ValueError("combining must be 'nocombining' or 'cr1'")
+        self.combining = combining
+        self.assoc_like = {}
+        for c in self.components:
+            a = assoc.get(c) if assoc is not None and c in assoc else ASSOC_DATABASE.get(c)
+            if a is not None:
+                self.assoc_like[c] = (float(a[0]), float(a[1]), tuple((str(x), int(y)) for x, y in a[2]))
+        self._build_assoc()
+
+    def _build_assoc(self):
+        """Site list and bonding pairs (the data pack_assoc_params flattens)."""
+        self.sites = []                      # per component: ((name, count), ...)
+        for c in self.components:
+            self.sites.append(self.assoc_like[c][2] if c in self.assoc_like else ())
+        self.site_comp, self.site_count, base = [], [], []
+        for i, st in enumerate(self.sites):
+            base.append(len(self.site_comp))
+            for _, cnt in st:
+                self.site_comp.append(i)
+                self.site_count.append(float(cnt))
+        pairs = []                           # (global site a, global site b, eps_AB, kappa_AB)
+
+        def donor_acceptor(i, j):
+            ia = [k for k, (nm, _) in enumerate(self.sites[i]) if nm == "e"]
+            ih = [k for k, (nm, _) in enumerate(self.sites[i]) if nm == "H"]
+            ja = [k for k, (nm, _) in enumerate(self.sites[j]) if nm == "e"]
+            jh = [k for k, (nm, _) in enumerate(self.sites[j]) if nm == "H"]
+            out = [(base[i] + a, base[j] + b) for a in ia for b in jh]
+            if i != j:
+                out += [(base[i] + a, base[j] + b) for a in ih for b in ja]
+            return out
+
+        names = self.components
+        for i in range(len(names)):
+            if names[i] not in self.assoc_like:
+                continue
+            eps_i, kap_i, _ = self.assoc_like[names[i]]
+            for a, b in donor_acceptor(i, i):
+                pairs.append((a, b, eps_i, kap_i))
+        if self.combining == "cr1":
+            for i in range(len(names)):
+                for j in range(i + 1, len(names)):
+                    if names[i] not in self.assoc_like or names[j] not in self.assoc_like:
+                        continue
+                    eps_i, kap_i, _ = self.assoc_like[names[i]]
+                    eps_j, kap_j, _ = self.assoc_like[names[j]]
+                    si, sj, sij = self.sigma[i, i], self.sigma[j, j], self.sigma[i, j]
+                    eps = 0.5 * (eps_i + eps_j)
+                    kap = math.sqrt(kap_i * kap_j) * (math.sqrt(si * sj) / sij) ** 3
+                    for a, b in donor_acceptor(i, j):
+                        pairs.append((a, b, eps, kap))
+        self.assoc_pairs = pairs
+
+    @property
+    def has_assoc(self):
+        return len(self.assoc_pairs) > 0
 
     def __len__(self):
         return len(self.components)
@@ -130,12 +213,22 @@ class PCSAFT:
         m3 = m2 * (mbar - 2.0) / mbar
         a = _A[:, 0] + m2 * _A[:, 1] + m3 * _A[:, 2]
         b = _B[:, 0] + m2 * _B[:, 1] + m3 * _B[:, 2]
-        return s, mbar, m2es3, m2e2s3, a, b, x, m, d
+        assoc = None
+        if self.has_assoc:
+            sc = np.array(self.site_comp)
+            pa = np.array([p[0] for p in self.assoc_pairs])
+            pb = np.array([p[1] for p in self.assoc_pairs])
+            ic, jc = sc[pa], sc[pb]
+            strength = np.array([self.sigma[i, j] ** 3 * (math.exp(p[2] / T) - 1.0) * p[3]
+                                 for p, i, j in zip(self.assoc_pairs, ic, jc)])
+            dij = d[ic] * d[jc] / (d[ic] + d[jc])
+            assoc = (pa, pb, strength, dij, x[sc] * np.array(self.site_count), np.array(self.site_count))
+        return s, mbar, m2es3, m2e2s3, a, b, x, m, d, assoc
 
     @staticmethod
     def _ares_rho(rho, c):
         """a_res per molecule over kT at number density rho (array- and complex-safe)."""
-        s, mbar, m2es3, m2e2s3, a, b, x, m, d = c
+        s, mbar, m2es3, m2e2s3, a, b, x, m, d, assoc = c
         z0, z1, z2, z3 = (s[k] * rho for k in range(4))
         ahs = (3.0 * z1 * z2 / (1.0 - z3) + z2 ** 3 / (z3 * (1.0 - z3) ** 2)
                + (z2 ** 3 / z3 ** 2 - z0) * np.log(1.0 - z3)) / z0
@@ -156,7 +249,9 @@ class PCSAFT:
             I2 = I2 + b[q] * pw
             pw = pw * eta
         adisp = -2.0 * math.pi * rho * I1 * m2es3 - math.pi * rho * mbar * C1 * I2 * m2e2s3
-        return ahc + adisp
+        if assoc is None:
+            return ahc + adisp
+        return ahc + adisp + _a_assoc(rho, z2, z3, assoc)
 
     def a_res(self, V, T, z):
         """Residual Helmholtz energy per mole over RT; V (m^3), z (mol).  Complex-safe."""
@@ -261,6 +356,10 @@ class PCSAFT:
         y[np.argmin(self.segment * np.diag(self.epsilon))] = 1.0
         y /= y.sum()
         p = 1e6 if p0 is None else p0
+        if p0 is None:
+            # lower the start until the trial vapour has a vapour-like root (Z > 0.3)
+            while p > 1.0 and p * self.volume(p, T, y, "vapour") / (RGAS * T) < 0.3:
+                p *= 0.1
         S, y, vl, vv = inner(p, y)
         lp_prev, g_prev = math.log(p), S - 1.0
         p = p * S
@@ -278,6 +377,52 @@ class PCSAFT:
             lp_prev, g_prev = lp, g
             p = math.exp(lp_new)
         return p, vl, vv, y
+
+    def tp_flash(self, p, T, z, equilibrium="lle"):
+        """Two-phase flash at (p, T, z) by successive substitution on K = phi1/phi2 with
+        the Rachford-Rice split (Clapeyron's tp_flash with RRTPFlash).  For
+        ``equilibrium="lle"`` both phases take their liquid root and the start splits
+        the feed into phases rich in its two most dissimilar components.
+        Returns (x, beta, g): x[k] is phase k's composition, beta the fraction in phase 2."""
+        z = np.asarray(z, dtype=np.float64)
+        z = z / z.sum()
+        nc = len(z)
+        phase = "liquid" if equilibrium == "lle" else "unknown"
+        x1 = np.full(nc, 1e-3)
+        x2 = np.full(nc, 1e-3)
+        e = self.segment * np.diag(self.epsilon)
+        if equilibrium == "lle" and self.has_assoc:
+            # the associating component forms one phase, the most apolar one the other
+            a = next(i for i, c in enumerate(self.components) if c in self.assoc_like)
+            b = int(np.argmax(np.where(np.arange(nc) == a, -np.inf, e)))
+        else:
+            a, b = int(np.argmax(e)), int(np.argmin(e))
+        x1[a] = 1.0
+        x2[b] = 1.0
+        x1 /= x1.sum()
+        x2 /= x2.sum()
+        beta = 0.5
+        for _ in range(2000):
+            v1 = self.volume(p, T, x1, phase)
+            v2 = self.volume(p, T, x2, phase if equilibrium == "lle" else "vapour")
+            K = np.exp(self.lnphi(v1, T, x1) - self.lnphi(v2, T, x2))
+
+            def rr(b_):
+                return np.sum(z * (K - 1.0) / (1.0 + b_ * (K - 1.0)))
+            lo = 1.0 / (1.0 - K.max()) + 1e-15
+            hi = 1.0 / (1.0 - K.min()) - 1e-15
+            lo, hi = max(lo, 0.0), min(hi, 1.0)
+            beta = brentq(rr, lo, hi, xtol=1e-16, rtol=1e-15) if rr(lo) * rr(hi) < 0 else beta
+            n1 = z / (1.0 + beta * (K - 1.0))
+            n2 = K * n1
+            n1 /= n1.sum()
+            n2 /= n2.sum()
+            done = np.max(np.abs(np.log(n1 / x1))) < 1e-13 and np.max(np.abs(np.log(n2 / x2))) < 1e-13
+            x1, x2 = n1, n2
+            if done:
+                break
+        g = (1.0 - beta) * self.gibbs_res(v1, p, T, x1) + beta * self.gibbs_res(v2, p, T, x2)
+        return np.vstack([x1, x2]), beta, g
 
     def saturation_pressure(self, T, p0=None):
         """Pure-component vapour pressure -> (p, v_l, v_v)."""
@@ -341,11 +486,51 @@ class PCSAFT:
         return T, p_of(rho, T), 1.0 / rho
 
     def split_pure(self):
+        """One pure model per component (Clapeyron.split_pure_model), with its own
+        association sites."""
         out = []
         for i, c in enumerate(self.components):
-            mdl = PCSAFT([c])
-            mdl.segment = self.segment[i:i + 1].copy()
-            mdl.sigma = self.sigma[i:i + 1, i:i + 1].copy()
-            mdl.epsilon = np.array([[math.sqrt(self.epsilon[i, i] ** 2)]])
-            out.append(mdl)
+            row = (self.segment[i], self.sigma[i, i] * 1e10, self.epsilon[i, i], self.Mw[i])
+            assoc = {c: self.assoc_like[c]} if c in self.assoc_like else {c: None}
+            out.append(PCSAFT([c], params={c: row}, assoc=assoc))
         return out
+
+
+def _a_assoc(rho, z2, z3, assoc):
+    """Wertheim association Helmholtz energy per molecule over kT at number density rho
+    (array- and complex-safe), sum_i x_i sum_a n_ia (ln X_ia - X_ia/2 + 1/2), with the
+    site fractions X from the mass-action law
+    X_s = 1 / (1 + sum_t rho x_c(t) n_t Delta_st X_t) and
+    Delta_p = g_ij sigma_ij^3 (exp(eps_AB/T) - 1) kappa_AB,
+    g_ij = 1/(1-z3) + d_ij 3 z2/(1-z3)^2 + d_ij^2 2 z2^2/(1-z3)^3, d_ij = d_i d_j/(d_i + d_j).
+    X is solved by successive substitution then Newton in the caller's number type, so a
+    complex-step perturbation carries the exact derivative of X through the solve."""
+    pa, pb, strength, dij, xn, cnt = assoc
+    rho = np.asarray(rho)
+    ns = len(xn)
+    i1 = 1.0 / (1.0 - z3)
+    shape = np.shape(rho)
+    M = np.zeros(shape + (ns, ns), dtype=np.result_type(rho, z2, complex if np.iscomplexobj(rho) else float))
+    for p in range(len(pa)):
+        g = i1 + dij[p] * 3.0 * z2 * i1 ** 2 + dij[p] ** 2 * 2.0 * z2 ** 2 * i1 ** 3
+        delta = g * strength[p]
+        M[..., pa[p], pb[p]] += delta
+        M[..., pb[p], pa[p]] += delta
+    K = M * (np.asarray(rho)[..., None, None] * xn[None, :])       # K[s, t] = Delta_st rho x_c(t) n_t
+    X = np.full(shape + (ns,), 0.9, dtype=M.dtype)
+    for _ in range(10):
+        X = 0.5 * (X + 1.0 / (1.0 + np.einsum("...st,...t->...s", K, X)))
+    eye = np.eye(ns)
+    for _ in range(60):
+        C_ = np.einsum("...st,...t->...s", K, X)
+        F = X * (1.0 + C_) - 1.0
+        J = eye * (1.0 + C_)[..., None] + X[..., :, None] * K
+        dX = np.linalg.solve(J, F[..., None])[..., 0]
+        Xn = X - dX
+        ok = (Xn.real > 0) & (Xn.real < 1)
+        X = np.where(ok, Xn, 1.0 / (1.0 + C_))
+        if np.all(ok) and np.all(np.abs(dX.real) <= 1e-15 * np.abs(X.real)) and \
+                np.all(np.abs(dX.imag) <= 1e-15 * np.abs(X.imag) + 1e-300):
+            break
+    xs = xn / cnt                                                   # x of each site's component
+    return np.sum(xs * cnt * (np.log(X) - 0.5 * X + 0.5), axis=-1)

@@ -77,8 +77,8 @@ MATRIX_3D_SOLVER = dict(anderson_start=0.5)
 
 
 def matrix_3d(options=None, n=256, box=30.0, nspheres=64, seed=0, T=200.0, p=3e6, x=(0.6, 0.3, 0.1),
-              components=("methane", "ethane", "propane"), rmin=1.0, rmax=2.5):
-    model = PCSAFT(list(components))
+              components=("methane", "ethane", "propane"), rmin=1.0, rmax=2.5, combining="nocombining"):
+    model = PCSAFT(list(components), combining=combining)
     x = np.asarray(x, dtype=np.float64)
     v = model.volume(p, T, x, phase="liquid")
     L = model.length_scale()

@@ -2,6 +2,8 @@
 
   surface_tension(model, T, x)        src/methods/surface_tension.jl:13-29
   surface_tension(system, rho)        src/methods/surface_tension.jl:31-53
+  interfacial_tension(model, p, T, x, xx) / (system, rho)
+                                      src/methods/interfacial_tension.jl:17-63
   adsorption(model, surface, p, T, n) src/methods/adsorption.jl:48-66
   adsorption(system, rho)             src/methods/adsorption.jl:22-32
 """
@@ -44,6 +46,26 @@ def surface_tension(model_or_system, T_or_rho, x=None, *, ngrid=201, box=10.0, o
     rho1 = x / vl
     rho2 = y / vv
     structure = TwoPhase1DCart((p, T), rho1, rho2, [-box * L, box * L], (ngrid,))
+    system = DFTSystem(model, structure, None, options)
+    rho = initialize_profiles(system)
+    converge_(system, rho, **kw)
+    return surface_tension_system(system, rho) / 2
+
+
+def interfacial_tension(model_or_system, p_or_rho, T=None, x=None, xx=None, *, ngrid=201, box=20.0, options=None,
+                        **kw):
+    """interfacial_tension(model, p, T, x, xx) -> gamma (N/m) of the liquid-liquid
+    interface between phases of compositions x and xx at (p, T), or
+    interfacial_tension(system, rho), which is surface_tension(system, rho)."""
+    if isinstance(model_or_system, DFTSystem):
+        return surface_tension_system(model_or_system, p_or_rho)
+    model, p = model_or_system, float(p_or_rho)
+    x = np.asarray(x, dtype=np.float64)
+    xx = np.asarray(xx, dtype=np.float64)
+    L = model.length_scale()
+    v1 = model.volume(p, T, x)
+    v2 = model.volume(p, T, xx)
+    structure = TwoPhase1DCart((p, T), x / v1, xx / v2, [-box * L, box * L], (ngrid,))
     system = DFTSystem(model, structure, None, options)
     rho = initialize_profiles(system)
     converge_(system, rho, **kw)

@@ -92,7 +92,7 @@ class DFTSystem:
                 ub=[b[1] for b in st.bounds], T=T, p=p, L=self.L,
                 psi=PSI, m=self.model.segment, d=self.species.size, sigma=self.model.sigma,
                 eps=self.model.epsilon, rho_bulk=self.species.bulk_density, mu_res=self.species.chempot_res,
-                vext=self.vext, precision=self.options.precision_code)
+                vext=self.vext, precision=self.options.precision_code, **self.assoc_descriptor())
         return self._dev
 
     def descriptor(self):
@@ -103,7 +103,18 @@ class DFTSystem:
                     ub=[b[1] for b in st.bounds], T=T, L=self.L,
                     m=self.model.segment.copy(), d=self.species.size.copy(), sigma=self.model.sigma.copy(),
                     eps=self.model.epsilon.copy(), mu_res=self.species.chempot_res.copy(),
-                    rho_bulk=self.species.bulk_density.copy(), vext=None if self.vext is None else self.vext.copy())
+                    rho_bulk=self.species.bulk_density.copy(), vext=None if self.vext is None else self.vext.copy(),
+                    **self.assoc_descriptor())
+
+    def assoc_descriptor(self):
+        """Association sites and bonding pairs (pack_assoc_params, assoc.jl:18-57);
+        empty for a non-associating model."""
+        mdl = self.model
+        pairs = getattr(mdl, "assoc_pairs", [])
+        return dict(assoc_site_comp=list(getattr(mdl, "site_comp", [])),
+                    assoc_site_count=list(getattr(mdl, "site_count", [])),
+                    assoc_pairs=[(a, b) for a, b, _, _ in pairs],
+                    assoc_eps=[e for _, _, e, _ in pairs], assoc_kappa=[k for _, _, _, k in pairs])
 
 
 def _cos_prof(x, start, stop, shift, coef):

@@ -334,11 +334,21 @@ void launch_transposed3d(cdft_sys *s, const T *dphi, double *dtot)
     }
 }
 
+template <int NC, int AS>
+void launch_phi32_as(cdft_sys *s)
+{
+    hipLaunchKernelGGL((k_phi32<NC, 3, AS>), dim3(nblocks(s->N)), dim3(256), 0, s->ctx->stream, s->rho, s->n32,
+                       s->dphi32, s->N, s->NA, s->pc);
+}
+
 template <int NC>
 void launch_phi32(cdft_sys *s)
 {
-    hipLaunchKernelGGL((k_phi32<NC, 3>), dim3(nblocks(s->N)), dim3(256), 0, s->ctx->stream, s->rho, s->n32,
-                       s->dphi32, s->N, s->NA, s->pc);
+    switch (s->assoc) {
+    case 0: launch_phi32_as<NC, 0>(s); break;
+    case 1: launch_phi32_as<NC, 1>(s); break;
+    default: launch_phi32_as<NC, 2>(s); break;
+    }
 }
 
 // The FP32 map of a mixed-precision 3D system: weighted densities, Phi's gradient
@@ -356,15 +366,26 @@ void launch_functional32(cdft_sys *s)
     launch_transposed3d<float>(s, s->dphi32, s->part);
 }
 
+template <int NC, int AS>
+void launch_phi_as(cdft_sys *s, hipStream_t st)
+{
+    if (s->nd == 3)
+        hipLaunchKernelGGL((k_phi<NC, 3, AS>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, s->n, s->dphi, s->fval,
+                           s->N, s->NA, s->pc);
+    else
+        hipLaunchKernelGGL((k_phi<NC, 1, AS>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, s->n, s->dphi, s->fval,
+                           s->N, s->NA, s->pc);
+}
+
+// s->assoc selects the association variant: 0 none, 1 one pair (closed form), 2 general.
 template <int NC>
 void launch_phi(cdft_sys *s, hipStream_t st)
 {
-    if (s->nd == 3)
-        hipLaunchKernelGGL((k_phi<NC, 3>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, s->n, s->dphi, s->fval,
-                           s->N, s->NA, s->pc);
-    else
-        hipLaunchKernelGGL((k_phi<NC, 1>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, s->n, s->dphi, s->fval,
-                           s->N, s->NA, s->pc);
+    switch (s->assoc) {
+    case 0: launch_phi_as<NC, 0>(s, st); break;
+    case 1: launch_phi_as<NC, 1>(s, st); break;
+    default: launch_phi_as<NC, 2>(s, st); break;
+    }
 }
 
 void launch_phi_any(cdft_sys *s)
@@ -411,15 +432,25 @@ void launch_functional(cdft_sys *s)
     launch_transposed(s);
 }
 
+template <int NC, int AS>
+void launch_phi_hvp_as(cdft_sys *s, const double *v, hipStream_t st)
+{
+    if (s->nd == 3)
+        hipLaunchKernelGGL((k_phi_hvp<NC, 3, AS>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, v, s->n, s->dphi,
+                           s->N, s->NA, s->pc);
+    else
+        hipLaunchKernelGGL((k_phi_hvp<NC, 1, AS>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, v, s->n, s->dphi,
+                           s->N, s->NA, s->pc);
+}
+
 template <int NC>
 void launch_phi_hvp(cdft_sys *s, const double *v, hipStream_t st)
 {
-    if (s->nd == 3)
-        hipLaunchKernelGGL((k_phi_hvp<NC, 3>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, v, s->n, s->dphi,
-                           s->N, s->NA, s->pc);
-    else
-        hipLaunchKernelGGL((k_phi_hvp<NC, 1>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, v, s->n, s->dphi,
-                           s->N, s->NA, s->pc);
+    switch (s->assoc) {
+    case 0: launch_phi_hvp_as<NC, 0>(s, v, st); break;
+    case 1: launch_phi_hvp_as<NC, 1>(s, v, st); break;
+    default: launch_phi_hvp_as<NC, 2>(s, v, st); break;
+    }
 }
 
 // Hessian-vector product of F_res at s->rho (weighted densities already in s->n)
@@ -527,6 +558,20 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
     if (!(d->T > 0) || !(d->L > 0)) return fail(ctx, CDFT_ERR_ARG, "T and L must be positive");
     if (!d->m || !d->d || !d->sigma || !d->eps || !d->rho_bulk || !d->mu_res)
         return fail(ctx, CDFT_ERR_ARG, "parameter array is NULL");
+    if (d->assoc_npair < 0 || d->assoc_nsite < 0) return fail(ctx, CDFT_ERR_ARG, "negative association counts");
+    if (d->assoc_npair > 0) {
+        if (d->assoc_nsite > CDFT_MAXSITE || d->assoc_npair > CDFT_MAXPAIR)
+            return fail(ctx, CDFT_ERR_UNSUPPORTED, "association: at most 4 sites and 8 pairs on the device");
+        if (!d->assoc_site_comp || !d->assoc_site_count || !d->assoc_pair_site || !d->assoc_pair_eps ||
+            !d->assoc_pair_kappa)
+            return fail(ctx, CDFT_ERR_ARG, "association array is NULL");
+        for (int t = 0; t < d->assoc_nsite; t++)
+            if (d->assoc_site_comp[t] < 0 || d->assoc_site_comp[t] >= d->nc || !(d->assoc_site_count[t] > 0))
+                return fail(ctx, CDFT_ERR_ARG, "association site with a bad component or count");
+        for (int q = 0; q < 2 * d->assoc_npair; q++)
+            if (d->assoc_pair_site[q] < 0 || d->assoc_pair_site[q] >= d->assoc_nsite)
+                return fail(ctx, CDFT_ERR_ARG, "association pair names a site out of range");
+    }
     if (d->precision != CDFT_FP64 && d->precision != CDFT_MIXED)
         return fail(ctx, CDFT_ERR_ARG, "unknown precision mode");
     for (int i = 0; i < d->nc; i++)
@@ -572,6 +617,26 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
             P.cm[i * nc + j] = s->m[i] * s->m[j] * sr * sr * sr;
             P.eT[i * nc + j] = s->eps[i * nc + j] / s->T;
         }
+    if (d->assoc_npair > 0) {
+        // pack_assoc_params (assoc.jl:18-57) in reduced units (PCSAFT.jl:244-246)
+        P.nsite = d->assoc_nsite;
+        P.npair = d->assoc_npair;
+        for (int t = 0; t < P.nsite; t++) {
+            P.site_comp[t] = d->assoc_site_comp[t];
+            P.site_count[t] = d->assoc_site_count[t];
+        }
+        for (int q = 0; q < P.npair; q++) {
+            const int a = d->assoc_pair_site[2 * q], b = d->assoc_pair_site[2 * q + 1];
+            const int ic = P.site_comp[a], jc = P.site_comp[b];
+            const double sr = s->sigma[ic * nc + jc] / s->L;
+            P.pair_a[q] = a;
+            P.pair_b[q] = b;
+            P.pair_k[q] = sr * sr * sr * (std::exp(d->assoc_pair_eps[q] / s->T) - 1.0) * d->assoc_pair_kappa[q];
+            P.pair_dij[q] = P.dd[ic] * P.dd[jc] / (P.dd[ic] + P.dd[jc]);
+        }
+        // one pair between two distinct sites has a closed form (assoc.jl:702-769)
+        s->assoc = (P.npair == 1 && P.pair_a[0] != P.pair_b[0]) ? 1 : 2;
+    }
 
     int N = s->N, F = s->F;
     long n = (long)N * nc;
@@ -677,7 +742,7 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
     }
     // a system the batched kernel can take gets its ring spectra now, so that the
     // allocation-free batched entry point finds it ready
-    if (!d3 && s->nc == 1 && s->N <= kRegNPT * BT && s->N >= 5) {
+    if (!d3 && s->nc == 1 && !s->assoc && s->N <= kRegNPT * BT && s->N >= 5) {
         int rc = batched_prepare(ctx, s);
         if (rc == CDFT_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = CDFT_ERR_HIP;
         if (rc != CDFT_OK) {
@@ -1257,6 +1322,9 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
             return fail(ctx, CDFT_ERR_UNSUPPORTED,
                         "batched path: at most 512 grid points and anderson m <= 5 for now; use cdft_converge");
         if (sys[i]->nd != 1) return fail(ctx, CDFT_ERR_UNSUPPORTED, "the batched path takes 1D systems");
+        if (sys[i]->assoc)
+            return fail(ctx, CDFT_ERR_UNSUPPORTED,
+                        "batched path: associating systems are solved per system; use cdft_converge");
         // prepared at creation (spectra on the ring, the ring's twiddles); the
         // allocating entry points prepare a system here if needed
         if (noalloc && (!sys[i]->H64 || !ctx->twiddles.count(sys[i]->M)))

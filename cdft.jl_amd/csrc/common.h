@@ -60,6 +60,7 @@ struct cdft_sys {
     double cp[CDFT_MAXC];      // mu_res + ln rho_bulk (converge.jl:121)
     int precision = CDFT_FP64;
     bool has_vext = false;
+    int assoc = 0;             // association variant of the pointwise kernels: 0 none, 1 one pair, 2 general
     PhiConst pc;
 
     // device buffers (FP64)

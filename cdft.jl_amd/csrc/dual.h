@@ -2,8 +2,9 @@
 //
 // phi_grad<dual, dual, NC, ND> evaluated at n + eps u returns, in the eps parts of
 // its gradient, the Hessian-vector product H_Phi(n) u: the second-derivative
-// kernel behind the adjoint of the fixed point.  Only the operations phi_grad and
-// i_lite use are provided; comparisons and selections act on the value part.
+// kernel behind the adjoint of the fixed point.  Only the operations phi_grad,
+// assoc_grad and i_lite use are provided; comparisons and selections act on the
+// value part.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -14,6 +15,7 @@ using ::copysign;
 using ::fabs;
 using ::fmax;
 using ::log;
+using ::sqrt;
 
 struct dual {
     double v, d;
@@ -42,6 +44,11 @@ CDFT_DI bool operator<=(dual a, dual b) { return a.v <= b.v; }
 CDFT_DI bool operator>=(dual a, dual b) { return a.v >= b.v; }
 CDFT_DI bool operator==(dual a, dual b) { return a.v == b.v; }
 CDFT_DI dual log(dual a) { return {::log(a.v), a.d / a.v}; }
+CDFT_DI dual sqrt(dual a)
+{
+    const double r = ::sqrt(a.v);
+    return {r, 0.5 * a.d / r};
+}
 CDFT_DI dual fabs(dual a) { return a.v < 0.0 ? -a : a; }
 CDFT_DI dual copysign(dual a, dual b) { return ((a.v < 0.0) != (b.v < 0.0)) ? -a : a; }
 CDFT_DI dual fmax(dual a, dual b) { return a.v >= b.v ? a : b; }

@@ -1,10 +1,11 @@
 // phi.h -- pointwise PC-SAFT Helmholtz density Phi(n) and its gradient dPhi/dn
 // at one grid point, for the HIP kernels.
 //
-// Restates f_res = f_hs + f_hc + f_disp (+ association = 0):
-//   f_hs   src/models/DFT/FMT.jl:17-47 (White-Bear n3 dependence)
-//   f_hc   src/models/DFT/SAFT/PCSAFT/PCSAFT.jl:123-144
-//   f_disp src/models/DFT/SAFT/PCSAFT/PCSAFT.jl:150-211
+// Restates f_res = f_hs + f_hc + f_disp + f_assoc (PCSAFT.jl:110-117):
+//   f_hs    src/models/DFT/FMT.jl:17-47 (White-Bear n3 dependence)
+//   f_hc    src/models/DFT/SAFT/PCSAFT/PCSAFT.jl:123-144
+//   f_disp  src/models/DFT/SAFT/PCSAFT/PCSAFT.jl:150-211
+//   f_assoc src/models/DFT/assoc.jl:702-831 (Wertheim association)
 // The gradient is hand-derived (the reference differentiates f_res with Enzyme,
 // src/models/models.jl:355-411); it is what the "backward kernel" computes.
 //
@@ -24,6 +25,8 @@
 #ifndef CDFT_MAXC
 #define CDFT_MAXC 8
 #endif
+#define CDFT_MAXSITE 4      // association sites on the device path
+#define CDFT_MAXPAIR 8      // association bonding pairs on the device path
 
 // Per-system constants in reduced units (lengths / L), set up on the host.
 struct PhiConst {
@@ -34,6 +37,13 @@ struct PhiConst {
     double dd[CDFT_MAXC];                 // d_i / L
     double cm[CDFT_MAXC * CDFT_MAXC];     // m_i m_j (sigma_ij / L)^3
     double eT[CDFT_MAXC * CDFT_MAXC];     // eps_ij / T
+    // Wertheim association (pack_assoc_params, assoc.jl:18-57); npair = 0: none
+    int nsite, npair;
+    int site_comp[CDFT_MAXSITE];          // component of each site
+    double site_count[CDFT_MAXSITE];      // multiplicity of each site type
+    int pair_a[CDFT_MAXPAIR], pair_b[CDFT_MAXPAIR];   // the two sites of each pair
+    double pair_k[CDFT_MAXPAIR];          // (sigma_ab / L)^3 (exp(eps_AB / T) - 1) kappa_AB
+    double pair_dij[CDFT_MAXPAIR];        // d_i d_j / (d_i + d_j) / L
 };
 
 namespace cdft {
@@ -88,10 +98,185 @@ __device__ __forceinline__ void i_lite(int which, T mb, T eta, T &I, T &Im, T &I
     Ie = ve;
 }
 
+// x[idx] for a runtime index into a register array, as a chain of selects (keeps the
+// array in registers).
+template <typename T, int K>
+__device__ __forceinline__ T pick(const T (&x)[K], int idx)
+{
+    T r = x[0];
+#pragma unroll
+    for (int k = 1; k < K; k++) r = (idx == k) ? x[k] : r;
+    return r;
+}
+
+// Association term (f_assoc, assoc.jl:702-831) at one point; adds its gradient to g
+// and returns Phi_assoc.  n2m, n3m, nv2m are the mixture densities of the hard-sphere
+// term (the same sums, assoc.jl:713-733).  Per component: w_i = n0_i xi_i with
+// n0_i = S_i / d_i, xi_i = 1 - |nv2_i|^2 / n2_i^2, n2_i = pi S_i d_i, nv2_i = -2 pi v_i.
+// Per pair: Delta_p = g_hs(d_ab; n2m, n3m, xi_m) (sigma_ab^3 (exp(eps/T) - 1) kappa)
+// (assoc.jl:159-168).  Site fractions solve X_s (1 + C_s) = 1 with
+// C_s = sum_t K_st X_t, K_st = sum of Delta_p v_t over the pairs joining s and t,
+// v_t = n_t w_c(t).  AS = 1: one pair of two distinct sites, in closed form
+// (assoc.jl:759-764); AS = 2: up to CDFT_MAXSITE sites, X0 = 0.9, 5 NS blended
+// substitutions, then 5 NS safeguarded Newton steps (assoc.jl:238-411).
+// Gradient: at the solution dPhi/dw_i = sum_{s in i} n_s ln X_s and
+// dPhi/dDelta_p = -v_a X_a v_b X_b (stationarity of the mass-action Q function; the
+// reference gets the same derivative from an implicit-function rule, assoc.jl:833-916).
+template <typename T, int NC, int ND, int AS>
+__device__ __forceinline__ T assoc_grad(const PhiConst &P, const T (&nl)[NC][6 + ND], T (&g)[NC][6 + ND], T n2m,
+                                        T n3m, const T (&nv2m)[ND])
+{
+    constexpr int NS = CDFT_MAXSITE;
+    const T pi = T(kPi), twopi = T(2.0 * kPi);
+    T w[NC], dwS[NC], dwv[NC][ND];
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        const T di = T(P.dd[i]), S = nl[i][1];
+        const T n0 = S / di, n2 = pi * S * di, in2 = T(1) / (n2 * n2);
+        T sq = 0;
+#pragma unroll
+        for (int j = 0; j < ND; j++) sq += (twopi * nl[i][3 + j]) * (twopi * nl[i][3 + j]);
+        T r = sq * in2;
+        if constexpr (std::is_same<T, float>::value) r = fmin(r, 1.0f);   // FP32 underflow in walls
+        const T xi = T(1) - r;
+        w[i] = n0 * xi;
+        dwS[i] = xi / di + n0 * T(2) * r / n2 * pi * di;
+#pragma unroll
+        for (int j = 0; j < ND; j++) dwv[i][j] = -n0 * T(2) * twopi * nl[i][3 + j] * in2 * twopi;
+    }
+    T nv2sq = 0;
+#pragma unroll
+    for (int j = 0; j < ND; j++) nv2sq += nv2m[j] * nv2m[j];
+    const T xim = T(1) - nv2sq / (n2m * n2m);
+    const T a1 = T(1) / (T(1) - n3m);
+    constexpr int NP = AS == 1 ? 1 : CDFT_MAXPAIR;
+    T delta[NP];
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        const T dd = T(P.pair_dij[p]);
+        const T gh = a1 + T(0.5) * dd * xim * n2m * a1 * a1 + dd * dd * n2m * n2m * xim * a1 * a1 * a1 / T(18);
+        delta[p] = (p < P.npair) ? gh * T(P.pair_k[p]) : T(0);
+    }
+    T v[NS], X[NS];
+#pragma unroll
+    for (int t = 0; t < NS; t++) v[t] = (t < P.nsite) ? T(P.site_count[t]) * pick(w, P.site_comp[t]) : T(0);
+
+    if constexpr (AS == 1) {
+        const int a = P.pair_a[0], b = P.pair_b[0];
+        const T va = pick(v, a), vb = pick(v, b);
+        const T kia = va * delta[0], kjb = vb * delta[0];
+        const T bb = T(1) - kia + kjb;
+        const T Xa = T(2) / (bb + sqrt(bb * bb + T(4) * kia));
+        const T Xb = T(1) / (T(1) + kia * Xa);
+#pragma unroll
+        for (int t = 0; t < NS; t++) X[t] = (t == a) ? Xa : ((t == b) ? Xb : T(1));
+    } else {
+        T K[NS][NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int t = 0; t < NS; t++) {
+                T k = 0;
+#pragma unroll
+                for (int p = 0; p < NP; p++) {
+                    const bool st = (P.pair_a[p] == s && P.pair_b[p] == t), ts = (P.pair_a[p] == t && P.pair_b[p] == s);
+                    k += (st ? delta[p] : T(0)) + (ts ? delta[p] : T(0));
+                }
+                K[s][t] = k * v[t];
+            }
+#pragma unroll
+        for (int t = 0; t < NS; t++) X[t] = T(0.9);
+        const int nit = 5 * P.nsite;
+        for (int it = 0; it < nit; it++) {
+            T Xn[NS];
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                T c = 0;
+#pragma unroll
+                for (int t = 0; t < NS; t++) c += K[s][t] * X[t];
+                Xn[s] = T(0.5) * (X[s] + T(1) / (T(1) + c));
+            }
+#pragma unroll
+            for (int s = 0; s < NS; s++) X[s] = Xn[s];
+        }
+        for (int it = 0; it < nit; it++) {
+            T C[NS], J[NS][NS + 1];
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                T c = 0;
+#pragma unroll
+                for (int t = 0; t < NS; t++) c += K[s][t] * X[t];
+                C[s] = c;
+#pragma unroll
+                for (int t = 0; t < NS; t++) J[s][t] = (s == t ? T(1) + c : T(0)) + X[s] * K[s][t];
+                J[s][NS] = X[s] * (T(1) + c) - T(1);
+            }
+#pragma unroll
+            for (int k = 0; k < NS; k++)
+#pragma unroll
+                for (int i = k + 1; i < NS; i++) {
+                    const T f = J[i][k] / J[k][k];
+#pragma unroll
+                    for (int j = k + 1; j <= NS; j++) J[i][j] -= f * J[k][j];
+                }
+            T dX[NS];
+#pragma unroll
+            for (int k = NS - 1; k >= 0; k--) {
+                T r = J[k][NS];
+#pragma unroll
+                for (int j = k + 1; j < NS; j++) r -= J[k][j] * dX[j];
+                dX[k] = r / J[k][k];
+            }
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                const T cand = X[s] - dX[s];
+                X[s] = (cand > T(0) && cand < T(1)) ? cand : T(1) / (T(1) + C[s]);
+            }
+        }
+    }
+
+    T phi = 0, lx[NS];
+#pragma unroll
+    for (int t = 0; t < NS; t++) {
+        lx[t] = log(fabs(X[t]));
+        phi += v[t] * (lx[t] - T(0.5) * X[t] + T(0.5));
+    }
+    T dw[NC];
+#pragma unroll
+    for (int i = 0; i < NC; i++) dw[i] = 0;
+#pragma unroll
+    for (int t = 0; t < NS; t++)
+#pragma unroll
+        for (int i = 0; i < NC; i++)
+            dw[i] += (t < P.nsite && P.site_comp[t] == i) ? T(P.site_count[t]) * lx[t] : T(0);
+    T dn3 = 0, dn2 = 0, dxi = 0;
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        const int a = P.pair_a[p], b = P.pair_b[p];
+        const T vx = pick(v, a) * pick(X, a) * pick(v, b) * pick(X, b);
+        const T dg = (p < P.npair) ? -vx * T(P.pair_k[p]) : T(0);
+        const T dd = T(P.pair_dij[p]);
+        dn3 += dg * (a1 * a1 + dd * xim * n2m * a1 * a1 * a1 + dd * dd * n2m * n2m * xim * a1 * a1 * a1 * a1 / T(6));
+        dn2 += dg * (T(0.5) * dd * xim * a1 * a1 + dd * dd * n2m * xim * a1 * a1 * a1 / T(9));
+        dxi += dg * (T(0.5) * dd * n2m * a1 * a1 + dd * dd * n2m * n2m * a1 * a1 * a1 / T(18));
+    }
+    dn2 += dxi * T(2) * nv2sq / (n2m * n2m * n2m);
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        const T mi = T(P.m[i]), di = T(P.dd[i]);
+        g[i][1] += dn2 * pi * mi * di + dw[i] * dwS[i];
+        g[i][2] += dn3 * mi;
+#pragma unroll
+        for (int j = 0; j < ND; j++) g[i][3 + j] += dxi * (-T(2) * nv2m[j] / (n2m * n2m)) * (-twopi * mi) + dw[i] * dwv[i][j];
+    }
+    return phi;
+}
+
 // Phi at one point; g receives dPhi/dn for every (component, slot).  CHAIN = false
 // is for callers that know every segment number is 1 (no chain term): the body is
 // then free of branches, so two inlined calls can be scheduled side by side.
-template <typename T, typename TS, int NC, int ND, bool CHAIN = true>
+// AS selects the association term: 0 none, 1 one pair in closed form, 2 general.
+template <typename T, typename TS, int NC, int ND, bool CHAIN = true, int AS = 0>
 __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + ND], T (&g)[NC][6 + ND])
 {
     constexpr int F = 6 + ND;
@@ -163,6 +348,8 @@ __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + N
             }
         }
     }
+
+    if constexpr (AS > 0) phi += assoc_grad<T, NC, ND, AS>(P, nl, g, n2, T(n3), nv2);
 
     // ---- hard chain ----
     if (CHAIN) {

@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void k_circ1d(const T *__restrict__ src, long 
 }
 
 // Phi and dPhi/dn at every grid point.
-template <int NC, int ND>
+template <int NC, int ND, int AS = 0>
 __global__ __launch_bounds__(256) void k_phi(const double *__restrict__ rho, const double *__restrict__ n,
                                              double *__restrict__ dphi, double *__restrict__ fval, int N,
                                              double NA, PhiConst P)
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void k_phi(const double *__restrict__ rho, con
 #pragma unroll
         for (int f = 1; f < F; f++) nl[c][f] = n[(size_t)(c * F + f) * N + i];
     }
-    double v = phi_grad<double, double, NC, ND>(P, nl, g);
+    double v = phi_grad<double, double, NC, ND, true, AS>(P, nl, g);
 #pragma unroll
     for (int c = 0; c < NC; c++)
 #pragma unroll
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void k_phi(const double *__restrict__ rho, con
 
 // FP32 map (mixed precision): Phi's gradient from FP32 weighted densities, with the
 // n3-singular hard-sphere factors in FP64 (phi_grad<float, double>).
-template <int NC, int ND>
+template <int NC, int ND, int AS = 0>
 __global__ __launch_bounds__(256) void k_phi32(const double *__restrict__ rho, const float *__restrict__ n,
                                                float *__restrict__ dphi, int N, double NA, PhiConst P)
 {
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(256) void k_phi32(const double *__restrict__ rho, c
 #pragma unroll
         for (int f = 1; f < F; f++) nl[c][f] = n[(size_t)(c * F + f) * N + i];
     }
-    phi_grad<float, double, NC, ND>(P, nl, g);
+    phi_grad<float, double, NC, ND, true, AS>(P, nl, g);
 #pragma unroll
     for (int c = 0; c < NC; c++)
 #pragma unroll
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) void k_phi32(const double *__restrict__ rho, c
 // the other slots along their weighted densities ndot = W v (in ndio on entry);
 // on exit ndio holds H_Phi(n) (N_A v, ndot) for every slot, the forward-mode
 // derivative of phi_grad taken with dual numbers.
-template <int NC, int ND>
+template <int NC, int ND, int AS = 0>
 __global__ __launch_bounds__(256) void k_phi_hvp(const double *__restrict__ rho, const double *__restrict__ v,
                                                  const double *__restrict__ n, double *__restrict__ ndio, int N,
                                                  double NA, PhiConst P)
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void k_phi_hvp(const double *__restrict__ rho,
 #pragma unroll
         for (int f = 1; f < F; f++) nl[c][f] = dual(n[(size_t)(c * F + f) * N + i], ndio[(size_t)(c * F + f) * N + i]);
     }
-    phi_grad<dual, dual, NC, ND>(P, nl, g);
+    phi_grad<dual, dual, NC, ND, true, AS>(P, nl, g);
 #pragma unroll
     for (int c = 0; c < NC; c++)
 #pragma unroll

@@ -68,6 +68,16 @@ typedef struct {
     const double *mu_res;   /* [nc] beta mu_res of the bulk                      */
     const double *vext;     /* [N*nc] beta V_ext (external_field.jl:6-19), NULL  */
     int precision;          /* CDFT_FP64 or CDFT_MIXED                           */
+    /* Wertheim association, the data pack_assoc_params flattens
+     * (src/models/DFT/assoc.jl:18-57).  npair = 0 (or a zeroed tail) means none;
+     * the device path takes up to 4 sites and 8 pairs. */
+    int assoc_nsite;                  /* site types over all components          */
+    int assoc_npair;                  /* bonding pairs                           */
+    const int *assoc_site_comp;       /* [nsite] component of each site, 0-based */
+    const double *assoc_site_count;   /* [nsite] multiplicity of each site type  */
+    const int *assoc_pair_site;       /* [2*npair] the two sites (a, b) per pair */
+    const double *assoc_pair_eps;     /* [npair] eps_AB / k_B, K                 */
+    const double *assoc_pair_kappa;   /* [npair] bonding volume kappa_AB         */
 } cdft_system_desc;
 
 /* converge! keywords (src/methods/converge.jl:3-13, :51-60). */

@@ -338,6 +338,151 @@ static void I_lite(const double corr[7][3], double mb, double eta, double *I, do
     *Ie = ve;
 }
 
+/* ---- Wertheim association (f_assoc, assoc.jl:702-831) ----
+ * Per component i: n0_i = S_i/d_i, xi_i = 1 - |nv2_i|^2/n2_i^2 with n2_i = pi S_i d_i,
+ * nv2_i = -2 pi v_i (assoc.jl:180-221); w_i = n0_i xi_i is the associating density.
+ * Per pair p between sites a and b: Delta_p = g_hs sigma_ab^3 (exp(eps/T) - 1) kappa
+ * with g_hs = 1/(1-n3) + d_ab xi n2/(2(1-n3)^2) + d_ab^2 n2^2 xi/(18(1-n3)^3) of the
+ * mixture n2, n3, xi (assoc.jl:159-168), d_ab = d_i d_j/(d_i + d_j).
+ * Site fractions: X_s = 1/(1 + C_s) with
+ * C_s = sum_{p: a_p = s} v_b Delta_p X_b + sum_{p: b_p = s} v_a Delta_p X_a, v_t = n_t w_c(t). */
+typedef struct {
+    int ns, np;
+    double v[OR_MAXS], delta[OR_MAXP];
+} assoc_pt;
+
+static double assoc_C(const or_system *s, const assoc_pt *A, const double *X, int site)
+{
+    double c = 0;
+    for (int p = 0; p < A->np; p++) {
+        if (s->pair_a[p] == site) c += A->v[s->pair_b[p]] * A->delta[p] * X[s->pair_b[p]];
+        if (s->pair_b[p] == site) c += A->v[s->pair_a[p]] * A->delta[p] * X[s->pair_a[p]];
+    }
+    return c;
+}
+
+/* X0 = 0.9, 5 NS blended substitutions, then 5 NS safeguarded Newton steps with
+ * Gaussian elimination without pivoting (assoc.jl:238-411). */
+static void assoc_solve_general(const or_system *s, const assoc_pt *A, double *X)
+{
+    int ns = A->ns;
+    double C[OR_MAXS], Xn[OR_MAXS], J[OR_MAXS][OR_MAXS + 1];
+    for (int t = 0; t < ns; t++) X[t] = 0.9;
+    for (int it = 0; it < 5 * ns; it++) {
+        for (int t = 0; t < ns; t++) Xn[t] = 0.5 * (X[t] + 1.0 / (1.0 + assoc_C(s, A, X, t)));
+        memcpy(X, Xn, sizeof(double) * ns);
+    }
+    for (int it = 0; it < 5 * ns; it++) {
+        for (int t = 0; t < ns; t++) C[t] = assoc_C(s, A, X, t);
+        for (int a = 0; a < ns; a++) {
+            for (int b = 0; b < ns; b++) J[a][b] = (a == b) ? 1.0 + C[a] : 0.0;
+            J[a][ns] = X[a] * (1.0 + C[a]) - 1.0;
+        }
+        for (int p = 0; p < A->np; p++) {
+            int a = s->pair_a[p], b = s->pair_b[p];
+            J[a][b] += X[a] * A->v[b] * A->delta[p];
+            J[b][a] += X[b] * A->v[a] * A->delta[p];
+        }
+        for (int k = 0; k < ns; k++)
+            for (int i = k + 1; i < ns; i++) {
+                double f = J[i][k] / J[k][k];
+                for (int j = k + 1; j <= ns; j++) J[i][j] -= f * J[k][j];
+            }
+        double dX[OR_MAXS];
+        for (int k = ns - 1; k >= 0; k--) {
+            double r = J[k][ns];
+            for (int j = k + 1; j < ns; j++) r -= J[k][j] * dX[j];
+            dX[k] = r / J[k][k];
+        }
+        for (int t = 0; t < ns; t++) {
+            double cand = X[t] - dX[t];
+            Xn[t] = (cand > 0.0 && cand < 1.0) ? cand : 1.0 / (1.0 + C[t]);
+        }
+        memcpy(X, Xn, sizeof(double) * ns);
+    }
+}
+
+/* X for the point's associating densities; one pair between two distinct sites has
+ * the closed form of assoc.jl:759-764. */
+static void assoc_solve(const or_system *s, const assoc_pt *A, double *X)
+{
+    if (A->np == 1 && s->pair_a[0] != s->pair_b[0]) {
+        int a = s->pair_a[0], b = s->pair_b[0];
+        for (int t = 0; t < A->ns; t++) X[t] = 1.0;
+        double kia = A->v[a] * A->delta[0], kjb = A->v[b] * A->delta[0];
+        double bb = 1.0 - kia + kjb;
+        X[a] = 2.0 / (bb + sqrt(bb * bb + 4.0 * kia));
+        X[b] = 1.0 / (1.0 + kia * X[a]);
+        return;
+    }
+    assoc_solve_general(s, A, X);
+}
+
+/* Phi_assoc = sum_s v_s (ln|X_s| - X_s/2 + 1/2) (assoc.jl:811-831) and, when dphi is
+ * given, its gradient added into dphi.  At the solution of the mass-action law,
+ * dPhi/dw_i = sum_{s in i} n_s ln X_s and dPhi/dDelta_p = -v_a X_a v_b X_b (the
+ * stationarity of Michelsen and Hendriks' Q function; the reference obtains the same
+ * derivative with an implicit-function-theorem rule, assoc.jl:833-916). */
+static double or_assoc(const or_system *s, const double *nl, double n2m, double n3m, const double *nv2m,
+                       double *dphi, double *Xout)
+{
+    int nc = s->nc, nd = s->nd, F = 6 + nd, ns = s->nsite;
+    const double twopi = 2.0 * PI;
+    double w[OR_MAXC], dwS[OR_MAXC], dwv[OR_MAXC][3];
+    for (int i = 0; i < nc; i++) {
+        double di = s->d[i] / s->L, S = nl[i * F + 1];
+        double n0 = S / di, n2 = PI * S * di, sq = 0, nv[3] = {0, 0, 0};
+        for (int j = 0; j < nd; j++) { nv[j] = -twopi * nl[i * F + 3 + j]; sq += nv[j] * nv[j]; }
+        double xi = 1.0 - sq / (n2 * n2);
+        w[i] = n0 * xi;
+        dwS[i] = xi / di + n0 * (2.0 * sq / (n2 * n2 * n2)) * PI * di;
+        for (int j = 0; j < 3; j++) dwv[i][j] = n0 * (-2.0 * nv[j] / (n2 * n2)) * (-twopi);
+    }
+    double nv2sq = 0;
+    for (int j = 0; j < nd; j++) nv2sq += nv2m[j] * nv2m[j];
+    double xim = 1.0 - nv2sq / (n2m * n2m);
+    double a1 = 1.0 / (1.0 - n3m);
+    assoc_pt A;
+    A.ns = ns;
+    A.np = s->npair;
+    double cpair[OR_MAXP], dij[OR_MAXP];
+    for (int t = 0; t < ns; t++) A.v[t] = s->site_count[t] * w[s->site_comp[t]];
+    for (int p = 0; p < A.np; p++) {
+        int ic = s->site_comp[s->pair_a[p]], jc = s->site_comp[s->pair_b[p]];
+        double sr = s->sigma[ic * nc + jc] / s->L, di = s->d[ic] / s->L, dj = s->d[jc] / s->L;
+        cpair[p] = sr * sr * sr * (exp(s->pair_eps[p] / s->T) - 1.0) * s->pair_kappa[p];
+        dij[p] = di * dj / (di + dj);
+        double g = a1 + 0.5 * dij[p] * xim * n2m * a1 * a1 + dij[p] * dij[p] * n2m * n2m * xim * a1 * a1 * a1 / 18.0;
+        A.delta[p] = g * cpair[p];
+    }
+    double X[OR_MAXS];
+    assoc_solve(s, &A, X);
+    if (Xout) memcpy(Xout, X, sizeof(double) * ns);
+    double phi = 0;
+    for (int t = 0; t < ns; t++) phi += A.v[t] * (log(fabs(X[t])) - 0.5 * X[t] + 0.5);
+    if (!dphi) return phi;
+
+    double dw[OR_MAXC] = {0}, dn3 = 0, dn2 = 0, dxi = 0;
+    for (int t = 0; t < ns; t++) dw[s->site_comp[t]] += s->site_count[t] * log(fabs(X[t]));
+    for (int p = 0; p < A.np; p++) {
+        int a = s->pair_a[p], b = s->pair_b[p];
+        double dg = -A.v[a] * X[a] * A.v[b] * X[b] * cpair[p];
+        double dd = dij[p];
+        dn3 += dg * (a1 * a1 + dd * xim * n2m * a1 * a1 * a1 + dd * dd * n2m * n2m * xim * a1 * a1 * a1 * a1 / 6.0);
+        dn2 += dg * (0.5 * dd * xim * a1 * a1 + dd * dd * n2m * xim * a1 * a1 * a1 / 9.0);
+        dxi += dg * (0.5 * dd * n2m * a1 * a1 + dd * dd * n2m * n2m * a1 * a1 * a1 / 18.0);
+    }
+    dn2 += dxi * 2.0 * nv2sq / (n2m * n2m * n2m);
+    for (int i = 0; i < nc; i++) {
+        double mi = s->m[i], di = s->d[i] / s->L;
+        dphi[i * F + 1] += dn2 * PI * mi * di + dw[i] * dwS[i];
+        dphi[i * F + 2] += dn3 * mi;
+        for (int j = 0; j < nd; j++)
+            dphi[i * F + 3 + j] += dxi * (-2.0 * nv2m[j] / (n2m * n2m)) * (-twopi * mi) + dw[i] * dwv[i][j];
+    }
+    return phi;
+}
+
 double or_phi(const or_system *s, const double *nl, double *dphi)
 {
     int nc = s->nc, nd = s->nd, F = 6 + nd;
@@ -389,6 +534,8 @@ double or_phi(const or_system *s, const double *nl, double *dphi)
             for (int j = 0; j < nd; j++) dphi[i * F + 3 + j] += mi * d_nv1[j] / di - twopi * mi * d_nv2[j];
         }
     }
+
+    double phi_assoc = s->npair > 0 ? or_assoc(s, nl, n2, n3, nv2, dphi, NULL) : 0.0;
 
     /* ---- hard chain (Sauer-Gross) ---- */
     int iz = 3 + nd, il = 4 + nd, idp = 5 + nd;
@@ -472,7 +619,21 @@ double or_phi(const or_system *s, const double *nl, double *dphi)
             dphi[k * F + idp] += g * factor / (dd[k] * dd[k] * dd[k]);
         }
     }
-    return phi_hs + phi_hc + phi_disp;
+    return phi_hs + phi_hc + phi_disp + phi_assoc;
+}
+
+void or_assoc_X(const or_system *s, const double *nl, double *X)
+{
+    int nc = s->nc, nd = s->nd, F = 6 + nd;
+    double n2 = 0, n3 = 0, nv2[3] = {0, 0, 0};
+    for (int i = 0; i < nc; i++) {
+        double di = s->d[i] / s->L;
+        n2 += PI * nl[i * F + 1] * s->m[i] * di;
+        n3 += nl[i * F + 2] * s->m[i];
+        for (int j = 0; j < nd; j++) nv2[j] -= 2.0 * PI * nl[i * F + 3 + j] * s->m[i];
+    }
+    for (int t = 0; t < s->nsite; t++) X[t] = 1.0;
+    if (s->npair > 0) or_assoc(s, nl, n2, n3, nv2, NULL, X);
 }
 
 /* delta F_res / delta rho (models.jl:131-186): fields, pointwise gradient, transposed. */

@@ -27,6 +27,8 @@ extern "C" {
 #endif
 
 #define OR_MAXC 8
+#define OR_MAXS 20                   /* association sites (assoc.jl pads to 20)  */
+#define OR_MAXP 20                   /* association bonding pairs               */
 
 typedef struct {
     int nc;                          /* components (one bead each)          */
@@ -43,6 +45,13 @@ typedef struct {
     double mu_res[OR_MAXC];          /* beta mu_res (bulk)                  */
     double rho_bulk[OR_MAXC];        /* mol/m^3                             */
     const double *vext;              /* beta V_ext [c*N+idx] or NULL        */
+    /* Wertheim association (assoc.jl:18-57 pack_assoc_params); npair = 0: none */
+    int nsite, npair;
+    int site_comp[OR_MAXS];          /* component of each site (0-based)    */
+    double site_count[OR_MAXS];      /* multiplicity n of each site type    */
+    int pair_a[OR_MAXP], pair_b[OR_MAXP]; /* global site indices of a pair  */
+    double pair_eps[OR_MAXP];        /* eps_AB / k_B, K                     */
+    double pair_kappa[OR_MAXP];      /* bonding volume kappa_AB             */
 } or_system;
 
 typedef struct {
@@ -71,6 +80,8 @@ void   or_fft(double *re, double *im, int nd, const int *ngrid, int inverse);
 void   or_omega(const or_system *s, double *om_re, double *om_im);
 void   or_weighted_densities(const or_system *s, const double *rho, double *n);
 double or_phi(const or_system *s, const double *nloc, double *dphi);
+/* association site fractions X (nsite values) at one point (assoc.jl:702-831) */
+void   or_assoc_X(const or_system *s, const double *nloc, double *X);
 void   or_transposed(const or_system *s, const double *dphi, double *out);
 void   or_dFdrho_res(const or_system *s, const double *rho, double *dfdrho, double *fval);
 double or_simpson(const double *f, int nd, const int *ngrid, const double *dz);

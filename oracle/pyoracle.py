@@ -13,6 +13,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 MAXC = 8
+MAXS = 20
+MAXP = 20
 
 _dp = C.POINTER(C.c_double)
 
@@ -26,6 +28,10 @@ class OrSystem(C.Structure):
         ("sigma", C.c_double * (MAXC * MAXC)), ("eps", C.c_double * (MAXC * MAXC)),
         ("mu_res", C.c_double * MAXC), ("rho_bulk", C.c_double * MAXC),
         ("vext", _dp),
+        ("nsite", C.c_int), ("npair", C.c_int),
+        ("site_comp", C.c_int * MAXS), ("site_count", C.c_double * MAXS),
+        ("pair_a", C.c_int * MAXP), ("pair_b", C.c_int * MAXP),
+        ("pair_eps", C.c_double * MAXP), ("pair_kappa", C.c_double * MAXP),
     ]
 
 
@@ -59,6 +65,7 @@ def lib():
         L.or_weighted_densities.argtypes = [P, _dp, _dp]
         L.or_phi.argtypes = [P, _dp, _dp]
         L.or_phi.restype = C.c_double
+        L.or_assoc_X.argtypes = [P, _dp, _dp]
         L.or_transposed.argtypes = [P, _dp, _dp]
         L.or_dFdrho_res.argtypes = [P, _dp, _dp, _dp]
         L.or_simpson.argtypes = [_dp, C.c_int, C.POINTER(C.c_int), _dp]
@@ -104,7 +111,8 @@ class Oracle:
     """One oracle system (mirrors the C struct; keeps numpy buffers alive)."""
 
     def __init__(self, *, nc, nd, ngrid, lb, ub, T, L, m, d, sigma, eps, mu_res, rho_bulk,
-                 vext=None, psi=1.3862):
+                 vext=None, psi=1.3862, assoc_site_comp=(), assoc_site_count=(), assoc_pairs=(),
+                 assoc_eps=(), assoc_kappa=()):
         s = OrSystem()
         s.nc, s.nd = nc, nd
         for a in range(nd):
@@ -120,6 +128,14 @@ class Oracle:
             for j in range(nc):
                 s.sigma[i * nc + j] = sig[i, j]
                 s.eps[i * nc + j] = ep[i, j]
+        s.nsite, s.npair = len(assoc_site_comp), len(assoc_pairs)
+        if s.nsite > MAXS or s.npair > MAXP:
+            raise ValueError("too many association sites or pairs")
+        for t in range(s.nsite):
+            s.site_comp[t] = int(assoc_site_comp[t]); s.site_count[t] = float(assoc_site_count[t])
+        for q in range(s.npair):
+            s.pair_a[q], s.pair_b[q] = int(assoc_pairs[q][0]), int(assoc_pairs[q][1])
+            s.pair_eps[q], s.pair_kappa[q] = float(assoc_eps[q]), float(assoc_kappa[q])
         self.N = int(np.prod(ngrid[:nd]))
         self.nc, self.nd, self.F = nc, nd, 6 + nd
         self._vext = None
@@ -153,6 +169,11 @@ class Oracle:
         d = np.zeros(self.F * self.nc)
         v = lib().or_phi(self.ref, _ptr(nloc), _ptr(d))
         return v, d
+
+    def assoc_X(self, nloc):
+        X = np.ones(max(self.s.nsite, 1))
+        lib().or_assoc_X(self.ref, _ptr(_f64(nloc)), _ptr(X))
+        return X[:self.s.nsite]
 
     def transposed(self, dphi):
         dphi = _f64(dphi).reshape(-1)
