@@ -10,12 +10,13 @@
   slit_pores      512 methane slit pores between two graphite Steele walls at 150 K,
                   widths W ~ U[2 sigma, 12 sigma], p ~ logU[1e3, 1e7] Pa from seed 0,
                   stable bulk root, N = round(40 W / sigma) points over [0.5 L, W - 0.5 L]
-  matrix_3d       methane / ethane / propane (x = 0.6, 0.3, 0.1) at 200 K and 3e6 Pa (its only
-                  volume root, a liquid) in a periodic cube of 30 sigma on a 256^3 grid, with a
-                  matrix of 64 non-overlapping hard spheres, radii ~ U[sigma, 2.5 sigma], from
-                  seed 0.  Solve with anderson_start=0.5 (MATRIX_3D_SOLVER): from the bulk
-                  seed, damped Picard alone stalls on this system far above the reference's
-                  default switch to Anderson at 0.1 res0 + 0.1.
+  matrix_3d       methane / ethane / propane (x = 0.6, 0.3, 0.1) at 200 K and 1e7 Pa (a
+                  stable liquid: the bubble pressure at 200 K is 3.2e6 Pa, below which the
+                  homogeneous liquid is metastable and dries next to the spheres) in a
+                  periodic cube of 30 sigma on a 256^3 grid, with a matrix of 64
+                  non-overlapping hard spheres, radii ~ U[sigma, 2.5 sigma], from seed 0.
+                  Solved with anderson_start=0.5 (MATRIX_3D_SOLVER): the damped Picard
+                  warm-up makes slow progress on this system, and Anderson takes over earlier.
 """
 import numpy as np
 
@@ -76,7 +77,7 @@ def slit_pores(n=512, seed=0, options=None, T=150.0, indices=None):
 MATRIX_3D_SOLVER = dict(anderson_start=0.5)
 
 
-def matrix_3d(options=None, n=256, box=30.0, nspheres=64, seed=0, T=200.0, p=3e6, x=(0.6, 0.3, 0.1),
+def matrix_3d(options=None, n=256, box=30.0, nspheres=64, seed=0, T=200.0, p=1e7, x=(0.6, 0.3, 0.1),
               components=("methane", "ethane", "propane"), rmin=1.0, rmax=2.5, combining="nocombining"):
     model = PCSAFT(list(components), combining=combining)
     x = np.asarray(x, dtype=np.float64)

@@ -105,7 +105,8 @@ double simpson1d(const double *f, int n, double dz)
 void free_sys_buffers(cdft_sys *s)
 {
     double **bufs[] = {&s->h, &s->vext, &s->cpd, &s->rho, &s->n, &s->dphi, &s->part, &s->fval, &s->out, &s->sol,
-                       &s->gx, &s->df, &s->dg, &s->res, &s->Q, &s->Qalt, &s->DG, &s->red, &s->fin, &s->hddft};
+                       &s->gx, &s->df, &s->dg, &s->res, &s->Q, &s->Qalt, &s->DG, &s->red, &s->fin, &s->hddft,
+                       &s->xsafe};
     for (double **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (s->H64) { (void)hipFree(s->H64); s->H64 = nullptr; }
@@ -617,6 +618,9 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
             P.cm[i * nc + j] = s->m[i] * s->m[j] * sr * sr * sr;
             P.eT[i * nc + j] = s->eps[i * nc + j] / s->T;
         }
+    // FP32 resolution of the lambda slot (2 d rho_b for a uniform profile):
+    // convolution rounding is ~1e-7 of the field's bulk scale
+    for (int i = 0; i < nc; i++) P.lsl_noise[i] = 1e-7 * 2.0 * P.dd[i] * d->rho_bulk[i] * s->NA;
     if (d->assoc_npair > 0) {
         // pack_assoc_params (assoc.jl:18-57) in reduced units (PCSAFT.jl:244-246)
         P.nsite = d->assoc_nsite;
@@ -655,7 +659,8 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
         (d3 && d->precision == CDFT_MIXED &&
          ((e = hipMalloc((void **)&s->G3f, sizeof(float) * 8 * nc * (size_t)N)) != hipSuccess ||
           (e = hipMalloc((void **)&s->n32, sizeof(float) * (size_t)F * nc * N)) != hipSuccess ||
-          (e = hipMalloc((void **)&s->dphi32, sizeof(float) * (size_t)F * nc * N)) != hipSuccess)) ||
+          (e = hipMalloc((void **)&s->dphi32, sizeof(float) * (size_t)F * nc * N)) != hipSuccess ||
+          (e = alloc(&s->xsafe, n)) != hipSuccess)) ||
         (e = alloc(&s->out, n)) != hipSuccess || (e = alloc(&s->sol, n)) != hipSuccess ||
         (e = alloc(&s->gx, n)) != hipSuccess || (e = alloc(&s->df, n)) != hipSuccess ||
         (e = alloc(&s->dg, n)) != hipSuccess || (e = alloc(&s->res, n)) != hipSuccess ||
@@ -836,6 +841,21 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     const int stall_window = 40;
     double best = 0.0;
     int since_best = 0;
+    // A non-finite FP32 map value (a step the FP32 range cannot hold, such as
+    // densities near exp(100) where the exponent clamps) ends the FP32 phase: the
+    // solve goes back to the last point whose FP32 value was finite and continues
+    // with the FP64 map.  (The FP64 map keeps the reference's NaN semantics.)
+    bool fp32_failed = false;
+    auto keep_safe = [&](double r) -> int {     // s->sol had a finite FP32 value
+        if (fp32 && std::isfinite(r))
+            HIPCHECK_CTX(ctx, hipMemcpyAsync(s->xsafe, s->sol, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        return CDFT_OK;
+    };
+    auto restore_safe = [&]() {                 // back to the last point with a finite FP32 value
+        hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->xsafe, s->sol, s->rho, (int)n,
+                           (const double *)nullptr);
+    };
+#define KEEP_SAFE(r) do { int rc_ = keep_safe(r); if (rc_ != CDFT_OK) return rc_; } while (0)
 
     // G evaluation + post-processing; returns the finalized reductions in fin_h[0..2]
     double *ptgt = adj ? adj->vbuf : s->rho;              // where a new point is expanded
@@ -870,6 +890,12 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     int k = 0;
     EVAL(1, beta);
     double resnorm = std::sqrt(fin_h[0]);
+    if (fp32 && !std::isfinite(resnorm)) {      // the seed itself: FP64 from the start
+        fp32 = false;
+        EVAL(1, beta);
+        resnorm = std::sqrt(fin_h[0]);
+    }
+    KEEP_SAFE(resnorm);
     const double up_bd = 1.e4 * resnorm;
     const double tol = rtol * resnorm + atol;
     const double picard_tol = picard_rtol * resnorm + picard_atol;
@@ -881,6 +907,14 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
         for (int it = 0; it < picard_maxit - 1; it++) {
             EVAL(1, picard_beta);
             resnorm = std::sqrt(fin_h[0]);
+            if (fp32 && !std::isfinite(resnorm)) {
+                hist(resnorm);
+                restore_safe();
+                fp32 = false;
+                EVAL(1, picard_beta);
+                resnorm = std::sqrt(fin_h[0]);
+            }
+            KEEP_SAFE(resnorm);
             hist(resnorm);
             if (resnorm <= picard_tol || resnorm >= up_bd) break;
             hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, ptgt, (int)n, pmask);
@@ -902,6 +936,12 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
         dfn2 = fin_h[1];
         dfres = fin_h[2];
         hist(resnorm);
+        if (fp32 && !std::isfinite(resnorm)) {
+            restore_safe();
+            fp32_failed = true;
+            resnorm = 0.0;                       // the switch at the top of the loop takes over
+        }
+        KEEP_SAFE(resnorm);
         best = resnorm;
     }
 
@@ -914,7 +954,7 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
             } else {
                 since_best++;
             }
-            if (resnorm <= switch_factor * tol || since_best >= stall_window) {
+            if (resnorm <= switch_factor * tol || since_best >= stall_window || fp32_failed) {
                 fp32 = false;                     // the FP64 map from here
                 EVAL(1, beta);
                 resnorm = std::sqrt(fin_h[0]);
@@ -1004,8 +1044,15 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
         dfn2 = fin_h[1];
         dfres = fin_h[2];
         hist(resnorm);
+        if (fp32 && !std::isfinite(resnorm)) {
+            restore_safe();
+            fp32_failed = true;
+            resnorm = 0.0;
+        }
+        KEEP_SAFE(resnorm);
     }
 #undef EVAL
+#undef KEEP_SAFE
 
     int idid, errcode;
     if (toosoon) { idid = 1; errcode = -1; }

@@ -86,6 +86,7 @@ struct cdft_sys {
     double *B3x[3] = {nullptr, nullptr, nullptr};
     // 3D mixed precision: FP32 spectra, weighted densities and gradients
     float *G3f = nullptr, *n32 = nullptr, *dphi32 = nullptr;
+    double *xsafe = nullptr;   // 3D mixed precision: the last point whose FP32 map value was finite
     double *hddft = nullptr;   // 1D DDFT derivative kernels [(c*4 + f)*N + j], f = 1 (Laplacian), 2, 3 (gradient)
     double *red = nullptr;     // block partials
     double *fin = nullptr;     // finalized reductions [CDFT_NQ]

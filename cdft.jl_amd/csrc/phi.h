@@ -44,6 +44,9 @@ struct PhiConst {
     int pair_a[CDFT_MAXPAIR], pair_b[CDFT_MAXPAIR];   // the two sites of each pair
     double pair_k[CDFT_MAXPAIR];          // (sigma_ab / L)^3 (exp(eps_AB / T) - 1) kappa_AB
     double pair_dij[CDFT_MAXPAIR];        // d_i d_j / (d_i + d_j) / L
+    // phi_grad<..., F32IN = true>: per component, the lambda-slot magnitude below
+    // which an FP32 convolution's value is rounding noise (1e-7 of its bulk value)
+    double lsl_noise[CDFT_MAXC];
 };
 
 namespace cdft {
@@ -276,7 +279,13 @@ __device__ __forceinline__ T assoc_grad(const PhiConst &P, const T (&nl)[NC][6 +
 // is for callers that know every segment number is 1 (no chain term): the body is
 // then free of branches, so two inlined calls can be scheduled side by side.
 // AS selects the association term: 0 none, 1 one pair in closed form, 2 general.
-template <typename T, typename TS, int NC, int ND, bool CHAIN = true, int AS = 0>
+// F32IN = true is for weighted densities from FP32 convolutions of a profile whose
+// density is negligible somewhere: where the chain's lambda slot is below its FP32
+// resolution (P.lsl_noise), -rho (m - 1) / lambda is rounding noise over rounding
+// noise, and that slot's gradient is taken as 0, as the FP64 map's is to rounding
+// there (its lambda, though small, is resolved against a negligible rho).  A
+// spuriously large value would reach every grid point through the transposed FFT.
+template <typename T, typename TS, int NC, int ND, bool CHAIN = true, int AS = 0, bool F32IN = false>
 __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + ND], T (&g)[NC][6 + ND])
 {
     constexpr int F = 6 + ND;
@@ -317,13 +326,18 @@ __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + N
     TS bkt = log1dn / (twelvepi * n3 * n3) + TS(1) / (twelvepi * n3 * dn * dn);
     TS dB = (TS(-1) / (n3 * n3 * dn) - TS(2) * log1dn / (n3 * n3 * n3) +
              (TS(3) * n3 - TS(1)) / (n3 * n3 * dn * dn * dn)) / twelvepi;
-    if (std::is_same<T, float>::value) {
-        // FP32 inputs: n3 is at rounding-noise level inside walls and may be 0.
-        // Series of the same bracket, sum_k ((k+2)^2 - 1)/(k+2) n3^k / (12 pi);
-        // selected rather than branched to, so the body stays one block.
-        const bool small = fabs(n3) < TS(1e-5);
-        const TS bs = (TS(1.5) + n3 * (TS(8.0 / 3.0) + n3 * (TS(3.75) + n3 * TS(4.8)))) / twelvepi;
-        const TS ds = (TS(8.0 / 3.0) + n3 * (TS(7.5) + n3 * TS(14.4))) / twelvepi;
+    {
+        // The closed form cancels two O(1/n3) terms: its rounding error grows as
+        // eps/n3^2 (eps/n3^3 for dB), which next to sharp walls, where a convolution
+        // rings around n3 = 0 while n2 does not, reaches the gradient through
+        // n2^3 dB.  Below |n3| = 1e-3 the same function comes from its Taylor series,
+        // sum_k (k+1)(k+3)/(k+2) n3^k / (12 pi), exact to FP64 rounding; selected
+        // rather than branched to, so the body stays one block.
+        const bool small = fabs(n3) < TS(1e-3);
+        const TS bs = (TS(1.5) + n3 * (TS(8.0 / 3.0) + n3 * (TS(15.0 / 4.0) + n3 * (TS(24.0 / 5.0) +
+                       n3 * (TS(35.0 / 6.0) + n3 * (TS(48.0 / 7.0) + n3 * TS(63.0 / 8.0))))))) / twelvepi;
+        const TS ds = (TS(8.0 / 3.0) + n3 * (TS(2 * 15.0 / 4.0) + n3 * (TS(3 * 24.0 / 5.0) + n3 * (TS(4 * 35.0 / 6.0) +
+                       n3 * (TS(5 * 48.0 / 7.0) + n3 * TS(6 * 63.0 / 8.0)))))) / twelvepi;
         bkt = small ? bs : bkt;
         dB = small ? ds : dB;
     }
@@ -403,7 +417,9 @@ __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + N
         mt += rb[i] * T(P.m[i]);
         es += T(P.m[i]) * rb[i] * di * di * di;
     }
-    if (std::is_same<T, float>::value) S = fmax(S, T(1e-30));
+    // FP32: keep an underflowed S off 0/0, and keep its sign (a convolution may ring
+    // below zero where the density is negligible, as in FP64)
+    if (std::is_same<T, float>::value) S = copysign(fmax(fabs(S), T(1e-30)), S);
     T mb = (std::is_same<T, float>::value && NC == 1) ? T(P.m[0]) : mt / S;
     T eta = es * pi / T(6);
     T A1 = 0, A2 = 0, dA1[NC], dA2[NC];
@@ -449,6 +465,10 @@ __device__ __forceinline__ T phi_grad(const PhiConst &P, const T (&nl)[NC][6 + N
                         mb * I2 * A2 * (C1m * dm + C1e * de) * iC1 * iC1);
             g[k][IDP] += gk * factor / (di * di * di);
         }
+    }
+    if constexpr (F32IN) {
+#pragma unroll
+        for (int i = 0; i < NC; i++) g[i][IL] = (fabs(nl[i][IL]) < T(P.lsl_noise[i])) ? T(0) : g[i][IL];
     }
     return phi;
 }

@@ -224,7 +224,8 @@ __global__ __launch_bounds__(256) void k_phi(const double *__restrict__ rho, con
 }
 
 // FP32 map (mixed precision): Phi's gradient from FP32 weighted densities, with the
-// n3-singular hard-sphere factors in FP64 (phi_grad<float, double>).
+// n3-singular hard-sphere factors in FP64 (phi_grad<float, double>); the chain's
+// lambda-slot gradient is 0 where FP32 does not resolve lambda.
 template <int NC, int ND, int AS = 0>
 __global__ __launch_bounds__(256) void k_phi32(const double *__restrict__ rho, const float *__restrict__ n,
                                                float *__restrict__ dphi, int N, double NA, PhiConst P)
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(256) void k_phi32(const double *__restrict__ rho, c
 #pragma unroll
         for (int f = 1; f < F; f++) nl[c][f] = n[(size_t)(c * F + f) * N + i];
     }
-    phi_grad<float, double, NC, ND, true, AS>(P, nl, g);
+    phi_grad<float, double, NC, ND, true, AS, true>(P, nl, g);
 #pragma unroll
     for (int c = 0; c < NC; c++)
 #pragma unroll

@@ -512,12 +512,27 @@ double or_phi(const or_system *s, const double *nl, double *dphi)
     double dn = 1.0 - n3;
     double log1dn = log(fabs(dn));
     double inv1dn = 1.0 / dn;
-    double bkt = log1dn / (twelvepi * n3 * n3) + 1.0 / (twelvepi * n3 * dn * dn);
+    /* bkt = ln(1-n3)/(12 pi n3^2) + 1/(12 pi n3 (1-n3)^2) (FMT.jl:40-41) cancels two
+     * O(1/n3) terms, so its rounding error grows as eps/n3^2 (its derivative's as
+     * eps/n3^3).  Below |n3| = 1e-3 the same function is evaluated from its Taylor
+     * series, sum_k (k+1)(k+3)/(k+2) n3^k / (12 pi), to FP64 accuracy. */
+    double bkt, dB;
+    if (fabs(n3) < 1e-3) {
+        const double c[7] = {1.5, 8.0 / 3.0, 15.0 / 4.0, 24.0 / 5.0, 35.0 / 6.0, 48.0 / 7.0, 63.0 / 8.0};
+        bkt = 0.0;
+        dB = 0.0;
+        for (int k = 6; k >= 0; k--) bkt = bkt * n3 + c[k];
+        for (int k = 6; k >= 1; k--) dB = dB * n3 + k * c[k];
+        bkt /= twelvepi;
+        dB /= twelvepi;
+    } else {
+        bkt = log1dn / (twelvepi * n3 * n3) + 1.0 / (twelvepi * n3 * dn * dn);
+        dB = (-1.0 / (n3 * n3 * dn) - 2.0 * log1dn / (n3 * n3 * n3) + (3.0 * n3 - 1.0) / (n3 * n3 * dn * dn * dn)) /
+             twelvepi;
+    }
     double phi_hs = -n0 * log1dn + (n1 * n2 - nv1v2) * inv1dn + (n2 * n2 * n2 / 3.0 - n2 * nv2sq) * bkt;
 
     if (dphi) {
-        double dB = (-1.0 / (n3 * n3 * dn) - 2.0 * log1dn / (n3 * n3 * n3) +
-                     (3.0 * n3 - 1.0) / (n3 * n3 * dn * dn * dn)) / twelvepi;
         double d_n0 = -log1dn;
         double d_n1 = n2 * inv1dn;
         double d_n2 = n1 * inv1dn + (n2 * n2 - nv2sq) * bkt;

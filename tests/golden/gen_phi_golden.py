@@ -137,6 +137,30 @@ def main():
             g = [[sp.N(grads[i][f].subs(vals), 50) for f in range(F)] for i in range(nc)]
             pts.append({"n": [[float(vals[nsym[i][f]]) for f in range(F)] for i in range(nc)],
                         "phi": str(phi), "dphi": [[str(g[i][f]) for f in range(F)] for i in range(nc)]})
+        # dilute and ringing-like points: packing fraction between -3e-4 and 2e-3 (the
+        # small-n3 series branch and just above it), the S slot up to 30 times what the
+        # packing fraction implies, as next to a sharp wall in a spectral convolution
+        rng2 = random.Random(20261204 + nc * 10 + nd)
+        for pack, sfac in ((3e-4, 1.0), (8e-4, 30.0), (-3e-4, 10.0), (2e-3, 5.0)):
+            xs = [rng2.uniform(0.2, 1.0) for _ in range(nc)]
+            tot = sum(x * m[i] * dd[i] ** 3 for i, x in enumerate(xs)) * 3.141592653589793 / 6
+            rho = [x * abs(pack) / tot for x in xs]
+            vals = {}
+            for i in range(nc):
+                r = rho[i]
+                base = [r, r * dd[i] * sfac, r * 4 * 3.141592653589793 / 3 * (dd[i] / 2) ** 3 * (1 if pack > 0 else -1)] + \
+                       [0.0] * nd + [r * 4 * 3.141592653589793 / 3 * dd[i] ** 3, r * 2 * dd[i],
+                                     r * 4 * 3.141592653589793 / 3 * (1.3862 * dd[i]) ** 3]
+                for f in range(F):
+                    if 3 <= f < 3 + nd:
+                        v = rng2.uniform(-0.15, 0.15) * r * dd[i] ** 2 * sfac
+                    else:
+                        v = base[f] * (1 + rng2.uniform(-0.1, 0.1))
+                    vals[nsym[i][f]] = sp.Float(repr(v), 60)
+            phi = sp.N(expr.subs(vals), 50)
+            g = [[sp.N(grads[i][f].subs(vals), 50) for f in range(F)] for i in range(nc)]
+            pts.append({"n": [[float(vals[nsym[i][f]]) for f in range(F)] for i in range(nc)],
+                        "phi": str(phi), "dphi": [[str(g[i][f]) for f in range(F)] for i in range(nc)]})
         out["cases"].append({"name": case["name"], "nc": nc, "nd": nd, "T": T, "L": L, "m": m,
                              "d": d, "sigma": [[(sig1[i] + sig1[j]) / 2 for j in range(nc)] for i in range(nc)],
                              "eps": [[(eps1[i] * eps1[j]) ** 0.5 for j in range(nc)] for i in range(nc)],

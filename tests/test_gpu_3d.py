@@ -182,6 +182,29 @@ def test_converge_3d_mixed_precision():
     assert e_ < 1e-7, e_
 
 
+def test_converge_3d_mixed_falls_back_to_fp64_on_fp32_overflow():
+    """A point whose density the FP32 map cannot hold (exp(99.5) mol/m^3: N_A L^3 rho
+    overflows FP32 downstream) makes the first FP32 map value non-finite; the solve
+    then continues from the last point with a finite FP32 value (here the seed) with
+    the FP64 map, and ends where the FP64 solve ends, bit for bit."""
+    say("3D mixed precision: FP32 overflow at the seed falls back to the FP64 map")
+    from cdft_amd import DFTOptions
+    kw = dict(n=16, box=5.0, nspheres=1, seed=2, T=150.0, x=(1.0,), components=("methane",))
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s, rho0 = configs.matrix_3d(options=DFTOptions(precision=prec), **kw)
+        r = rho0.copy()
+        r[0, 1234] = np.exp(99.5)
+        converge_(s, r, tol=1e-9)
+        out[prec] = (r, s.last_result)
+    (r64, a), (r32, b) = out["fp64"], out["mixed"]
+    say(f"  FP64 errcode {a['errcode']} evals {a['nevals']} | mixed errcode {b['errcode']} evals {b['nevals']} "
+        f"(FP32 {b['nevals_fp32']})")
+    assert a["errcode"] == 0 and b["errcode"] == 0
+    assert b["nevals_fp32"] == 1 and b["nevals"] == a["nevals"] + 1
+    assert np.array_equal(r32, r64)
+
+
 def test_adjoint_3d_adsorption_sensitivity():
     say("3D adjoint: d(adsorption)/d(sigma, eps, m), methane around a hard sphere, 16^3, vs FD of full runs")
     from cdft_amd import adjoint as adj

@@ -255,6 +255,9 @@ def test_batched_mixed_matches_oracle_tight():
 # grids of 80-230 points; 88, 207 and 257 are pores on which the FP64 oracle's Anderson
 # iterate blows up or stalls (reported, not compared)
 SMALL_PORES = [269, 11, 403, 13, 18, 88, 207, 257]
+# pores whose termination (converged or not within maxit) depends on rounding: the FP64
+# oracle and the device part ways on them in either direction
+CHAOTIC_PORES = {88, 207, 257}
 MIXED_PORES = [269, 13, 0, 4, 252, 426]               # 80, 93, 335, 405, 478, 349 points
 
 
@@ -278,9 +281,11 @@ def test_batched_kernels_match_oracle(threads, which, monkeypatch):
         e_ = rel(b.reshape(-1), x) if rr["errcode"] == ro.errcode == 0 else float("nan")
         say(f"  pore {i} N={s.N}: batched {rr['errcode']} evals {rr['nevals']} | oracle {ro.errcode} evals {ro.nevals}"
             f" | profile {e_:.2e}")
-        if ro.errcode == 0:                 # pores the FP64 oracle converges: same result
+        if rr["errcode"] == ro.errcode == 0:  # both converge: the same fixed point
             nok += 1
-            assert rr["errcode"] == 0 and e_ < 1e-10, (i, rr["errcode"], e_)
+            assert e_ < 1e-10, (i, e_)
+        elif i not in CHAOTIC_PORES:          # elsewhere the same termination
+            assert rr["errcode"] == ro.errcode, (i, rr["errcode"], ro.errcode)
     assert nok >= 5
 
 
