@@ -54,7 +54,8 @@ def perturbed_system(system, comp, param, delta):
     r = list(rows[name])
     r[{"segment": 0, "sigma": 1, "epsilon": 2}[param]] += delta
     rows[name] = tuple(r)
-    new = PCSAFT(model.components, kij=model.kij, params=rows)
+    new = PCSAFT(model.components, kij=model.kij, params=rows,
+                 assoc={c: model.assoc_like.get(c) for c in model.components}, combining=model.combining)
     st = system.structure
     p, T = st.conditions
     x = st.rho_bulk / st.rho_bulk.sum()
@@ -124,23 +125,27 @@ def translation_mode(system, rho):
     return np.real(np.fft.ifft(1j * k * np.fft.fft(x, axis=1), axis=1)).reshape(-1)
 
 
-def sensitivities(system, rho, kind, comp=0, params=PARAMS, rel_h=1e-6, tol=1e-10, maxit=10000, beta=1e-2,
-                  anderson_start=1e-1, m=5, deflate=None):
-    """dJ/dtheta for theta = (param, component) pairs through the adjoint.
+def jacobian(system, rho, kind, comps, params=PARAMS, rel_h=1e-6, tol=1e-10, maxit=10000, beta=1e-2,
+             anderson_start=1e-1, m=5, deflate=None):
+    """dJ_c/dtheta for the objectives J_c (kind, component c for c in comps) and every
+    theta = (param, component): one adjoint solve per objective, and one pair of
+    perturbed systems per parameter shared by all objectives.
 
     deflate (default: when the system has no external field) removes from g its
     component along the translation mode, the solvability condition of the
-    adjoint equation for a free interface.  Returns ({(param, i): derivative},
-    lambda, adjoint result)."""
+    adjoint equation for a free interface.  Returns ({(c, param, i): derivative},
+    {c: lambda}, {c: adjoint result})."""
     rho = np.asarray(rho, dtype=np.float64).reshape(-1)
-    g = rho * objective_grad_rho(system, rho, kind, comp)
     if deflate is None:
         deflate = system.vext is None
-    if deflate:
-        t = translation_mode(system, rho)
-        g = g - (g @ t) / (t @ t) * t
-    lam, res = system.device_system.adjoint_solve(rho, g, maxit=maxit, beta=beta, tol=tol,
-                                                  anderson_start=anderson_start, m=m)
+    t = translation_mode(system, rho) if deflate else None
+    lams, results = {}, {}
+    for c in comps:
+        g = rho * objective_grad_rho(system, rho, kind, c)
+        if deflate:
+            g = g - (g @ t) / (t @ t) * t
+        lams[c], results[c] = system.device_system.adjoint_solve(rho, g, maxit=maxit, beta=beta, tol=tol,
+                                                                 anderson_start=anderson_start, m=m)
     out = {}
     rows = model_params(system.model)
     for i, name in enumerate(system.model.components):
@@ -149,7 +154,18 @@ def sensitivities(system, rho, kind, comp=0, params=PARAMS, rel_h=1e-6, tol=1e-1
             h = rel_h * abs(base)
             sp = perturbed_system(system, i, prm, +h)
             sm = perturbed_system(system, i, prm, -h)
-            dJ = (objective(sp, rho, kind, comp) - objective(sm, rho, kind, comp)) / (2 * h)
             dG = (log_map(sp, rho) - log_map(sm, rho)) / (2 * h)
-            out[(prm, i)] = dJ + float(lam @ dG)
-    return out, lam, res
+            for c in comps:
+                dJ = (objective(sp, rho, kind, c) - objective(sm, rho, kind, c)) / (2 * h)
+                out[(c, prm, i)] = dJ + float(lams[c] @ dG)
+            del sp, sm                # release their device systems before the next pair
+    return out, lams, results
+
+
+def sensitivities(system, rho, kind, comp=0, params=PARAMS, rel_h=1e-6, tol=1e-10, maxit=10000, beta=1e-2,
+                  anderson_start=1e-1, m=5, deflate=None):
+    """dJ/dtheta of one objective for theta = (param, component) pairs through the
+    adjoint.  Returns ({(param, i): derivative}, lambda, adjoint result)."""
+    out, lams, results = jacobian(system, rho, kind, [comp], params, rel_h, tol, maxit, beta, anderson_start, m,
+                                  deflate)
+    return {(prm, i): v for (_, prm, i), v in out.items()}, lams[comp], results[comp]

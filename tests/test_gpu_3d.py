@@ -233,3 +233,37 @@ def test_adjoint_3d_adsorption_sensitivity():
         fd = (vals[0] - vals[1]) / (2 * h)
         say(f"  dGamma/d {prm}: adjoint {val:.10e}  FD {fd:.10e}  rel {abs(val - fd) / abs(fd):.2e}")
         assert abs(val - fd) <= 1e-4 * abs(fd), (i, val, fd)
+
+
+def test_adjoint_3d_ternary_adsorption_jacobian():
+    """The matrix workload's sensitivity d(N_i)/d(eps_j) (all nine entries, three
+    adjoint solves) for the ternary methane/ethane/propane at 200 K and 1e7 Pa, on a
+    32^3 grid around two small hard spheres, against central differences of full
+    converge runs (h = 1e-5 relative, tol 1e-11)."""
+    say("3D adjoint: d(N_i)/d(eps_j), ternary around two hard spheres, 32^3, vs FD of full runs")
+    from cdft_amd import adjoint as adj
+    kw_sys = dict(n=32, box=8.0, nspheres=2, seed=3, rmin=0.6, rmax=1.0)
+    s, rho0 = configs.matrix_3d(**kw_sys)
+    kw = dict(tol=1e-11, maxit=5000)
+    rho = rho0.copy()
+    converge_(s, rho, **kw)
+    assert s.last_result["errcode"] == 0, s.last_result
+    rho = rho.reshape(-1)
+    jac, lams, results = adj.jacobian(s, rho, "adsorption", [0, 1, 2], params=("epsilon",), tol=1e-11)
+    for c, r in results.items():
+        say(f"  adjoint for N_{c}: errcode {r['errcode']} evals {r['nevals']}")
+        assert r["errcode"] == 0, (c, r)
+    J = np.array([[jac[(c, "epsilon", j)] for j in range(3)] for c in range(3)])
+    fd = np.zeros((3, 3))
+    for j in range(3):
+        h = 1e-5 * adj.model_params(s.model)[s.model.components[j]][2]
+        ads = []
+        for sg in (+1, -1):
+            sp = adj.perturbed_system(s, j, "epsilon", sg * h)
+            r = rho0.copy()
+            converge_(sp, r, **kw)
+            assert sp.last_result["errcode"] == 0, sp.last_result
+            ads.append(adj.adsorption_system(sp, r))
+        fd[:, j] = (ads[0] - ads[1]) / (2 * h)
+    say(f"  adjoint d(N_i)/d(eps_j):\n{J}\n  finite differences:\n{fd}")
+    np.testing.assert_allclose(J, fd, rtol=1e-4, atol=1e-4 * np.max(np.abs(fd)))
