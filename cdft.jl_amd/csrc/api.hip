@@ -45,52 +45,6 @@ int fail(cdft_ctx *ctx, int code, const std::string &msg)
 
 inline int nblocks(long n, int bs = 256) { return (int)((n + bs - 1) / bs); }
 
-// Householder QR of the m x (m-1) block R[:, 2:m] with LAPACK's conventions
-// (dgeqrf/dlarfg, then the thin Q of dorg2r); R column-major, leading dim m.
-// Used by the Anderson downdate (downdate_aaqr!, src/utils/anderson.jl:552-583).
-void householder_downdate(double *R, int m, double *Qx)
-{
-    int nr = m, ncol = m - 1;
-    double A[CDFT_MAXM * CDFT_MAXM], tau[CDFT_MAXM], v[CDFT_MAXM];
-    for (int j = 0; j < ncol; j++)
-        for (int i = 0; i < nr; i++) A[j * CDFT_MAXM + i] = R[(j + 1) * m + i];
-    for (int j = 0; j < ncol; j++) {
-        double alpha = A[j * CDFT_MAXM + j], xn = 0.0;
-        for (int i = j + 1; i < nr; i++) xn += A[j * CDFT_MAXM + i] * A[j * CDFT_MAXM + i];
-        xn = std::sqrt(xn);
-        if (xn == 0.0) { tau[j] = 0.0; continue; }
-        double beta = -std::copysign(std::hypot(alpha, xn), alpha);
-        tau[j] = (beta - alpha) / beta;
-        double sc = 1.0 / (alpha - beta);
-        for (int i = j + 1; i < nr; i++) A[j * CDFT_MAXM + i] *= sc;
-        A[j * CDFT_MAXM + j] = beta;
-        for (int jj = j + 1; jj < ncol; jj++) {
-            double w = A[jj * CDFT_MAXM + j];
-            for (int i = j + 1; i < nr; i++) w += A[j * CDFT_MAXM + i] * A[jj * CDFT_MAXM + i];
-            w *= tau[j];
-            A[jj * CDFT_MAXM + j] -= w;
-            for (int i = j + 1; i < nr; i++) A[jj * CDFT_MAXM + i] -= w * A[j * CDFT_MAXM + i];
-        }
-    }
-    for (int j = 0; j < ncol; j++)
-        for (int i = 0; i < nr; i++) Qx[j * nr + i] = (i == j) ? 1.0 : 0.0;
-    for (int h = ncol - 1; h >= 0; h--) {
-        if (tau[h] == 0.0) continue;
-        v[h] = 1.0;
-        for (int i = h + 1; i < nr; i++) v[i] = A[h * CDFT_MAXM + i];
-        for (int j = 0; j < ncol; j++) {
-            double w = 0.0;
-            for (int i = h; i < nr; i++) w += v[i] * Qx[j * nr + i];
-            w *= tau[h];
-            for (int i = h; i < nr; i++) Qx[j * nr + i] -= w * v[i];
-        }
-    }
-    // R[1:m-1, 1:m-1] = Rd; R[:, m] = 0
-    for (int j = 0; j < ncol; j++)
-        for (int i = 0; i < ncol; i++) R[j * m + i] = (i <= j) ? A[j * CDFT_MAXM + i] : 0.0;
-    for (int i = 0; i < m; i++) R[(m - 1) * m + i] = 0.0;
-}
-
 // Composite Simpson exactly as _int (src/utils/integrals.jl:28-40), 1D.
 double simpson1d(const double *f, int n, double dz)
 {
@@ -110,6 +64,9 @@ void free_sys_buffers(cdft_sys *s)
     for (double **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (s->H64) { (void)hipFree(s->H64); s->H64 = nullptr; }
+    if (s->aa_exec) { (void)hipGraphExecDestroy(s->aa_exec); s->aa_exec = nullptr; }
+    if (s->ctl) { (void)hipFree(s->ctl); s->ctl = nullptr; }
+    if (s->dhist) { (void)hipFree(s->dhist); s->dhist = nullptr; s->dhist_cap = 0; }
     double **b3[] = {&s->G3, &s->A3, &s->B3, &s->S3, &s->B3x[0], &s->B3x[1], &s->B3x[2]};
     for (double **b : b3)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
@@ -171,6 +128,7 @@ template <typename T, int AX, int IN, int OUT>
 void fft3_pass(const cdft_sys *s, Pass3 p, bool inverse)
 {
     p.inverse = inverse ? 1 : 0;
+    p.gate = s->gate;
     p.tw = reinterpret_cast<const cx<double> *>(s->ctx->twiddles.at(s->g3.n[AX]));
     static_assert(TB3 == 8, "EPT = L TB3 / 1024 below");
     switch (s->g3.n[AX]) {
@@ -237,7 +195,9 @@ void zpairs_launch_ept(const cdft_sys *s, const ZPairs3 &p)
         attr = true;
     }
     const long nblk = (long)(G.n[0] / TBZ) * G.n[1];
-    hipLaunchKernelGGL((k_fft3z_accum<T, EPT>), dim3((unsigned)nblk), dim3(256), lds, s->ctx->stream, G, p);
+    ZPairs3 pg = p;
+    pg.gate = s->gate;
+    hipLaunchKernelGGL((k_fft3z_accum<T, EPT>), dim3((unsigned)nblk), dim3(256), lds, s->ctx->stream, G, pg);
 }
 
 template <typename T>
@@ -339,7 +299,7 @@ template <int NC, int AS>
 void launch_phi32_as(cdft_sys *s)
 {
     hipLaunchKernelGGL((k_phi32<NC, 3, AS>), dim3(nblocks(s->N)), dim3(256), 0, s->ctx->stream, s->rho, s->n32,
-                       s->dphi32, s->N, s->NA, s->pc);
+                       s->dphi32, s->N, s->NA, s->pc, s->gate);
 }
 
 template <int NC>
@@ -372,10 +332,10 @@ void launch_phi_as(cdft_sys *s, hipStream_t st)
 {
     if (s->nd == 3)
         hipLaunchKernelGGL((k_phi<NC, 3, AS>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, s->n, s->dphi, s->fval,
-                           s->N, s->NA, s->pc);
+                           s->N, s->NA, s->pc, s->gate);
     else
         hipLaunchKernelGGL((k_phi<NC, 1, AS>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, s->n, s->dphi, s->fval,
-                           s->N, s->NA, s->pc);
+                           s->N, s->NA, s->pc, s->gate);
 }
 
 // s->assoc selects the association variant: 0 none, 1 one pair (closed form), 2 general.
@@ -410,7 +370,7 @@ void launch_transposed(cdft_sys *s)
     }
     int N = s->N, F = s->F;
     hipLaunchKernelGGL(k_circ1d<double>, dim3((N + CIRC_TW - 1) / CIRC_TW, s->nc * (F - 1)), dim3(256), 0, s->ctx->stream, s->dphi,
-                       (long)F * N, (long)N, s->h, s->part, (long)(F - 1) * N, (long)N, 1, N, F, 1.0, 1);
+                       (long)F * N, (long)N, s->h, s->part, (long)(F - 1) * N, (long)N, 1, N, F, 1.0, 1, s->gate);
 }
 
 // Weighted densities of rho (s->rho) into s->n.
@@ -422,7 +382,7 @@ void launch_weighted(cdft_sys *s)
     }
     int N = s->N, F = s->F;
     hipLaunchKernelGGL(k_circ1d<double>, dim3((N + CIRC_TW - 1) / CIRC_TW, s->nc * (F - 1)), dim3(256), 0, s->ctx->stream, s->rho,
-                       (long)N, 0L, s->h, s->n, (long)F * N, (long)N, 0, N, F, s->NA, 0);
+                       (long)N, 0L, s->h, s->n, (long)F * N, (long)N, 0, N, F, s->NA, 0, s->gate);
 }
 
 // Weighted densities, Phi + gradient and transposed convolutions into s->dphi / s->part.
@@ -438,10 +398,10 @@ void launch_phi_hvp_as(cdft_sys *s, const double *v, hipStream_t st)
 {
     if (s->nd == 3)
         hipLaunchKernelGGL((k_phi_hvp<NC, 3, AS>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, v, s->n, s->dphi,
-                           s->N, s->NA, s->pc);
+                           s->N, s->NA, s->pc, s->gate);
     else
         hipLaunchKernelGGL((k_phi_hvp<NC, 1, AS>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, v, s->n, s->dphi,
-                           s->N, s->NA, s->pc);
+                           s->N, s->NA, s->pc, s->gate);
 }
 
 template <int NC>
@@ -465,7 +425,7 @@ void launch_hvp(cdft_sys *s, const double *v)
     } else {
         dim3 g((N + CIRC_TW - 1) / CIRC_TW, nc * (F - 1));
         hipLaunchKernelGGL(k_circ1d<double>, g, dim3(256), 0, st, v, (long)N, 0L, s->h, s->dphi, (long)F * N,
-                           (long)N, 0, N, F, s->NA, 0);
+                           (long)N, 0, N, F, s->NA, 0, s->gate);
     }
     switch (nc) {
     case 1: launch_phi_hvp<1>(s, v, st); break;
@@ -781,7 +741,7 @@ int cdft_functional_eval(cdft_sys *s, const double *rho, double *dfdrho, double 
     hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part,
                        (const double *)nullptr, s->N, s->nc, upd_src(s).F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
                        (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
-                       (const double *)nullptr, (const double *)nullptr);
+                       (const double *)nullptr, (const double *)nullptr, (const AACtl *)nullptr);
     HIPCHECK_CTX(ctx, hipGetLastError());
     HIPCHECK_CTX(ctx, hipMemcpyAsync(dfdrho, s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     std::vector<double> fv;
@@ -807,10 +767,32 @@ struct AdjMap {
     double *vbuf;              // mask lambda, the direction of the next product
 };
 
-// aasol (src/utils/anderson.jl:301-443) on the device: the fixed-point map and
-// every N-length vector operation run on the device; the m x m QR downdate and
-// the triangular solve run on the host, as in the reference.  adj == nullptr: the
-// Euler-Lagrange map of converge! on x = log rho; otherwise the adjoint map on x.
+// Bodies per graph launch, i.e. per host check of the solver state (CDFT_AA_CHUNK,
+// default 16).  Bodies enqueued after the end of a solve find every gate set and
+// cost only their launches.
+static int aa_chunk()
+{
+    const char *e = std::getenv("CDFT_AA_CHUNK");
+    const int v = e ? std::atoi(e) : 16;
+    return std::max(1, std::min(v, 256));
+}
+
+// CDFT_AA_GRAPH=0 enqueues the same bodies one launch at a time instead of
+// replaying a captured graph (same kernels, same results).
+static bool aa_use_graph()
+{
+    const char *e = std::getenv("CDFT_AA_GRAPH");
+    return !(e && e[0] == '0');
+}
+
+// aasol (src/utils/anderson.jl:301-443) entirely on the device: every N-length
+// operation and the m x m QR downdate and triangular solve run in kernels, the
+// solver's control flow in one thread of the kernels that finish the reductions
+// (k_aa_fin: aa_mid, aa_post, aa_pre), twelve launches per body in 1D.  The host
+// enqueues bodies of one map evaluation each — replayed from a hipGraph of
+// aa_chunk() bodies captured once per system and kind of map — and reads the
+// solver state once per chunk.  adj == nullptr: the Euler-Lagrange map of
+// converge! on x = log rho; otherwise the adjoint map on x.
 static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdft_result *r, const AdjMap *adj)
 {
     if (!s || !rho_inout || !o || !r) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_converge: NULL argument");
@@ -823,61 +805,46 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     hipStream_t st = ctx->stream;
     const long n = (long)s->N * s->nc;
     const int nb = s->nblk;
-    const int maxit = o->maxit, picard_maxit = o->maxit;
-    const double rtol = o->tol, atol = o->tol, picard_beta = beta;
-    const double picard_rtol = o->anderson_start, picard_atol = o->anderson_start;
     HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
+    if (!s->ctl) HIPCHECK_CTX(ctx, hipMalloc((void **)&s->ctl, sizeof(AACtl)));
+    const int hcap = (o->hist && o->hist_cap > 0) ? o->hist_cap : 0;
+    if (hcap > s->dhist_cap) {
+        if (s->dhist) HIPCHECK_CTX(ctx, hipFree(s->dhist));
+        s->dhist = nullptr;
+        s->dhist_cap = 0;
+        HIPCHECK_CTX(ctx, hipMalloc((void **)&s->dhist, sizeof(double) * hcap));
+        s->dhist_cap = hcap;
+    }
+    AACtl *hc = nullptr;
+    HIPCHECK_CTX(ctx, hipHostMalloc((void **)&hc, sizeof(AACtl)));
+    struct HostFree { AACtl *p; ~HostFree() { if (p) (void)hipHostFree(p); } } hf{hc};
 
-    double *fin_h = nullptr;
-    HIPCHECK_CTX(ctx, hipHostMalloc((void **)&fin_h, sizeof(double) * 4 * CDFT_NQ));
-    struct HostFree { double *p; ~HostFree() { if (p) (void)hipHostFree(p); } } hf{fin_h};
-    int nh = 0, nevals = 0, nevals32 = 0;
-    auto hist = [&](double v) { if (o->hist && nh < o->hist_cap) o->hist[nh] = v; nh++; };
     // Mixed precision (3D): the FP32 map until the residual reaches the FP32 noise
-    // floor (switch_factor tol, or no halving in stall_window iterations), then the
-    // FP64 map with a fresh Anderson history, as in the batched path.
-    bool fp32 = (!adj && s->precision == CDFT_MIXED && s->nd == 3);
-    const double switch_factor = 4.0;
-    const int stall_window = 40;
-    double best = 0.0;
-    int since_best = 0;
-    // A non-finite FP32 map value (a step the FP32 range cannot hold, such as
-    // densities near exp(100) where the exponent clamps) ends the FP32 phase: the
-    // solve goes back to the last point whose FP32 value was finite and continues
-    // with the FP64 map.  (The FP64 map keeps the reference's NaN semantics.)
-    bool fp32_failed = false;
-    auto keep_safe = [&](double r) -> int {     // s->sol had a finite FP32 value
-        if (fp32 && std::isfinite(r))
-            HIPCHECK_CTX(ctx, hipMemcpyAsync(s->xsafe, s->sol, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-        return CDFT_OK;
-    };
-    auto restore_safe = [&]() {                 // back to the last point with a finite FP32 value
-        hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->xsafe, s->sol, s->rho, (int)n,
-                           (const double *)nullptr);
-    };
-#define KEEP_SAFE(r) do { int rc_ = keep_safe(r); if (rc_ != CDFT_OK) return rc_; } while (0)
+    // floor (4 tol, or no halving in 40 iterations), then the FP64 map with a fresh
+    // Anderson history; the reported residual and termination are the FP64 ones.
+    const bool mixed = (!adj && s->precision == CDFT_MIXED && s->nd == 3);
+    std::memset(hc, 0, sizeof(AACtl));
+    hc->m = m;
+    hc->maxit = o->maxit;
+    hc->picard_maxit = o->maxit;
+    hc->hist_cap = hcap;
+    hc->adjoint = adj ? 1 : 0;
+    hc->rtol = hc->atol = o->tol;
+    hc->picard_rtol = hc->picard_atol = o->anderson_start;
+    hc->beta = hc->picard_beta = beta;
+    hc->fp32 = mixed ? 1 : 0;
+    hc->phase = AA_FIRST;
+    hc->umode = 1;
+    hc->beta_u = beta;
+    hc->g_proj = 1;
+    hc->g_f64 = hc->fp32;
+    hc->g_f32 = !hc->fp32;
+    hc->evfp32 = hc->fp32;
+    for (int j = 0; j < CDFT_MAXM; j++) hc->ring[j] = j;
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(s->ctl, hc, sizeof(AACtl), hipMemcpyHostToDevice, st));
 
-    // G evaluation + post-processing; returns the finalized reductions in fin_h[0..2]
     double *ptgt = adj ? adj->vbuf : s->rho;              // where a new point is expanded
     const double *pmask = adj ? adj->mask : nullptr;
-    auto evalG = [&](int mode, double b) -> int {
-        if (adj) launch_hvp(s, adj->vbuf);
-        else if (fp32) launch_functional32(s);
-        else launch_functional(s);
-        if (fp32) nevals32++;
-        hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part, s->vext, s->N, s->nc,
-                           upd_src(s).F,
-                           s->cpd, s->out, s->sol, s->gx, s->res, s->df, s->dg, b, mode, s->red, nb,
-                           adj ? adj->g : nullptr, adj ? (const double *)s->rho : nullptr);
-        hipLaunchKernelGGL(k_finalize, dim3(3), dim3(256), 0, st, s->red, nb, s->fin);
-        HIPCHECK_CTX(ctx, hipGetLastError());
-        HIPCHECK_CTX(ctx, hipMemcpyAsync(fin_h, s->fin, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
-        HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
-        nevals++;
-        return CDFT_OK;
-    };
-#define EVAL(mode, b) do { int rc_ = evalG((mode), (b)); if (rc_ != CDFT_OK) return rc_; } while (0)
-
     if (adj) {
         HIPCHECK_CTX(ctx, hipMemcpyAsync(s->sol, rho_inout, sizeof(double) * n, hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->sol, s->sol, ptgt, (int)n, pmask);
@@ -886,193 +853,106 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
         hipLaunchKernelGGL(k_init_x, dim3(nb), dim3(256), 0, st, s->out, s->sol, s->rho, (int)n);
     }
 
+    AACtl *c = s->ctl;
+    auto body = [&]() {
+        hipLaunchKernelGGL(k_aa_proj1, dim3(nb), dim3(256), 0, st, (int)n, (const double *)s->Q, s->Qalt,
+                           (const AACtl *)c, (const double *)s->df, (const double *)s->res, s->red, nb);
+        hipLaunchKernelGGL(k_aa_fin<0>, dim3(2 * CDFT_MAXM), dim3(256), 0, st, (const double *)s->red, nb, s->fin, c, 0,
+                           s->fin, s->dhist);
+        hipLaunchKernelGGL(k_aa_proj2, dim3(nb), dim3(256), 0, st, (int)n, (const double *)s->Q,
+                           (const double *)s->Qalt, (const AACtl *)c, s->df, (const double *)s->res,
+                           (const double *)s->fin, 0, s->red, nb);
+        hipLaunchKernelGGL(k_aa_fin<0>, dim3(CDFT_MAXM), dim3(256), 0, st, (const double *)s->red, nb,
+                           s->fin + CDFT_NQ, c, 0, s->fin, s->dhist);
+        hipLaunchKernelGGL(k_aa_proj2, dim3(nb), dim3(256), 0, st, (int)n, (const double *)s->Q,
+                           (const double *)s->Qalt, (const AACtl *)c, s->df, (const double *)s->res,
+                           (const double *)(s->fin + CDFT_NQ), 1, s->red, nb);
+        hipLaunchKernelGGL(k_aa_fin<1>, dim3(2), dim3(256), 0, st, (const double *)s->red, nb, s->fin + 2 * CDFT_NQ, c, 1,
+                           s->fin, s->dhist);
+        hipLaunchKernelGGL(k_aa_point, dim3(nb), dim3(256), 0, st, (int)n, (const AACtl *)c, (const double *)s->df,
+                           s->Q, s->Qalt, s->sol, (const double *)s->gx, (const double *)s->dg, s->DG, s->xsafe, ptgt,
+                           pmask);
+        s->gate = &c->g_f64;
+        if (adj) launch_hvp(s, adj->vbuf);
+        else launch_functional(s);
+        if (mixed) {
+            s->gate = &c->g_f32;
+            launch_functional32(s);
+        }
+        s->gate = nullptr;
+        hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part, s->vext, s->N, s->nc,
+                           upd_src(s).F, s->cpd, s->out, s->sol, s->gx, s->res, s->df, s->dg, beta, 2, s->red, nb,
+                           adj ? adj->g : nullptr, adj ? (const double *)s->rho : nullptr, (const AACtl *)c);
+        hipLaunchKernelGGL(k_aa_fin<2>, dim3(3), dim3(256), 0, st, (const double *)s->red, nb, s->fin, c, 2, s->fin,
+                           s->dhist);
+    };
+
+    const bool use_graph = aa_use_graph();
+    const int K = aa_chunk();
+    const void *key[5] = {adj ? (const void *)adj->g : nullptr, adj ? (const void *)adj->mask : nullptr,
+                          adj ? (const void *)adj->vbuf : nullptr, (const void *)s->dhist, (const void *)(intptr_t)K};
+    // every path through aasol takes at most 2 evaluations per iteration counted in
+    // maxit + picard_maxit, plus the first ones at the seed and at the switch
+    const long bound = 4L * (2L * o->maxit + 4) + 2L * K;
+
     HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev0, st));
-    int k = 0;
-    EVAL(1, beta);
-    double resnorm = std::sqrt(fin_h[0]);
-    if (fp32 && !std::isfinite(resnorm)) {      // the seed itself: FP64 from the start
-        fp32 = false;
-        EVAL(1, beta);
-        resnorm = std::sqrt(fin_h[0]);
-    }
-    KEEP_SAFE(resnorm);
-    const double up_bd = 1.e4 * resnorm;
-    const double tol = rtol * resnorm + atol;
-    const double picard_tol = picard_rtol * resnorm + picard_atol;
-    hist(resnorm);
-    bool toosoon = (resnorm <= tol);
-
-    if (!toosoon && picard_maxit > 0) {
-        hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, ptgt, (int)n, pmask);
-        for (int it = 0; it < picard_maxit - 1; it++) {
-            EVAL(1, picard_beta);
-            resnorm = std::sqrt(fin_h[0]);
-            if (fp32 && !std::isfinite(resnorm)) {
-                hist(resnorm);
-                restore_safe();
-                fp32 = false;
-                EVAL(1, picard_beta);
-                resnorm = std::sqrt(fin_h[0]);
+    body();                                       // eager first body
+    HIPCHECK_CTX(ctx, hipGetLastError());
+    long bodies = 1;
+    while (true) {
+        HIPCHECK_CTX(ctx, hipMemcpyAsync(hc, c, sizeof(AACtl), hipMemcpyDeviceToHost, st));
+        HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+        if (hc->phase == AA_DONE) break;
+        if (bodies > bound) return fail(ctx, CDFT_ERR_HIP, "aasol: the device-side solver did not terminate");
+        if (use_graph) {
+            bool same = s->aa_exec != nullptr;
+            for (int j = 0; j < 5 && same; j++) same = (s->aa_key[j] == key[j]);
+            if (!same) {
+                if (s->aa_exec) { (void)hipGraphExecDestroy(s->aa_exec); s->aa_exec = nullptr; }
+                hipGraph_t g = nullptr;
+                HIPCHECK_CTX(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+                for (int b = 0; b < K; b++) body();
+                hipError_t e1 = hipGetLastError();
+                hipError_t e2 = hipStreamEndCapture(st, &g);
+                if (e1 != hipSuccess || e2 != hipSuccess) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return fail(ctx, CDFT_ERR_HIP, std::string("aasol graph capture: ") +
+                                                       hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+                }
+                hipError_t e3 = hipGraphInstantiate(&s->aa_exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e3 != hipSuccess) {
+                    s->aa_exec = nullptr;
+                    return fail(ctx, CDFT_ERR_HIP, std::string("aasol graph instantiate: ") + hipGetErrorString(e3));
+                }
+                for (int j = 0; j < 5; j++) s->aa_key[j] = key[j];
             }
-            KEEP_SAFE(resnorm);
-            hist(resnorm);
-            if (resnorm <= picard_tol || resnorm >= up_bd) break;
-            hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, ptgt, (int)n, pmask);
-        }
-        toosoon = (resnorm <= tol);
-    }
-
-    double R[CDFT_MAXM * CDFT_MAXM] = {0};
-    int ring[CDFT_MAXM];
-    for (int j = 0; j < CDFT_MAXM; j++) ring[j] = j;
-    double dfn2 = 0.0, dfres = 0.0;
-    int ka = 0;                                   // Anderson steps since the (re)start
-    if (!toosoon) {
-        hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, ptgt, (int)n, pmask);
-        k += 1;
-        ka = 1;
-        EVAL(2, beta);
-        resnorm = std::sqrt(fin_h[0]);
-        dfn2 = fin_h[1];
-        dfres = fin_h[2];
-        hist(resnorm);
-        if (fp32 && !std::isfinite(resnorm)) {
-            restore_safe();
-            fp32_failed = true;
-            resnorm = 0.0;                       // the switch at the top of the loop takes over
-        }
-        KEEP_SAFE(resnorm);
-        best = resnorm;
-    }
-
-    double *Qc = s->Q, *Qa = s->Qalt;
-    while (k < maxit + picard_maxit && (resnorm > tol || fp32) && !toosoon && resnorm < up_bd) {
-        if (fp32) {
-            if (resnorm < 0.5 * best) {
-                best = resnorm;
-                since_best = 0;
-            } else {
-                since_best++;
-            }
-            if (resnorm <= switch_factor * tol || since_best >= stall_window || fp32_failed) {
-                fp32 = false;                     // the FP64 map from here
-                EVAL(1, beta);
-                resnorm = std::sqrt(fin_h[0]);
-                hist(resnorm);
-                if (resnorm <= tol) break;
-                hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, ptgt, (int)n, pmask);
-                k += 1;
-                ka = 1;
-                EVAL(2, beta);
-                resnorm = std::sqrt(fin_h[0]);
-                dfn2 = fin_h[1];
-                dfres = fin_h[2];
-                hist(resnorm);
-                std::memset(R, 0, sizeof(R));
-                for (int j = 0; j < CDFT_MAXM; j++) ring[j] = j;
-                continue;
-            }
-        }
-        if (m == 0) {
-            hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->gx, s->sol, ptgt, (int)n, pmask);
+            HIPCHECK_CTX(ctx, hipGraphLaunch(s->aa_exec, st));
         } else {
-            // BuildDG!(DG, m, k+1, dg) with a ring of physical columns
-            int kk = ka + 1, slot;
-            if (m == 1) {
-                slot = ring[0];
-            } else if (kk > m + 1) {
-                int first = ring[0];
-                for (int j = 0; j < m - 1; j++) ring[j] = ring[j + 1];
-                ring[m - 1] = first;
-                slot = ring[m - 1];
-            } else {
-                slot = ring[kk - 2];
-            }
-            HIPCHECK_CTX(ctx, hipMemcpyAsync(s->DG + (size_t)slot * n, s->dg, sizeof(double) * n,
-                                             hipMemcpyDeviceToDevice, st));
-            // aa_qr_update!(Q, R, df, m, k-1, Qd)
-            int kq = ka - 1, qcol;
-            double nrm, tres[CDFT_MAXM];
-            if (kq == 0) {
-                nrm = std::sqrt(dfn2);
-                R[0] = nrm;
-                qcol = 0;
-                tres[0] = dfres / nrm;
-            } else {
-                AAMat qx;
-                std::memset(&qx, 0, sizeof(qx));
-                int down = (kq > m - 1);
-                if (down) householder_downdate(R, m, qx.a);
-                int kqq = std::min(kq, m - 1);
-                hipLaunchKernelGGL(k_aa_proj1, dim3(nb), dim3(256), 0, st, (int)n, Qc, Qa, qx, down, m, kqq, s->df,
-                                   s->res, s->red, nb);
-                hipLaunchKernelGGL(k_finalize, dim3(2 * CDFT_MAXM), dim3(256), 0, st, s->red, nb, s->fin);
-                if (down) std::swap(Qc, Qa);
-                hipLaunchKernelGGL(k_aa_proj2, dim3(nb), dim3(256), 0, st, (int)n, Qc, kqq, s->df, s->res, s->fin, 0,
-                                   s->red, nb);
-                hipLaunchKernelGGL(k_finalize, dim3(CDFT_MAXM), dim3(256), 0, st, s->red, nb, s->fin + CDFT_NQ);
-                hipLaunchKernelGGL(k_aa_proj2, dim3(nb), dim3(256), 0, st, (int)n, Qc, kqq, s->df, s->res,
-                                   s->fin + CDFT_NQ, 1, s->red, nb);
-                hipLaunchKernelGGL(k_finalize, dim3(2), dim3(256), 0, st, s->red, nb, s->fin + 2 * CDFT_NQ);
-                HIPCHECK_CTX(ctx, hipGetLastError());
-                HIPCHECK_CTX(ctx, hipMemcpyAsync(fin_h, s->fin, sizeof(double) * 3 * CDFT_NQ, hipMemcpyDeviceToHost, st));
-                HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
-                const double *h1 = fin_h, *tq = fin_h + CDFT_MAXM, *h2 = fin_h + CDFT_NQ;
-                for (int j = 0; j < kqq; j++) R[kqq * m + j] = h1[j] + h2[j];
-                nrm = std::sqrt(fin_h[2 * CDFT_NQ]);
-                R[kqq * m + kqq] = nrm;
-                qcol = kqq;
-                for (int j = 0; j < kqq; j++) tres[j] = tq[j];
-                tres[kqq] = fin_h[2 * CDFT_NQ + 1] / nrm;
-            }
-            int mk = std::min(m, ka);
-            AAMat th;
-            std::memset(&th, 0, sizeof(th));
-            for (int j = mk - 1; j >= 0; j--) {          // theta = RA \ tres
-                double acc = tres[j];
-                for (int l = j + 1; l < mk; l++) acc -= R[l * m + j] * th.a[l];
-                th.a[j] = acc / R[j * m + j];
-            }
-            for (int j = 0; j < mk; j++) th.col[j] = ring[j];
-            hipLaunchKernelGGL(k_aa_newpoint, dim3(nb), dim3(256), 0, st, (int)n, s->df, nrm,
-                               Qc + (size_t)qcol * n, s->sol, s->gx, s->DG, th, mk, ptgt, pmask);
+            for (int b = 0; b < K; b++) body();
+            HIPCHECK_CTX(ctx, hipGetLastError());
         }
-        k += 1;
-        ka += 1;
-        EVAL(2, beta);
-        resnorm = std::sqrt(fin_h[0]);
-        dfn2 = fin_h[1];
-        dfres = fin_h[2];
-        hist(resnorm);
-        if (fp32 && !std::isfinite(resnorm)) {
-            restore_safe();
-            fp32_failed = true;
-            resnorm = 0.0;
-        }
-        KEEP_SAFE(resnorm);
+        bodies += K;
     }
-#undef EVAL
-#undef KEEP_SAFE
-
-    int idid, errcode;
-    if (toosoon) { idid = 1; errcode = -1; }
-    else if (resnorm >= up_bd) { idid = 0; errcode = -2; }
-    else if (resnorm <= tol && !fp32) { idid = 1; errcode = 0; }
-    else { idid = 0; errcode = 10; }
-
+    // a restore left pending by the last step (back to the last finite FP32 point)
+    hipLaunchKernelGGL(k_aa_point, dim3(nb), dim3(256), 0, st, (int)n, (const AACtl *)c, (const double *)s->df, s->Q,
+                       s->Qalt, s->sol, (const double *)s->gx, (const double *)s->dg, s->DG, s->xsafe, ptgt, pmask);
     if (!adj) hipLaunchKernelGGL(k_exp, dim3(nb), dim3(256), 0, st, s->sol, s->out, (int)n);
     HIPCHECK_CTX(ctx, hipGetLastError());
     HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev1, st));
     HIPCHECK_CTX(ctx, hipMemcpyAsync(rho_inout, adj ? s->sol : s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    const int nh = hc->nh;
+    if (hcap > 0 && nh > 0)
+        HIPCHECK_CTX(ctx, hipMemcpyAsync(o->hist, s->dhist, sizeof(double) * std::min(nh, hcap), hipMemcpyDeviceToHost, st));
     HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
-    r->idid = idid;
-    r->errcode = errcode;
-    r->iters = k;
-    r->nevals = nevals;
+    r->idid = hc->idid;
+    r->errcode = hc->errcode;
+    r->iters = hc->k;
+    r->nevals = hc->nevals;
     r->nhist = nh;
-    r->resnorm = resnorm;
-    r->tol = tol;
-    r->nevals_fp32 = nevals32;
+    r->resnorm = hc->r;
+    r->tol = hc->tol;
+    r->nevals_fp32 = hc->nevals32;
     r->reserved = 0;
     float ms = 0.0f;
     HIPCHECK_CTX(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -1121,7 +1001,7 @@ int cdft_phi_vjp(cdft_sys *s, const double *rho, const double *cot, double *out)
     hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part,
                        (const double *)nullptr, s->N, s->nc, upd_src(s).F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
                        (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
-                       (const double *)nullptr, (const double *)nullptr);
+                       (const double *)nullptr, (const double *)nullptr, (const AACtl *)nullptr);
     HIPCHECK_CTX(ctx, hipGetLastError());
     HIPCHECK_CTX(ctx, hipMemcpyAsync(out, s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
@@ -1145,7 +1025,7 @@ int cdft_hvp(cdft_sys *s, const double *rho, const double *v, double *out)
     hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part,
                        (const double *)nullptr, s->N, s->nc, upd_src(s).F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
                        (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
-                       (const double *)nullptr, (const double *)nullptr);
+                       (const double *)nullptr, (const double *)nullptr, (const AACtl *)nullptr);
     HIPCHECK_CTX(ctx, hipGetLastError());
     HIPCHECK_CTX(ctx, hipMemcpyAsync(out, s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
@@ -1173,7 +1053,7 @@ int cdft_adjoint_solve(cdft_sys *s, const double *rho, const double *g, double *
     hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part,
                        (const double *)nullptr, s->N, s->nc, upd_src(s).F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
                        (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
-                       (const double *)nullptr, (const double *)nullptr);
+                       (const double *)nullptr, (const double *)nullptr, (const AACtl *)nullptr);
     hipLaunchKernelGGL(k_clamp_mask, dim3(nblocks(n)), dim3(256), 0, st, s->out, s->vext, s->cpd, s->N, s->nc, maskd);
     HIPCHECK_CTX(ctx, hipGetLastError());
     AdjMap am{gd, maskd, vbuf};
@@ -1198,7 +1078,7 @@ int cdft_ddft_rhs(cdft_sys *s, const double *eta, double *deta)
     hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part,
                        (const double *)nullptr, s->N, s->nc, upd_src(s).F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
                        (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
-                       (const double *)nullptr, (const double *)nullptr);
+                       (const double *)nullptr, (const double *)nullptr, (const AACtl *)nullptr);
     if (s->nd == 1) {
         if (!s->hddft) {
             HIPCHECK_CTX(ctx, hipMalloc((void **)&s->hddft, sizeof(double) * 4 * n));
@@ -1212,7 +1092,8 @@ int cdft_ddft_rhs(cdft_sys *s, const double *eta, double *deta)
         hipLaunchKernelGGL(k_ddft_src, dim3(nblocks(n)), dim3(256), 0, st, etad, s->out, s->vext, N, nc, src,
                            (double *)nullptr);
         hipLaunchKernelGGL(k_circ1d<double>, dim3((N + CIRC_TW - 1) / CIRC_TW, nc * 3), dim3(256), 0, st, src,
-                           (long)4 * N, (long)N, s->hddft, d, (long)3 * N, (long)N, 1, N, 4, 1.0, 0);
+                           (long)4 * N, (long)N, s->hddft, d, (long)3 * N, (long)N, 1, N, 4, 1.0, 0,
+                           (const int *)nullptr);
         hipLaunchKernelGGL(k_ddft_combine, dim3(nblocks(n)), dim3(256), 0, st, d, N, nc, detad);
     } else {
         double *mu = nullptr;

@@ -14,6 +14,8 @@
 #define CDFT_MAXM 8          // largest Anderson depth supported
 #define CDFT_NQ 16           // reduction slots per vector pass
 
+#include "kernels.h"
+
 namespace cdft {
 // clamp!(ln_Gx, -100, 100) (src/methods/converge.jl:150) with Julia's clamp
 // semantics: a NaN stays NaN (fmin/fmax would replace it by a bound), so a
@@ -91,6 +93,14 @@ struct cdft_sys {
     double *red = nullptr;     // block partials
     double *fin = nullptr;     // finalized reductions [CDFT_NQ]
     int nblk = 0;
+    // device-side Anderson solve: its state, residual history, and the captured
+    // graph of solver bodies with the pointers it was captured for
+    AACtl *ctl = nullptr;
+    double *dhist = nullptr;
+    int dhist_cap = 0;
+    hipGraphExec_t aa_exec = nullptr;
+    const void *aa_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    const int *gate = nullptr;  // gate of the kernels being enqueued (null: always run)
 };
 
 #define HIPCHECK_CTX(ctx, call)                                                              \

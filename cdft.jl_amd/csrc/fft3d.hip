@@ -51,6 +51,7 @@ struct Pass3 {
     int inverse;
     double scale;             // OUT_FIELDS / OUT_REAL / OUT_DDFT*
     int mkind;                // IN_MULT: 0 the Laplacian map, 1 + a the gradient map along axis a
+    const int *gate;          // the pass returns at once when *gate is nonzero (device-side solver)
 };
 
 // The DDFT derivative maps at grid index g (ext/SciMLBasecDFTExt.jl:16-19): the
@@ -251,6 +252,7 @@ __device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const c
 template <typename T, int AX, int IN, int OUT, int EPT>
 __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
 {
+    if (p.gate && *p.gate) return;
     constexpr int TB = TB3;
     constexpr int EPR = EPT > 0 ? EPT : 1;     // radix-4 butterflies per thread and pass
     extern __shared__ __align__(16) unsigned char smem3[];
@@ -370,6 +372,7 @@ struct ZPairs3 {
     int v1[4], v2[4];         // 1 where the field is a vector weight (H = i G)
     int np;
     const cx<double> *tw;     // W_L^t along z
+    const int *gate;          // as Pass3::gate
 };
 
 template <typename T>
@@ -386,6 +389,7 @@ constexpr int TBZ = 8;
 template <typename T, int EPT>
 __global__ __launch_bounds__(256) void k_fft3z_accum(Grid3 G, ZPairs3 p)
 {
+    if (p.gate && *p.gate) return;
     static_assert(EPT > 0, "full tiles only");
     constexpr int TB = TBZ, EL = 4 * EPT;             // L = 128 EPT
     extern __shared__ __align__(16) unsigned char smem3[];

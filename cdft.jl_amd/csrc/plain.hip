@@ -148,8 +148,10 @@ constexpr int CIRC_TW = 64;
 template <typename T>
 __global__ __launch_bounds__(256) void k_circ1d(const T *__restrict__ src, long s_cstride, long s_fstride,
                                                 const T *__restrict__ h, T *__restrict__ dst, long d_cstride,
-                                                long d_fstride, int d_fofs, int N, int F, T scale, int vec_sign)
+                                                long d_fstride, int d_fofs, int N, int F, T scale, int vec_sign,
+                                                const int *gate)
 {
+    if (gate && *gate) return;
     constexpr int TW = CIRC_TW, NG = 4, KC = 256;
     __shared__ T hs[NG][KC];
     __shared__ T ws[NG][TW + KC];
@@ -203,8 +205,9 @@ __global__ __launch_bounds__(256) void k_circ1d(const T *__restrict__ src, long 
 template <int NC, int ND, int AS = 0>
 __global__ __launch_bounds__(256) void k_phi(const double *__restrict__ rho, const double *__restrict__ n,
                                              double *__restrict__ dphi, double *__restrict__ fval, int N,
-                                             double NA, PhiConst P)
+                                             double NA, PhiConst P, const int *gate)
 {
+    if (gate && *gate) return;
     constexpr int F = 6 + ND;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -228,8 +231,10 @@ __global__ __launch_bounds__(256) void k_phi(const double *__restrict__ rho, con
 // lambda-slot gradient is 0 where FP32 does not resolve lambda.
 template <int NC, int ND, int AS = 0>
 __global__ __launch_bounds__(256) void k_phi32(const double *__restrict__ rho, const float *__restrict__ n,
-                                               float *__restrict__ dphi, int N, double NA, PhiConst P)
+                                               float *__restrict__ dphi, int N, double NA, PhiConst P,
+                                               const int *gate)
 {
+    if (gate && *gate) return;
     constexpr int F = 6 + ND;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -255,8 +260,9 @@ __global__ __launch_bounds__(256) void k_phi32(const double *__restrict__ rho, c
 template <int NC, int ND, int AS = 0>
 __global__ __launch_bounds__(256) void k_phi_hvp(const double *__restrict__ rho, const double *__restrict__ v,
                                                  const double *__restrict__ n, double *__restrict__ ndio, int N,
-                                                 double NA, PhiConst P)
+                                                 double NA, PhiConst P, const int *gate)
 {
+    if (gate && *gate) return;
     constexpr int F = 6 + ND;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -295,7 +301,7 @@ __global__ void k_scale_slots(double *dphi, const double *cot, int N, int nslots
 }
 
 // ---------------------------------------------------------------------------
-// Reductions: each block writes one partial per quantity; k_finalize sums the
+// Reductions: each block writes one partial per quantity; k_aa_fin sums the
 // partials of each quantity in a fixed order (bitwise reproducible).
 // ---------------------------------------------------------------------------
 template <int NQ>
@@ -314,23 +320,6 @@ __device__ __forceinline__ void block_partials(double (&v)[NQ], double *part, in
         double s = 0.0;
         for (int w = 0; w < nw; w++) s += sm[threadIdx.x][w];
         part[threadIdx.x * nblk + blockIdx.x] = s;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_finalize(const double *part, int nblk, double *out)
-{
-    __shared__ double sm[16];
-    int q = blockIdx.x;
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[q * nblk + b];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    if (lane == 0) sm[wid] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int w = 0; w < nw; w++) t += sm[w];
-        out[q] = t;
     }
 }
 
@@ -365,6 +354,8 @@ __global__ void k_exp(const double *x, double *rho, int n)
 //  mode 0: out = dF_res/drho (no field, no clamp)                functional only
 //  mode 1: gx = betafix(lnG); res = gx - sol; partial |res|^2     first/Picard step
 //  mode 2: aa_point! (anderson.jl:470-481); partials |res|^2, |df|^2, df.res
+// With ctl set, mode and beta come from the solver state (and nothing runs once
+// the solve has ended).
 // With adj_g set, the map value is the adjoint's g - rho* (dphi + parts) instead of
 // the clamped exponent (dphi, part then hold a Hessian-vector product).
 __global__ __launch_bounds__(256) void k_update(const double *__restrict__ dphi, const double *__restrict__ part,
@@ -374,8 +365,13 @@ __global__ __launch_bounds__(256) void k_update(const double *__restrict__ dphi,
                                                 double *__restrict__ res, double *__restrict__ df,
                                                 double *__restrict__ dg, double beta, int mode, double *red,
                                                 int nblk, const double *__restrict__ adj_g,
-                                                const double *__restrict__ adj_rho)
+                                                const double *__restrict__ adj_rho, const AACtl *ctl)
 {
+    if (ctl) {                                 // mode and damping from the device-side solver
+        if (ctl->g_done) return;
+        mode = ctl->umode;
+        beta = ctl->beta_u;
+    }
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
     if (t < N * nc) {
@@ -417,13 +413,319 @@ __global__ __launch_bounds__(256) void k_update(const double *__restrict__ dphi,
     if (mode != 0) block_partials<3>(v, red, nblk);
 }
 
-// ----- Anderson QR update (aa_qr_update!, anderson.jl:498-583) --------------
-// pass 1: optional downdate Q' = Q Qx, then h1 = Q'^T df and t = Q'^T res.
-__global__ __launch_bounds__(256) void k_aa_proj1(int n, const double *__restrict__ Q, double *__restrict__ Qout,
-                                                  AAMat qx, int downdate, int m, int kq,
-                                                  const double *__restrict__ df, const double *__restrict__ res,
-                                                  double *red, int nblk)
+// ----- Anderson on the device (aasol, src/utils/anderson.jl:301-583) --------
+// The host enqueues bodies of one map evaluation each (eagerly or as a captured
+// hipGraph); the single-thread control kernels below advance the solver state in
+// AACtl between the N-wide kernels, which read their work from it and return at
+// once when their gate is set, so bodies after the end of a solve do nothing.
+
+// Householder QR of the m x (m-1) block R[:, 2:m] with LAPACK's conventions
+// (dgeqrf/dlarfg, then the thin Q of dorg2r); R column-major, leading dim m.
+// The Anderson downdate (downdate_aaqr!, src/utils/anderson.jl:552-583).
+__host__ __device__ inline void householder_downdate(double *R, int m, double *Qx)
 {
+#pragma clang fp contract(off)
+    int nr = m, ncol = m - 1;
+    double A[CDFT_MAXM * CDFT_MAXM], tau[CDFT_MAXM], v[CDFT_MAXM];
+    for (int j = 0; j < ncol; j++)
+        for (int i = 0; i < nr; i++) A[j * CDFT_MAXM + i] = R[(j + 1) * m + i];
+    for (int j = 0; j < ncol; j++) {
+        double alpha = A[j * CDFT_MAXM + j], xn = 0.0;
+        for (int i = j + 1; i < nr; i++) xn += A[j * CDFT_MAXM + i] * A[j * CDFT_MAXM + i];
+        xn = sqrt(xn);
+        if (xn == 0.0) { tau[j] = 0.0; continue; }
+        double beta = -copysign(hypot(alpha, xn), alpha);
+        tau[j] = (beta - alpha) / beta;
+        double sc = 1.0 / (alpha - beta);
+        for (int i = j + 1; i < nr; i++) A[j * CDFT_MAXM + i] *= sc;
+        A[j * CDFT_MAXM + j] = beta;
+        for (int jj = j + 1; jj < ncol; jj++) {
+            double w = A[jj * CDFT_MAXM + j];
+            for (int i = j + 1; i < nr; i++) w += A[j * CDFT_MAXM + i] * A[jj * CDFT_MAXM + i];
+            w *= tau[j];
+            A[jj * CDFT_MAXM + j] -= w;
+            for (int i = j + 1; i < nr; i++) A[jj * CDFT_MAXM + i] -= w * A[j * CDFT_MAXM + i];
+        }
+    }
+    for (int j = 0; j < ncol; j++)
+        for (int i = 0; i < nr; i++) Qx[j * nr + i] = (i == j) ? 1.0 : 0.0;
+    for (int h = ncol - 1; h >= 0; h--) {
+        if (tau[h] == 0.0) continue;
+        v[h] = 1.0;
+        for (int i = h + 1; i < nr; i++) v[i] = A[h * CDFT_MAXM + i];
+        for (int j = 0; j < ncol; j++) {
+            double w = 0.0;
+            for (int i = h; i < nr; i++) w += v[i] * Qx[j * nr + i];
+            w *= tau[h];
+            for (int i = h; i < nr; i++) Qx[j * nr + i] -= w * v[i];
+        }
+    }
+    // R[1:m-1, 1:m-1] = Rd; R[:, m] = 0
+    for (int j = 0; j < ncol; j++)
+        for (int i = 0; i < ncol; i++) R[j * m + i] = (i <= j) ? A[j * CDFT_MAXM + i] : 0.0;
+    for (int i = 0; i < m; i++) R[(m - 1) * m + i] = 0.0;
+}
+
+__device__ inline void aa_hist(AACtl &c, double *hist, double v)
+{
+    if (c.nh < c.hist_cap) hist[c.nh] = v;
+    c.nh++;
+}
+
+// AndersonOK's errcode (anderson.jl:727-737)
+__device__ inline void aa_finish(AACtl &c)
+{
+    if (c.toosoon) { c.idid = 1; c.errcode = -1; }
+    else if (c.r >= c.up_bd) { c.idid = 0; c.errcode = -2; }
+    else if (c.r <= c.tol && !c.fp32) { c.idid = 1; c.errcode = 0; }
+    else { c.idid = 0; c.errcode = 10; }
+    c.phase = AA_DONE;
+}
+
+// The while condition of aasol's Anderson loop (anderson.jl:393).  In mixed
+// precision the FP32 phase continues below tol until the switch to the FP64 map.
+__device__ inline void aa_loop_check(AACtl &c)
+{
+    if (c.k < c.maxit + c.picard_maxit && (c.r > c.tol || c.fp32) && !c.toosoon && c.r < c.up_bd) c.phase = AA_LOOP;
+    else aa_finish(c);
+}
+
+// After a map evaluation: the residual norm and the Picard/Anderson book-keeping
+// of the evaluation site that ran.  A non-finite FP32 residual ends the FP32 phase:
+// the point goes back to the last one with a finite FP32 value (xsafe).
+__device__ inline void aa_post(AACtl &c, const double *fin, double *hist)
+{
+#pragma clang fp contract(off)
+    const double r = sqrt(fin[0]);
+    c.nevals++;
+    if (c.evfp32) c.nevals32++;
+    c.keep = 0;
+    switch (c.phase) {
+    case AA_FIRST:
+        if (c.fp32 && !isfinite(r)) {          // the seed itself: FP64 from the start
+            c.fp32 = 0;
+            c.r = r;
+            return;
+        }
+        c.r = r;
+        c.keep = c.fp32;
+        c.up_bd = 1.e4 * r;
+        c.tol = c.rtol * r + c.atol;
+        c.picard_tol = c.picard_rtol * r + c.picard_atol;
+        aa_hist(c, hist, r);
+        c.toosoon = (r <= c.tol);
+        if (c.toosoon) { aa_finish(c); return; }
+        if (c.picard_maxit > 1) { c.phase = AA_PICARD; c.it = 0; }
+        else c.phase = AA_START;
+        return;
+    case AA_PICARD:
+        if (c.fp32 && !isfinite(r)) {          // retry this Picard step with the FP64 map at xsafe
+            aa_hist(c, hist, r);
+            c.r = r;
+            c.restore = 1;
+            c.fp32 = 0;
+            return;
+        }
+        c.r = r;
+        c.it++;
+        c.keep = c.fp32;
+        aa_hist(c, hist, r);
+        if (r <= c.picard_tol || r >= c.up_bd || c.it >= c.picard_maxit - 1) {
+            c.toosoon = (r <= c.tol);
+            if (c.toosoon) aa_finish(c);
+            else c.phase = AA_START;
+        }
+        return;
+    case AA_SWITCH:
+        c.r = r;
+        aa_hist(c, hist, r);
+        if (r <= c.tol) aa_finish(c);
+        else c.phase = AA_START;
+        return;
+    case AA_START:
+    case AA_STEP: {
+        c.k += 1;
+        c.ka = (c.phase == AA_START) ? 1 : c.ka + 1;
+        c.r = r;
+        c.dfn2 = fin[1];
+        c.dfres = fin[2];
+        aa_hist(c, hist, r);
+        const bool failed = c.fp32 && !isfinite(r);
+        if (failed) {
+            c.restore = 1;
+            c.fp32_failed = 1;
+            c.r = 0.0;                         // the switch at the top of the loop takes over
+        }
+        c.keep = c.fp32 && !failed;
+        if (c.phase == AA_START) c.best = c.r;
+        aa_loop_check(c);
+        return;
+    }
+    default:
+        return;
+    }
+}
+
+__device__ inline void aa_mid(AACtl &c, const double *fin);
+
+// Before a map evaluation: the work of the next body (which point, which map,
+// which update), with BuildDG!'s ring and the QR downdate of aa_qr_update!.
+__device__ inline void aa_pre(AACtl &c)
+{
+#pragma clang fp contract(off)
+    c.g_proj = 1;
+    c.pmode = 0;
+    c.umode = 1;
+    c.beta_u = c.beta;
+    switch (c.phase) {
+    case AA_FIRST:
+        break;
+    case AA_PICARD:
+        c.pmode = c.restore ? 2 : 1;
+        c.restore = 0;
+        c.beta_u = c.picard_beta;
+        break;
+    case AA_START:
+        c.pmode = 1;
+        c.umode = 2;
+        for (int j = 0; j < CDFT_MAXM * CDFT_MAXM; j++) c.R[j] = 0.0;
+        for (int j = 0; j < CDFT_MAXM; j++) c.ring[j] = j;
+        break;
+    case AA_LOOP: {
+        if (c.fp32) {
+            if (c.r < 0.5 * c.best) { c.best = c.r; c.since_best = 0; }
+            else c.since_best++;
+            if (c.r <= 4.0 * c.tol || c.since_best >= 40 || c.fp32_failed) {
+                c.fp32 = 0;                    // the FP64 map from here, at the (restored) point
+                c.phase = AA_SWITCH;
+                c.pmode = c.restore ? 2 : 0;
+                c.restore = 0;
+                break;
+            }
+        }
+        c.phase = AA_STEP;
+        c.umode = 2;
+        const int m = c.m;
+        if (m == 0) { c.pmode = 1; break; }
+        const int kk = c.ka + 1;               // BuildDG!(DG, m, k+1, dg)
+        if (m == 1) {
+            c.slot = c.ring[0];
+        } else if (kk > m + 1) {
+            const int first = c.ring[0];
+            for (int j = 0; j < m - 1; j++) c.ring[j] = c.ring[j + 1];
+            c.ring[m - 1] = first;
+            c.slot = c.ring[m - 1];
+        } else {
+            c.slot = c.ring[kk - 2];
+        }
+        const int kq = c.ka - 1;              // aa_qr_update!(Q, R, df, m, k-1, Qd)
+        c.down = 0;
+        if (kq == 0) {
+            c.nrm = sqrt(c.dfn2);
+            c.R[0] = c.nrm;
+            c.qcol = 0;
+            c.tres[0] = c.dfres / c.nrm;
+            c.kqq = 0;
+            c.pmode = 3;
+            aa_mid(c, nullptr);
+        } else {
+            for (int j = 0; j < CDFT_MAXM * CDFT_MAXM; j++) c.qx.a[j] = 0.0;
+            c.down = (kq > m - 1);
+            if (c.down) householder_downdate(c.R, m, c.qx.a);
+            c.kqq = min(kq, m - 1);
+            c.g_proj = 0;
+        }
+        c.pmode = 3;
+        break;
+    }
+    default:                                   // AA_DONE: a restore left pending by the last step
+        c.pmode = c.restore ? 2 : 0;
+        c.restore = 0;
+        c.keep = 0;
+        break;
+    }
+    const int done = (c.phase == AA_DONE);
+    c.g_done = done;
+    if (done) c.g_proj = 1;
+    c.g_f64 = done || c.fp32;
+    c.g_f32 = done || !c.fp32;
+    c.evfp32 = !done && c.fp32;
+}
+
+// After the projections (or in aa_pre when there were none): the new R column
+// and Q^T res, then the triangular solve theta = R \ (Q^T res).
+__device__ inline void aa_mid(AACtl &c, const double *fin)
+{
+#pragma clang fp contract(off)
+    if (c.pmode != 3 || c.g_done) return;
+    const int m = c.m;
+    if (!c.g_proj) {
+        const int kqq = c.kqq;
+        const double *h1 = fin, *tq = fin + CDFT_MAXM, *h2 = fin + CDFT_NQ;
+        for (int j = 0; j < kqq; j++) c.R[kqq * m + j] = h1[j] + h2[j];
+        c.nrm = sqrt(fin[2 * CDFT_NQ]);
+        c.R[kqq * m + kqq] = c.nrm;
+        c.qcol = kqq;
+        for (int j = 0; j < kqq; j++) c.tres[j] = tq[j];
+        c.tres[kqq] = fin[2 * CDFT_NQ + 1] / c.nrm;
+        c.qsel ^= c.down;
+    }
+    const int mk = min(m, c.ka);
+    c.mk = mk;
+    for (int j = 0; j < CDFT_MAXM; j++) c.th.a[j] = 0.0;
+    for (int j = mk - 1; j >= 0; j--) {
+        double acc = c.tres[j];
+        for (int l = j + 1; l < mk; l++) acc -= c.R[l * m + j] * c.th.a[l];
+        c.th.a[j] = acc / c.R[j * m + j];
+    }
+    for (int j = 0; j < mk; j++) c.th.col[j] = c.ring[j];
+}
+
+// Sums of the partials of quantity q = blockIdx.x over nblk blocks in a fixed
+// order (a strided sum per thread, a wave butterfly, the waves in order), then,
+// in thread 0 of the last of the nq blocks to finish (a device-scope counter,
+// reset for the next launch; the fences order each block's result before the
+// count and the others' results before the step), the solver step that needs
+// them: STEP 1 aa_mid after the projections, STEP 2 aa_post and aa_pre after a
+// map evaluation.  Gated by g_proj (STEP 0, 1) or g_done (STEP 2).
+template <int STEP>
+__global__ __launch_bounds__(256) void k_aa_fin(const double *part, int nblk, double *out, AACtl *ctl, int site,
+                                                double *fin, double *hist)
+{
+    if (STEP == 2 ? ctl->g_done : ctl->g_proj) return;
+    __shared__ double sm[16];
+    const int q = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[q * nblk + b];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if (lane == 0) sm[wid] = s;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double t = 0.0;
+    for (int w = 0; w < nw; w++) t += sm[w];
+    out[q] = t;
+    if (STEP == 0) return;
+    __threadfence();
+    if (atomicAdd(&ctl->cnt[site], 1) != (int)gridDim.x - 1) return;
+    atomicExch(&ctl->cnt[site], 0);
+    __threadfence();
+    if (STEP == 1) aa_mid(*ctl, fin);
+    else {
+        aa_post(*ctl, fin, hist);
+        aa_pre(*ctl);
+    }
+}
+
+// pass 1: optional downdate Q' = Q Qx, then h1 = Q'^T df and t = Q'^T res.
+__global__ __launch_bounds__(256) void k_aa_proj1(int n, const double *__restrict__ Q0, double *__restrict__ Q1,
+                                                  const AACtl *ctl, const double *__restrict__ df,
+                                                  const double *__restrict__ res, double *red, int nblk)
+{
+    if (ctl->g_proj) return;
+    const int downdate = ctl->down, m = ctl->m, kq = ctl->kqq;
+    const double *Q = ctl->qsel ? Q1 : Q0;
+    double *Qout = ctl->qsel ? const_cast<double *>(Q0) : Q1;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[2 * CDFT_MAXM];
     for (int q = 0; q < 2 * CDFT_MAXM; q++) v[q] = 0.0;
@@ -434,7 +736,7 @@ __global__ __launch_bounds__(256) void k_aa_proj1(int n, const double *__restric
             for (int l = 0; l < m; l++) qo[l] = Q[(size_t)l * n + i];
             for (int j = 0; j < m - 1; j++) {
                 double a = 0.0;
-                for (int l = 0; l < m; l++) a += qo[l] * qx.a[j * m + l];
+                for (int l = 0; l < m; l++) a += qo[l] * ctl->qx.a[j * m + l];
                 qv[j] = a;
                 Qout[(size_t)j * n + i] = a;
             }
@@ -452,11 +754,16 @@ __global__ __launch_bounds__(256) void k_aa_proj1(int n, const double *__restric
 }
 
 // pass 2/3: df -= Q' h (h from the finalized reductions); then partials of
-// Q'^T df (pass 2) or |df|^2 and df.res (pass 3).
-__global__ __launch_bounds__(256) void k_aa_proj2(int n, const double *__restrict__ Q, int kq,
-                                                  double *__restrict__ df, const double *__restrict__ res,
-                                                  const double *__restrict__ hcoef, int last, double *red, int nblk)
+// Q'^T df (pass 2) or |df|^2 and df.res (pass 3).  Q' is the basis after the
+// downdate of pass 1.
+__global__ __launch_bounds__(256) void k_aa_proj2(int n, const double *__restrict__ Q0, const double *__restrict__ Q1,
+                                                  const AACtl *ctl, double *__restrict__ df,
+                                                  const double *__restrict__ res, const double *__restrict__ hcoef,
+                                                  int last, double *red, int nblk)
 {
+    if (ctl->g_proj) return;
+    const int kq = ctl->kqq;
+    const double *Q = (ctl->qsel ^ ctl->down) ? Q1 : Q0;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[CDFT_MAXM];
     for (int q = 0; q < CDFT_MAXM; q++) v[q] = 0.0;
@@ -479,19 +786,37 @@ __global__ __launch_bounds__(256) void k_aa_proj2(int n, const double *__restric
     block_partials<CDFT_MAXM>(v, red, nblk);
 }
 
-// New Q column, the Anderson point sol = gx - DG theta, and rho = exp(sol).
-__global__ __launch_bounds__(256) void k_aa_newpoint(int n, const double *__restrict__ df, double nrm,
-                                                     double *__restrict__ qcol, double *__restrict__ sol,
-                                                     const double *__restrict__ gx, const double *__restrict__ DG,
-                                                     AAMat th, int mk, double *__restrict__ rho,
-                                                     const double *__restrict__ mask)
+// The next point of the body: sol = gx (Picard), sol = xsafe (back to the last
+// point with a finite FP32 map value), or the Anderson point sol = gx - DG theta
+// with the new Q column (vnew ./= hv[k+1], anderson.jl:538) and BuildDG!'s copy of
+// dg into its ring slot; then rho = exp(sol), or rho = mask sol for the adjoint.
+__global__ __launch_bounds__(256) void k_aa_point(int n, const AACtl *__restrict__ ctl, const double *__restrict__ df,
+                                                  double *__restrict__ Q0, double *__restrict__ Q1,
+                                                  double *__restrict__ sol, const double *__restrict__ gx,
+                                                  const double *__restrict__ dg, double *__restrict__ DG,
+                                                  double *__restrict__ xsafe, double *__restrict__ rho,
+                                                  const double *__restrict__ mask)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pm = ctl->pmode, keep = ctl->keep;
+    if (pm == 0 && !keep) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (qcol) qcol[i] = df[i] / nrm;   // vnew ./= hv[k+1] (anderson.jl:538)
-    double a = 0.0;
-    for (int j = 0; j < mk; j++) a += DG[(size_t)th.col[j] * n + i] * th.a[j];
-    double x = gx[i] - a;
+    if (keep) xsafe[i] = sol[i];
+    if (pm == 0) return;
+    double x;
+    if (pm == 1) {
+        x = gx[i];
+    } else if (pm == 2) {
+        x = xsafe[i];
+    } else {
+        DG[(size_t)ctl->slot * n + i] = dg[i];
+        double *Qc = ctl->qsel ? Q1 : Q0;
+        Qc[(size_t)ctl->qcol * n + i] = df[i] / ctl->nrm;
+        const int mk = ctl->mk;
+        double a = 0.0;
+        for (int j = 0; j < mk; j++) a += DG[(size_t)ctl->th.col[j] * n + i] * ctl->th.a[j];
+        x = gx[i] - a;
+    }
     sol[i] = x;
     rho[i] = mask ? mask[i] * x : exp(x);
 }
