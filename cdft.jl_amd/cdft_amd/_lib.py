@@ -22,7 +22,8 @@ _dp = C.POINTER(C.c_double)
 # every symbol include/cdft_hip.h declares
 EXPORTS = [
     "cdft_ctx_create", "cdft_ctx_destroy", "cdft_last_error", "cdft_system_create",
-    "cdft_system_destroy", "cdft_system_npoints", "cdft_functional_eval", "cdft_converge",
+    "cdft_system_destroy", "cdft_system_npoints", "cdft_functional_eval", "cdft_functional_eval_fp32",
+    "cdft_converge",
     "cdft_converge_batch", "cdft_converge_batch_device", "cdft_converge_batch_device_noalloc",
     "cdft_last_kernel_ms",
     "cdft_surface_tension", "cdft_adsorption", "cdft_phi_vjp", "cdft_hvp", "cdft_adjoint_solve",
@@ -84,6 +85,7 @@ def load():
     L.cdft_system_destroy.argtypes = [vp]
     L.cdft_system_npoints.argtypes = [vp]
     L.cdft_functional_eval.argtypes = [vp, _dp, _dp, _dp, _dp]
+    L.cdft_functional_eval_fp32.argtypes = [vp, _dp, _dp]
     L.cdft_converge.argtypes = [vp, _dp, C.POINTER(SolverOpts), C.POINTER(Result)]
     L.cdft_converge_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(_dp),
                                       C.POINTER(SolverOpts), C.POINTER(Result)]
@@ -203,6 +205,14 @@ class DeviceSystem:
         check(load().cdft_functional_eval(self.handle, ptr(rho), ptr(out), ptr(fres),
                                           ptr(fval) if want_fval else None), self.ctx.handle)
         return out, float(fres[0]), fval
+
+    def functional_eval_fp32(self, rho):
+        """delta F_res / delta rho from the FP32 map of a mixed-precision system."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        assert rho.size == self.N * self.nc
+        out = np.zeros(self.N * self.nc)
+        check(load().cdft_functional_eval_fp32(self.handle, ptr(rho), ptr(out)), self.ctx.handle)
+        return out
 
     def converge(self, rho_inout, *, maxit=10000, beta=1e-2, tol=1e-4, anderson_start=1e-1,
                  m=5, hist_cap=0):

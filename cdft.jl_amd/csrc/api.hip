@@ -70,7 +70,7 @@ void free_sys_buffers(cdft_sys *s)
     double **b3[] = {&s->G3, &s->A3, &s->B3, &s->S3, &s->B3x[0], &s->B3x[1], &s->B3x[2]};
     for (double **b : b3)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
-    float **f3[] = {&s->G3f, &s->n32, &s->dphi32};
+    float **f3[] = {&s->G3f, &s->n32, &s->dphi32, &s->h32};
     for (float **b : f3)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
 }
@@ -298,8 +298,12 @@ void launch_transposed3d(cdft_sys *s, const T *dphi, double *dtot)
 template <int NC, int AS>
 void launch_phi32_as(cdft_sys *s)
 {
-    hipLaunchKernelGGL((k_phi32<NC, 3, AS>), dim3(nblocks(s->N)), dim3(256), 0, s->ctx->stream, s->rho, s->n32,
-                       s->dphi32, s->N, s->NA, s->pc, s->gate);
+    if (s->nd == 3)
+        hipLaunchKernelGGL((k_phi32<NC, 3, AS>), dim3(nblocks(s->N)), dim3(256), 0, s->ctx->stream, s->rho, s->n32,
+                           s->dphi32, s->N, s->NA, s->pc, s->gate, (double *)nullptr);
+    else
+        hipLaunchKernelGGL((k_phi32<NC, 1, AS>), dim3(nblocks(s->N)), dim3(256), 0, s->ctx->stream, s->rho, s->n32,
+                           s->dphi32, s->N, s->NA, s->pc, s->gate, s->dphi);
 }
 
 template <int NC>
@@ -312,11 +316,30 @@ void launch_phi32(cdft_sys *s)
     }
 }
 
-// The FP32 map of a mixed-precision 3D system: weighted densities, Phi's gradient
-// and transposed convolutions in FP32 (n3-singular factors in FP64); the sum
-// lands in FP64 (s->part) for the exponent assembly.
+// The FP32 map of a mixed-precision system: weighted densities, Phi's gradient
+// and transposed convolutions in FP32 (n3-singular factors in FP64); the sums
+// land in FP64 (s->part) for the exponent assembly.
 void launch_functional32(cdft_sys *s)
 {
+    if (s->nd == 1) {
+        // 1D: compensated FP32 circulant sums; the transposed sums land in FP64 in
+        // s->part and the rho slot of the gradient in s->dphi, where k_update reads them
+        const int N = s->N, F = s->F;
+        const dim3 grid((N + CIRC_TW - 1) / CIRC_TW, s->nc * (F - 1));
+        hipLaunchKernelGGL((k_circ1d<float, double, float, true>), grid, dim3(256), 0, s->ctx->stream,
+                           (const double *)s->rho, (long)N, 0L, (const float *)s->h32, s->n32, (long)F * N, (long)N, 0,
+                           N, F, (float)s->NA, 0, s->gate);
+        switch (s->nc) {
+        case 1: launch_phi32<1>(s); break;
+        case 2: launch_phi32<2>(s); break;
+        case 3: launch_phi32<3>(s); break;
+        default: launch_phi32<4>(s); break;
+        }
+        hipLaunchKernelGGL((k_circ1d<float, float, double, true>), grid, dim3(256), 0, s->ctx->stream,
+                           (const float *)s->dphi32, (long)F * N, (long)N, (const float *)s->h32, s->part,
+                           (long)(F - 1) * N, (long)N, 1, N, F, 1.0f, 1, s->gate);
+        return;
+    }
     launch_weighted3d<float>(s, s->rho, s->n32);
     switch (s->nc) {
     case 1: launch_phi32<1>(s); break;
@@ -616,6 +639,11 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
         (e = alloc(&s->fval, N)) != hipSuccess ||
         (d3 && ((e = alloc(&s->G3, (size_t)8 * nc * N)) != hipSuccess || (e = alloc(&s->A3, (size_t)2 * N)) != hipSuccess ||
                 (e = alloc(&s->B3, (size_t)2 * N)) != hipSuccess || (e = alloc(&s->S3, (size_t)2 * N)) != hipSuccess)) ||
+        (!d3 && d->precision == CDFT_MIXED &&
+         ((e = hipMalloc((void **)&s->h32, sizeof(float) * (size_t)F * nc * N)) != hipSuccess ||
+          (e = hipMalloc((void **)&s->n32, sizeof(float) * (size_t)F * nc * N)) != hipSuccess ||
+          (e = hipMalloc((void **)&s->dphi32, sizeof(float) * (size_t)F * nc * N)) != hipSuccess ||
+          (e = alloc(&s->xsafe, n)) != hipSuccess)) ||
         (d3 && d->precision == CDFT_MIXED &&
          ((e = hipMalloc((void **)&s->G3f, sizeof(float) * 8 * nc * (size_t)N)) != hipSuccess ||
           (e = hipMalloc((void **)&s->n32, sizeof(float) * (size_t)F * nc * N)) != hipSuccess ||
@@ -695,6 +723,9 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
         if (e == hipSuccess) {
             hipLaunchKernelGGL(k_setup_h1d, dim3(nblocks(N), nc * F), dim3(256), 0, ctx->stream, s->h, N, nc, ddev,
                                s->psi, mult, s->L);
+            if (s->h32)
+                hipLaunchKernelGGL(k_to_f32, dim3(nblocks((long)nc * F * N)), dim3(256), 0, ctx->stream,
+                                   (const double *)s->h, s->h32, (long)nc * F * N);
             e = hipGetLastError();
         }
     }
@@ -819,10 +850,10 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     HIPCHECK_CTX(ctx, hipHostMalloc((void **)&hc, sizeof(AACtl)));
     struct HostFree { AACtl *p; ~HostFree() { if (p) (void)hipHostFree(p); } } hf{hc};
 
-    // Mixed precision (3D): the FP32 map until the residual reaches the FP32 noise
+    // Mixed precision: the FP32 map until the residual reaches the FP32 noise
     // floor (4 tol, or no halving in 40 iterations), then the FP64 map with a fresh
     // Anderson history; the reported residual and termination are the FP64 ones.
-    const bool mixed = (!adj && s->precision == CDFT_MIXED && s->nd == 3);
+    const bool mixed = (!adj && s->precision == CDFT_MIXED);
     std::memset(hc, 0, sizeof(AACtl));
     hc->m = m;
     hc->maxit = o->maxit;
@@ -964,6 +995,27 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
 int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdft_result *r)
 {
     return aasol_run(s, rho_inout, o, r, nullptr);
+}
+
+int cdft_functional_eval_fp32(cdft_sys *s, const double *rho, double *dfdrho)
+{
+    if (!s || !rho || !dfdrho) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_functional_eval_fp32: NULL argument");
+    cdft_ctx *ctx = s->ctx;
+    if (s->precision != CDFT_MIXED)
+        return fail(ctx, CDFT_ERR_UNSUPPORTED, "cdft_functional_eval_fp32: the system was not created for mixed precision");
+    hipStream_t st = ctx->stream;
+    const long n = (long)s->N * s->nc;
+    HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(s->rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    launch_functional32(s);
+    hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, upd_src(s).dphi, upd_src(s).part,
+                       (const double *)nullptr, s->N, s->nc, upd_src(s).F, (const double *)nullptr, s->out, (const double *)nullptr, (double *)nullptr,
+                       (double *)nullptr, (double *)nullptr, (double *)nullptr, 1.0, 0, (double *)nullptr, 0,
+                       (const double *)nullptr, (const double *)nullptr, (const AACtl *)nullptr);
+    HIPCHECK_CTX(ctx, hipGetLastError());
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(dfdrho, s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+    return CDFT_OK;
 }
 
 // Device scratch of one call, freed on every return path.

@@ -86,9 +86,10 @@ struct cdft_sys {
     double *G3 = nullptr, *A3 = nullptr, *B3 = nullptr, *S3 = nullptr;
     // three more complex work arrays when z lines fit the all-pairs z pass (128, 256 points)
     double *B3x[3] = {nullptr, nullptr, nullptr};
-    // 3D mixed precision: FP32 spectra, weighted densities and gradients
+    // mixed precision: FP32 spectra (3D), weighted densities and gradients
     float *G3f = nullptr, *n32 = nullptr, *dphi32 = nullptr;
-    double *xsafe = nullptr;   // 3D mixed precision: the last point whose FP32 map value was finite
+    float *h32 = nullptr;      // 1D mixed precision: FP32 copy of the real-space kernels h
+    double *xsafe = nullptr;   // mixed precision: the last point whose FP32 map value was finite
     double *hddft = nullptr;   // 1D DDFT derivative kernels [(c*4 + f)*N + j], f = 1 (Laplacian), 2, 3 (gradient)
     double *red = nullptr;     // block partials
     double *fin = nullptr;     // finalized reductions [CDFT_NQ]

@@ -143,11 +143,26 @@ __global__ void k_ddft_combine(const double *d, int N, int nc, double *deta)
 // k range (in LDS chunks: h chunk with broadcast reads, the matching window of src
 // with consecutive reads per lane), and the quarters are added in a fixed order,
 // so the result is reproducible and many workgroups share the N^2 work.
+// The sums run in T, loads convert from TS and the result is stored as TD.  COMP
+// (the FP32 map): compensated summation, blocks of eight products summed plainly
+// and the block sums added with Kahan's correction, the four waves' sums and
+// corrections combined in TD; the sum's rounding error then no longer grows with
+// the number of taps (a plain FP32 sum over N taps loses up to ~N eps of the
+// largest partial sum), at 1.5 extra operations per tap.
 constexpr int CIRC_TW = 64;
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_circ1d(const T *__restrict__ src, long s_cstride, long s_fstride,
-                                                const T *__restrict__ h, T *__restrict__ dst, long d_cstride,
+__device__ __forceinline__ void kahan_add(T &s, T &c, T x)
+{
+    const T y = x - c;
+    const T t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+template <typename T, typename TS = T, typename TD = T, bool COMP = false>
+__global__ __launch_bounds__(256) void k_circ1d(const TS *__restrict__ src, long s_cstride, long s_fstride,
+                                                const T *__restrict__ h, TD *__restrict__ dst, long d_cstride,
                                                 long d_fstride, int d_fofs, int N, int F, T scale, int vec_sign,
                                                 const int *gate)
 {
@@ -156,18 +171,19 @@ __global__ __launch_bounds__(256) void k_circ1d(const T *__restrict__ src, long 
     __shared__ T hs[NG][KC];
     __shared__ T ws[NG][TW + KC];
     __shared__ T part[NG][TW];
+    __shared__ T partc[COMP ? NG : 1][TW];
     const int y = blockIdx.y;
     const int c = y / (F - 1), f = 1 + y % (F - 1);
-    const T *s = src + c * s_cstride + f * s_fstride;
+    const TS *s = src + c * s_cstride + f * s_fstride;
     const T *hk = h + (size_t)(c * F + f) * N;
-    T *d = dst + c * d_cstride + (f - d_fofs) * d_fstride;
+    TD *d = dst + c * d_cstride + (f - d_fofs) * d_fstride;
     T sc = scale;
     if (vec_sign && f == 3) sc = -scale;   // 1D vector slot is f = 3
     const int i0 = blockIdx.x * TW;
     const int g = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int Kg = (N + NG - 1) / NG;                   // taps per wave
     const int kb = g * Kg, ke = min(N, kb + Kg);
-    T acc = 0;
+    T acc = 0, accc = 0;
     for (int kc0 = 0; kc0 < Kg; kc0 += KC) {            // the same trip count in every wave
         const int kc = kb + kc0;
         const int kcn = max(0, min(KC, ke - kc));
@@ -178,27 +194,52 @@ __global__ __launch_bounds__(256) void k_circ1d(const T *__restrict__ src, long 
         for (int t = l; t < wn; t += TW) {
             int gi = (base + t) % N;
             if (gi < 0) gi += N;
-            ws[g][t] = s[gi];
+            ws[g][t] = T(s[gi]);
         }
         __syncthreads();
         if (kcn > 0) {
             const T *wp = &ws[g][l + kcn - 1];
             int kk = 0;
-            T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-            for (; kk + 3 < kcn; kk += 4) {
-                a0 += hs[g][kk] * wp[-kk];
-                a1 += hs[g][kk + 1] * wp[-kk - 1];
-                a2 += hs[g][kk + 2] * wp[-kk - 2];
-                a3 += hs[g][kk + 3] * wp[-kk - 3];
+            if (COMP) {
+                for (; kk + 7 < kcn; kk += 8) {
+                    T a0 = hs[g][kk] * wp[-kk], a1 = hs[g][kk + 1] * wp[-kk - 1];
+                    a0 += hs[g][kk + 2] * wp[-kk - 2];
+                    a1 += hs[g][kk + 3] * wp[-kk - 3];
+                    a0 += hs[g][kk + 4] * wp[-kk - 4];
+                    a1 += hs[g][kk + 5] * wp[-kk - 5];
+                    a0 += hs[g][kk + 6] * wp[-kk - 6];
+                    a1 += hs[g][kk + 7] * wp[-kk - 7];
+                    kahan_add(acc, accc, a0 + a1);
+                }
+                T a0 = 0;
+                for (; kk < kcn; kk++) a0 += hs[g][kk] * wp[-kk];
+                kahan_add(acc, accc, a0);
+            } else {
+                T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+                for (; kk + 3 < kcn; kk += 4) {
+                    a0 += hs[g][kk] * wp[-kk];
+                    a1 += hs[g][kk + 1] * wp[-kk - 1];
+                    a2 += hs[g][kk + 2] * wp[-kk - 2];
+                    a3 += hs[g][kk + 3] * wp[-kk - 3];
+                }
+                for (; kk < kcn; kk++) a0 += hs[g][kk] * wp[-kk];
+                acc += (a0 + a1) + (a2 + a3);
             }
-            for (; kk < kcn; kk++) a0 += hs[g][kk] * wp[-kk];
-            acc += (a0 + a1) + (a2 + a3);
         }
     }
     part[g][l] = acc;
+    if (COMP) partc[g][l] = accc;
     __syncthreads();
     const int i = i0 + l;
-    if (g == 0 && i < N) d[i] = sc * ((part[0][l] + part[1][l]) + (part[2][l] + part[3][l]));
+    if (g == 0 && i < N) {
+        if (COMP) {
+            const TD hi = (TD(part[0][l]) + TD(part[1][l])) + (TD(part[2][l]) + TD(part[3][l]));
+            const TD lo = (TD(partc[0][l]) + TD(partc[1][l])) + (TD(partc[2][l]) + TD(partc[3][l]));
+            d[i] = TD(sc) * (hi - lo);
+        } else {
+            d[i] = TD(sc * ((part[0][l] + part[1][l]) + (part[2][l] + part[3][l])));
+        }
+    }
 }
 
 // Phi and dPhi/dn at every grid point.
@@ -228,11 +269,12 @@ __global__ __launch_bounds__(256) void k_phi(const double *__restrict__ rho, con
 
 // FP32 map (mixed precision): Phi's gradient from FP32 weighted densities, with the
 // n3-singular hard-sphere factors in FP64 (phi_grad<float, double>); the chain's
-// lambda-slot gradient is 0 where FP32 does not resolve lambda.
+// lambda-slot gradient is 0 where FP32 does not resolve lambda.  dphi0 (1D): the
+// rho slot is also written in FP64 into the FP64 gradient array.
 template <int NC, int ND, int AS = 0>
 __global__ __launch_bounds__(256) void k_phi32(const double *__restrict__ rho, const float *__restrict__ n,
                                                float *__restrict__ dphi, int N, double NA, PhiConst P,
-                                               const int *gate)
+                                               const int *gate, double *__restrict__ dphi0)
 {
     if (gate && *gate) return;
     constexpr int F = 6 + ND;
@@ -250,6 +292,9 @@ __global__ __launch_bounds__(256) void k_phi32(const double *__restrict__ rho, c
     for (int c = 0; c < NC; c++)
 #pragma unroll
         for (int f = 0; f < F; f++) dphi[(size_t)(c * F + f) * N + i] = g[c][f];
+    if (dphi0)                                  // the rho slot in FP64, where the exponent assembly reads it
+#pragma unroll
+        for (int c = 0; c < NC; c++) dphi0[(size_t)(c * F) * N + i] = g[c][0];
 }
 
 // Hessian-vector product of Phi at every point (no reference counterpart: the
@@ -341,6 +386,12 @@ __global__ void k_set_point(const double *src, double *sol, double *rho, int n, 
     double x = src[i];
     sol[i] = x;
     rho[i] = mask ? mask[i] * x : exp(x);
+}
+
+__global__ void k_to_f32(const double *x, float *y, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = float(x[i]);
 }
 
 __global__ void k_exp(const double *x, double *rho, int n)

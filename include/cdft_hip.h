@@ -127,9 +127,18 @@ int cdft_system_npoints(const cdft_sys *sys);
  * (src/models/models.jl:86-91,131-200). */
 int cdft_functional_eval(cdft_sys *sys, const double *rho, double *dfdrho, double *f_res, double *fval);
 
+/* The same derivative from the FP32 map of a mixed-precision system (weighted
+ * densities, Phi's gradient and transposed convolutions in FP32, compensated sums
+ * in 1D, the n3-singular factors and the result in FP64): the map the mixed solve
+ * iterates before its switch to FP64.  CDFT_ERR_UNSUPPORTED for an FP64 system.
+ * (No reference counterpart: the reference runs one precision throughout,
+ * src/base/devices.jl:29-40.) */
+int cdft_functional_eval_fp32(cdft_sys *sys, const double *rho, double *dfdrho);
+
 /* Euler-Lagrange fixed point: rho is overwritten with the converged profile.
- * Plain path: one system, per-operator kernels, Anderson control on the host
- * with device reductions.  Replaces converge!(prob, ::AASol, rho)
+ * Plain path: one system, per-operator kernels; the whole Anderson iteration,
+ * its control flow included, runs on the device from a captured hipGraph and the
+ * host reads the solver state once per 16 map evaluations.  Replaces converge!(prob, ::AASol, rho)
  * (src/methods/converge.jl:220-254) and aasol (src/utils/anderson.jl:301-443). */
 int cdft_converge(cdft_sys *sys, double *rho_inout, const cdft_solver_opts *opts, cdft_result *result);
 
