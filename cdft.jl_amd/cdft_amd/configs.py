@@ -17,12 +17,41 @@
                   non-overlapping hard spheres, radii ~ U[sigma, 2.5 sigma], from seed 0.
                   Solved with anderson_start=0.5 (MATRIX_3D_SOLVER): the damped Picard
                   warm-up makes slow progress on this system, and Anderson takes over earlier.
+  matrix_2d       the same mixture in a periodic square with a matrix of hard disks
+                  (infinite cylinders), on a 2D grid
+  c1c10_lamellar  the methane / n-decane interface as a TwoPhase2DLamCart: the 1D grid
+                  along x, uniform along y
 """
 import numpy as np
 
 from .bulk import PCSAFT
-from .structure import HardSphereMatrix, Steele, TwoPhase1DCart, Uniform1DCart, Uniform3DCart
+from .structure import (HardSphereMatrix, Steele, TwoPhase1DCart, TwoPhase2DLamCart, Uniform1DCart, Uniform2DCart,
+                        Uniform3DCart)
 from .system import DFTOptions, DFTSystem, initialize_profiles
+
+
+def c1c10_lamellar(options=None, ngrid=(2048, 16), box=20.0, ywidth=2.0, x=(0.5, 0.5)):
+    model = PCSAFT(["methane", "n-decane"])
+    T = 300.0
+    x = np.asarray(x, dtype=np.float64)
+    p, vl, vv, y = model.bubble_pressure(T, x)
+    L = model.length_scale()
+    structure = TwoPhase2DLamCart((p, T), x / vl, y / vv, [[-box * L, box * L], [0.0, ywidth * L]], ngrid)
+    system = DFTSystem(model, structure, None, options)
+    return system, initialize_profiles(system)
+
+
+def matrix_2d(options=None, n=256, box=30.0, ndisks=16, seed=0, T=200.0, p=1e7, x=(0.6, 0.3, 0.1),
+              components=("methane", "ethane", "propane"), rmin=1.0, rmax=2.5, combining="nocombining"):
+    model = PCSAFT(list(components), combining=combining)
+    x = np.asarray(x, dtype=np.float64)
+    v = model.volume(p, T, x, phase="liquid")
+    L = model.length_scale()
+    side = box * L
+    structure = Uniform2DCart((p, T), x / v, [[0.0, side]] * 2, n)
+    field = HardSphereMatrix.random(ndisks, rmin * L, rmax * L, side, seed=seed, ndim=2) if ndisks else None
+    system = DFTSystem(model, structure, field, options)
+    return system, initialize_profiles(system)
 
 
 def methane_wall(options=None, ngrid=201):

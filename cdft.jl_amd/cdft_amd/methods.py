@@ -30,9 +30,8 @@ def surface_tension_system(system, rho):
     p, T = st.conditions
     F = free_energy(system, rho)
     mu = bulk_chemical_potential(system)
-    dz = st.dz()
-    chem = sum(mu[i] * simpson(rho[i], dz) for i in range(system.nc))
-    return F * KB * T - chem + p * simpson(np.ones(system.N), dz)
+    chem = sum(mu[i] * integrate(st, rho[i]) for i in range(system.nc))
+    return F * KB * T - chem + p * integrate(st, np.ones(system.N))
 
 
 def surface_tension(model_or_system, T_or_rho, x=None, *, ngrid=201, box=10.0, options=None, **kw):

@@ -1,18 +1,21 @@
 """Structures (grids + state conditions) and external fields, host side.
 
 Mirrors the reference's constructors:
-  Uniform1DCart   src/base/structure.jl:310
-  Uniform3DCart   src/base/structure.jl:335-352
-  TwoPhase1DCart  src/base/structure.jl:491 (``TwoPhase1DCart``)
+  Uniform1DCart      src/base/structure.jl:310
+  Uniform2DCart      src/base/structure.jl:315-332
+  Uniform3DCart      src/base/structure.jl:335-352
+  TwoPhase1DCart     src/base/structure.jl:491 (``TwoPhase1DCart``)
+  TwoPhase2DLamCart  src/base/structure.jl:494-513 (lamellar interface along the first axis)
   Steele          src/structure/external_fields/steele.jl:31-46 (database/Steele/steele_like.csv)
 and adds a field the reference does not have:
-  HardSphereMatrix  a periodic matrix of impenetrable spheres on a 3D grid
+  HardSphereMatrix  a periodic matrix of impenetrable spheres on a 3D grid (disks,
+                    i.e. infinite cylinders, on a 2D grid)
 """
 import numpy as np
 
 
 class Structure:
-    """A 1D or 3D Cartesian grid with (p, T) conditions and bulk densities."""
+    """A 1D, 2D or 3D Cartesian grid with (p, T) conditions and bulk densities."""
 
     def __init__(self, conditions, rho_bulk, bounds, ngrid, topology="uniform", rho_bulk2=None, ndim=1):
         p, T = conditions
@@ -50,7 +53,7 @@ class Structure:
         return (1.0 - t) * lb + t * ub
 
     def coords(self):
-        """1D: the coordinate vector; 3D: the list of per-axis coordinate vectors."""
+        """1D: the coordinate vector; 2D and 3D: the list of per-axis coordinate vectors."""
         if self.ndim == 1:
             return self.axis_coords(0)
         return [self.axis_coords(a) for a in range(self.ndim)]
@@ -69,6 +72,18 @@ class Structure:
 
 def Uniform1DCart(conditions, rho_bulk, bounds, ngrid):
     return Structure(conditions, rho_bulk, bounds, ngrid, "uniform")
+
+
+def Uniform2DCart(conditions, rho_bulk, bounds, ngrid):
+    """Uniform2DCart(conditions, rho_bulk, bounds (2x2), ngrid) (base/structure.jl:315-332)."""
+    return Structure(conditions, rho_bulk, bounds, ngrid, "uniform", ndim=2)
+
+
+def TwoPhase2DLamCart(conditions, rho_bulk, rho_bulk2, bounds, ngrid):
+    """TwoPhase2DLamCart(conditions, rho_bulk, rho_bulk2, bounds (2x2), ngrid): a lamellar
+    two-phase interface along the first axis, uniform along the second
+    (base/structure.jl:494-513)."""
+    return Structure(conditions, rho_bulk, bounds, ngrid, "two_phase", rho_bulk2, ndim=2)
 
 
 def Uniform3DCart(conditions, rho_bulk, bounds, ngrid):
@@ -128,15 +143,16 @@ class HardSphereMatrix:
     component.  Any cap above about 100 gives the same fixed point, because the
     exponent is clamped to [-100, 100] (converge.jl:150)."""
 
-    def __init__(self, centres, radii, cap=500.0):
-        self.centres = np.asarray(centres, dtype=np.float64).reshape(-1, 3)
+    def __init__(self, centres, radii, cap=500.0, ndim=3):
+        self.ndim = int(ndim)
+        self.centres = np.asarray(centres, dtype=np.float64).reshape(-1, self.ndim)
         self.radii = np.asarray(radii, dtype=np.float64).reshape(-1)
         self.cap = float(cap)
 
     @classmethod
-    def random(cls, n, rmin, rmax, box, seed=0, cap=500.0, max_tries=200000):
-        """n non-overlapping spheres with radii ~ U[rmin, rmax] in the periodic cube
-        [0, box)^3, by random sequential addition from `seed`."""
+    def random(cls, n, rmin, rmax, box, seed=0, cap=500.0, max_tries=200000, ndim=3):
+        """n non-overlapping spheres (disks for ndim = 2) with radii ~ U[rmin, rmax] in the
+        periodic cube [0, box)^ndim, by random sequential addition from `seed`."""
         rng = np.random.default_rng(seed)
         cs, rs = [], []
         tries = 0
@@ -145,7 +161,7 @@ class HardSphereMatrix:
             if tries > max_tries:
                 raise RuntimeError("could not place the spheres without overlap")
             r = rng.uniform(rmin, rmax)
-            c = rng.uniform(0.0, box, size=3)
+            c = rng.uniform(0.0, box, size=ndim)
             ok = True
             for c2, r2 in zip(cs, rs):
                 d = c - c2
@@ -156,16 +172,19 @@ class HardSphereMatrix:
             if ok:
                 cs.append(c)
                 rs.append(r)
-        return cls(np.array(cs), np.array(rs), cap)
+        return cls(np.array(cs), np.array(rs), cap, ndim)
 
     def mask(self, structure):
         """Boolean (N,) mask of the points inside a sphere, column-major (x fastest)."""
-        xs = [structure.axis_coords(a) for a in range(3)]
+        nd = structure.ndim
+        if nd != self.ndim:
+            raise ValueError(f"a {self.ndim}D matrix on a {nd}D grid")
+        xs = [structure.axis_coords(a) for a in range(nd)]
         lens = [ub - lb for lb, ub in structure.bounds]
-        inside = np.zeros(structure.ngrid[::-1], dtype=bool)             # [z, y, x]
+        inside = np.zeros(structure.ngrid[::-1], dtype=bool)             # [z,] y, x
         for c, r in zip(self.centres, self.radii):
             d2, idx = [], []
-            for a in range(3):                                           # per-axis window of the sphere
+            for a in range(nd):                                          # per-axis window of the sphere
                 d = xs[a] - c[a]
                 d -= lens[a] * np.round(d / lens[a])
                 sel = np.nonzero(d * d < r * r)[0]
@@ -173,8 +192,12 @@ class HardSphereMatrix:
                 d2.append(d[sel] ** 2)
             if min(len(i) for i in idx) == 0:
                 continue
-            blk = d2[2][:, None, None] + d2[1][None, :, None] + d2[0][None, None, :] < r * r
-            sub = np.ix_(idx[2], idx[1], idx[0])
+            if nd == 3:
+                blk = d2[2][:, None, None] + d2[1][None, :, None] + d2[0][None, None, :] < r * r
+                sub = np.ix_(idx[2], idx[1], idx[0])
+            else:
+                blk = d2[1][:, None] + d2[0][None, :] < r * r
+                sub = np.ix_(idx[1], idx[0])
             inside[sub] |= blk
         return inside.reshape(-1)
 

@@ -130,7 +130,7 @@ def _clamp_init(rho):
 
 
 def initialize_profiles(system, noise=0.0, rng=None):
-    """Seed profile (nc, N): uniform bulk, or the TwoPhase1DCart cos_prof seed, times
+    """Seed profile (nc, N): uniform bulk, or the TwoPhase1DCart / TwoPhase2DLamCart cos_prof seed, times
     exp(-beta V/10) per external field, then clamped (structure.jl:50-94)."""
     st = system.structure
     N, nc = system.N, system.nc
@@ -139,7 +139,9 @@ def initialize_profiles(system, noise=0.0, rng=None):
         lb, ub = st.bounds[0]
         H = ub - lb
         T = st.conditions[1]
-        x = st.coords() / (ub - lb)
+        x = st.axis_coords(0) / (ub - lb)
+        if st.ndim == 2:                    # lamellar: the 1D profile along x, uniform along y (two_phase.jl:37-69)
+            x = np.tile(x, st.ngrid[1])
         shift = ub / 4 + 3 * lb / 4
         for i, pure in enumerate(system.model.split_pure()):
             Tc = pure.crit_pure()[0]

@@ -81,17 +81,23 @@ double simpson_nd(const double *f, int nd, const int *ngrid, const double *dz)
 {
     if (nd == 1) return simpson1d(f, ngrid[0], dz[0]);
     auto wgt = [](int k, int n) { return (k == 0 || k == n - 1) ? 1.0 : ((k % 2 == 1) ? 4.0 : 2.0); };
+    const int n2 = nd == 3 ? ngrid[2] : 1;
     double s = 0.0;
-    for (int i2 = 0; i2 < ngrid[2]; i2++)
+    for (int i2 = 0; i2 < n2; i2++)
         for (int i1 = 0; i1 < ngrid[1]; i1++) {
-            const double w12 = wgt(i1, ngrid[1]) * wgt(i2, ngrid[2]);
+            const double w12 = wgt(i1, ngrid[1]) * (nd == 3 ? wgt(i2, n2) : 1.0);
             const double *row = f + ((size_t)i2 * ngrid[1] + i1) * ngrid[0];
             double r = 0.0;
             for (int i0 = 0; i0 < ngrid[0]; i0++) r += wgt(i0, ngrid[0]) * row[i0];
             s += w12 * r;
         }
-    return s * (dz[0] / 3.0) * (dz[1] / 3.0) * (dz[2] / 3.0);
+    double sc = 1.0;
+    for (int a = 0; a < nd; a++) sc *= dz[a] / 3.0;
+    return s * sc;
 }
+
+// 2D and 3D grids take the spectral path (a 2D grid is a 3D grid with one z plane).
+inline bool spectral(const cdft_sys *s) { return s->nd >= 2; }
 
 // Arguments of k_update: per-field parts (1D) or the finished transposed sum (3D).
 struct UpdSrc {
@@ -100,7 +106,7 @@ struct UpdSrc {
 };
 UpdSrc upd_src(const cdft_sys *s)
 {
-    if (s->nd == 3) return {s->part, nullptr, 1};
+    if (spectral(s)) return {s->part, nullptr, 1};
     return {s->dphi, s->part, s->F};
 }
 
@@ -213,6 +219,7 @@ template <typename T>
 void launch_weighted3d(cdft_sys *s, const double *src, T *nout)
 {
     const int N = s->N, F = s->F;
+    const bool plane = (s->g3.n[2] == 1);            // a 2D grid: no z passes
     for (int c = 0; c < s->nc; c++) {
         Pass3 p;
         std::memset(&p, 0, sizeof(p));
@@ -221,14 +228,19 @@ void launch_weighted3d(cdft_sys *s, const double *src, T *nout)
         fft3_pass<T, 0, IN_RHO, OUT_CPLX>(s, p, false);
         p.in = s->A3;
         fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, p, false);
-        fft3_pass<T, 2, IN_CPLX, OUT_CPLX>(s, p, false);
+        if (!plane) fft3_pass<T, 2, IN_CPLX, OUT_CPLX>(s, p, false);
         for (int q = 0; q < 4; q++) {
             Pass3 pq = pair_pass<T>(s, c, q);
             pq.in = s->A3;
             pq.out = s->B3;
-            fft3_pass<T, 2, IN_SPEC, OUT_CPLX>(s, pq, true);
+            if (plane) {
+                fft3_pass<T, 1, IN_SPEC, OUT_CPLX>(s, pq, true);
+            } else {
+                fft3_pass<T, 2, IN_SPEC, OUT_CPLX>(s, pq, true);
+                pq.in = s->B3;
+                fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, pq, true);
+            }
             pq.in = s->B3;
-            fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, pq, true);
             pq.out = nout + (size_t)(c * F + kPair3[q][0]) * N;
             pq.out2 = nout + (size_t)(c * F + kPair3[q][1]) * N;
             pq.scale = s->NA / (double)N;
@@ -244,6 +256,7 @@ template <typename T>
 void launch_transposed3d(cdft_sys *s, const T *dphi, double *dtot)
 {
     const int N = s->N, F = s->F;
+    const bool plane = (s->g3.n[2] == 1);            // a 2D grid: no z passes
     for (int c = 0; c < s->nc; c++) {
         if (s->B3x[2]) {
             ZPairs3 zp = zpairs_for<T>(s, c);
@@ -277,16 +290,21 @@ void launch_transposed3d(cdft_sys *s, const T *dphi, double *dtot)
             pq.out = s->B3;
             fft3_pass<T, 0, IN_PAIR, OUT_CPLX>(s, pq, false);
             pq.in = s->B3;
-            fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, pq, false);
-            pq.out = s->S3;
             pq.first = (q == 0);
-            fft3_pass<T, 2, IN_CPLX, OUT_ACCUM>(s, pq, false);
+            if (plane) {
+                pq.out = s->S3;
+                fft3_pass<T, 1, IN_CPLX, OUT_ACCUM>(s, pq, false);
+            } else {
+                fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, pq, false);
+                pq.out = s->S3;
+                fft3_pass<T, 2, IN_CPLX, OUT_ACCUM>(s, pq, false);
+            }
         }
         Pass3 p;
         std::memset(&p, 0, sizeof(p));
         p.in = s->S3;
         p.out = s->S3;
-        fft3_pass<T, 2, IN_CPLX, OUT_CPLX>(s, p, true);
+        if (!plane) fft3_pass<T, 2, IN_CPLX, OUT_CPLX>(s, p, true);
         fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, p, true);
         p.out = dtot + (size_t)c * N;
         p.base = dphi + (size_t)(c * F) * N;
@@ -298,7 +316,7 @@ void launch_transposed3d(cdft_sys *s, const T *dphi, double *dtot)
 template <int NC, int AS>
 void launch_phi32_as(cdft_sys *s)
 {
-    if (s->nd == 3)
+    if (spectral(s))
         hipLaunchKernelGGL((k_phi32<NC, 3, AS>), dim3(nblocks(s->N)), dim3(256), 0, s->ctx->stream, s->rho, s->n32,
                            s->dphi32, s->N, s->NA, s->pc, s->gate, (double *)nullptr);
     else
@@ -353,7 +371,7 @@ void launch_functional32(cdft_sys *s)
 template <int NC, int AS>
 void launch_phi_as(cdft_sys *s, hipStream_t st)
 {
-    if (s->nd == 3)
+    if (spectral(s))
         hipLaunchKernelGGL((k_phi<NC, 3, AS>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, s->n, s->dphi, s->fval,
                            s->N, s->NA, s->pc, s->gate);
     else
@@ -387,7 +405,7 @@ void launch_phi_any(cdft_sys *s)
 // on the vector slot.
 void launch_transposed(cdft_sys *s)
 {
-    if (s->nd == 3) {
+    if (spectral(s)) {
         launch_transposed3d<double>(s, s->dphi, s->part);
         return;
     }
@@ -399,7 +417,7 @@ void launch_transposed(cdft_sys *s)
 // Weighted densities of rho (s->rho) into s->n.
 void launch_weighted(cdft_sys *s)
 {
-    if (s->nd == 3) {
+    if (spectral(s)) {
         launch_weighted3d<double>(s, s->rho, s->n);
         return;
     }
@@ -419,7 +437,7 @@ void launch_functional(cdft_sys *s)
 template <int NC, int AS>
 void launch_phi_hvp_as(cdft_sys *s, const double *v, hipStream_t st)
 {
-    if (s->nd == 3)
+    if (spectral(s))
         hipLaunchKernelGGL((k_phi_hvp<NC, 3, AS>), dim3(nblocks(s->N)), dim3(256), 0, st, s->rho, v, s->n, s->dphi,
                            s->N, s->NA, s->pc, s->gate);
     else
@@ -443,7 +461,7 @@ void launch_hvp(cdft_sys *s, const double *v)
 {
     hipStream_t st = s->ctx->stream;
     int N = s->N, F = s->F, nc = s->nc;
-    if (s->nd == 3) {
+    if (spectral(s)) {
         launch_weighted3d<double>(s, v, s->dphi);
     } else {
         dim3 g((N + CIRC_TW - 1) / CIRC_TW, nc * (F - 1));
@@ -525,17 +543,17 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
     if (!ctx || !d || !out) return fail(ctx, CDFT_ERR_ARG, "cdft_system_create: NULL argument");
     *out = nullptr;
     if (d->nc < 1 || d->nc > 4) return fail(ctx, CDFT_ERR_UNSUPPORTED, "nc must be in 1..4");
-    if (d->ndim != 1 && d->ndim != 3) return fail(ctx, CDFT_ERR_UNSUPPORTED, "only 1D and 3D Cartesian grids are supported");
+    if (d->ndim < 1 || d->ndim > 3) return fail(ctx, CDFT_ERR_UNSUPPORTED, "only 1D, 2D and 3D Cartesian grids are supported");
     if (d->ndim == 1 && d->ngrid[0] < 3) return fail(ctx, CDFT_ERR_ARG, "ngrid must be >= 3");
-    if (d->ndim == 3) {
+    if (d->ndim >= 2) {
         long N3 = 1;
-        for (int a = 0; a < 3; a++) {
+        for (int a = 0; a < d->ndim; a++) {
             const int g = d->ngrid[a];
             if (g < 16 || g > 512 || (g & (g - 1)))
-                return fail(ctx, CDFT_ERR_UNSUPPORTED, "3D grids need power-of-two axes of 16..512 points");
+                return fail(ctx, CDFT_ERR_UNSUPPORTED, "2D and 3D grids need power-of-two axes of 16..512 points");
             N3 *= g;
         }
-        if (N3 > (1L << 27)) return fail(ctx, CDFT_ERR_UNSUPPORTED, "3D grid larger than 2^27 points");
+        if (N3 > (1L << 27)) return fail(ctx, CDFT_ERR_UNSUPPORTED, "grid larger than 2^27 points");
     }
     for (int a = 0; a < d->ndim; a++)
         if (!(d->ub[a] > d->lb[a])) return fail(ctx, CDFT_ERR_ARG, "bounds must satisfy ub > lb");
@@ -573,7 +591,15 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
         s->ub[a] = d->ub[a];
         s->N *= d->ngrid[a];
     }
-    s->F = 6 + d->ndim;
+    // a 2D grid runs as a 3D grid with one z plane: its third vector slot has a zero
+    // weight (the z frequencies are all 0), so its weighted density is zero and Phi
+    // is that of the two-component vector weight (PCSAFT.jl:55-61 with D = 2)
+    if (d->ndim == 2) {
+        s->ngrid[2] = 1;
+        s->lb[2] = 0.0;
+        s->ub[2] = 1.0;
+    }
+    s->F = d->ndim == 1 ? 7 : 9;
     s->T = d->T;
     s->p = d->p;
     s->L = d->L;
@@ -632,7 +658,7 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
     auto alloc = [&](double **p, size_t count) { return hipMalloc((void **)p, sizeof(double) * count); };
     hipError_t e = hipSuccess;
     double *ddev = nullptr;
-    const bool d3 = (s->nd == 3);
+    const bool d3 = spectral(s);
     if ((!d3 && (e = alloc(&s->h, (size_t)nc * F * N)) != hipSuccess) || (e = alloc(&s->rho, n)) != hipSuccess ||
         (e = alloc(&s->n, (size_t)nc * F * N)) != hipSuccess || (e = alloc(&s->dphi, (size_t)nc * F * N)) != hipSuccess ||
         (e = alloc(&s->part, d3 ? (size_t)n : (size_t)nc * (F - 1) * N)) != hipSuccess ||
@@ -696,7 +722,7 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
             G.n[a] = s->ngrid[a];
             while ((1 << G.lg[a]) < G.n[a]) G.lg[a]++;
             G.mult[a] = s->L / (s->ub[a] - s->lb[a]);             // fftfreq(n, n/(ub-lb)) * L
-            if (!ctx->twiddles.count(G.n[a])) {
+            if (G.n[a] > 1 && !ctx->twiddles.count(G.n[a])) {
                 double *t64 = nullptr;
                 if ((e = alloc(&t64, (size_t)2 * G.n[a])) == hipSuccess) {
                     hipLaunchKernelGGL(k_setup_tw, dim3(nblocks(G.n[a])), dim3(256), 0, ctx->stream, G.n[a], t64);
@@ -1148,6 +1174,7 @@ int cdft_ddft_rhs(cdft_sys *s, const double *eta, double *deta)
                            (const int *)nullptr);
         hipLaunchKernelGGL(k_ddft_combine, dim3(nblocks(n)), dim3(256), 0, st, d, N, nc, detad);
     } else {
+        const bool plane = (s->g3.n[2] == 1);        // a 2D grid: no z passes
         double *mu = nullptr;
         HIPCHECK_CTX(ctx, tmp.get(&mu, n));
         hipLaunchKernelGGL(k_ddft_src, dim3(nblocks(n)), dim3(256), 0, st, etad, s->out, s->vext, N, nc,
@@ -1161,16 +1188,21 @@ int cdft_ddft_rhs(cdft_sys *s, const double *eta, double *deta)
             fft3_pass<double, 0, IN_PAIR, OUT_CPLX>(s, p, false);          // FFT(eta + i mu)
             p.in = s->A3;
             fft3_pass<double, 1, IN_CPLX, OUT_CPLX>(s, p, false);
-            fft3_pass<double, 2, IN_CPLX, OUT_CPLX>(s, p, false);
-            for (int kind = 0; kind < 4; kind++) {
+            if (!plane) fft3_pass<double, 2, IN_CPLX, OUT_CPLX>(s, p, false);
+            for (int kind = 0; kind < (plane ? 3 : 4); kind++) {    // 2D: no z gradient
                 Pass3 q;
                 std::memset(&q, 0, sizeof(q));
                 q.in = s->A3;
                 q.out = s->B3;
                 q.mkind = kind;
-                fft3_pass<double, 2, IN_MULT, OUT_CPLX>(s, q, true);
+                if (plane) {
+                    fft3_pass<double, 1, IN_MULT, OUT_CPLX>(s, q, true);
+                } else {
+                    fft3_pass<double, 2, IN_MULT, OUT_CPLX>(s, q, true);
+                    q.in = s->B3;
+                    fft3_pass<double, 1, IN_CPLX, OUT_CPLX>(s, q, true);
+                }
                 q.in = s->B3;
-                fft3_pass<double, 1, IN_CPLX, OUT_CPLX>(s, q, true);
                 q.out = detad + (size_t)c * N;
                 q.scale = 1.0 / (double)N;
                 if (kind == 0) fft3_pass<double, 0, IN_CPLX, OUT_DDFT0>(s, q, true);

@@ -107,3 +107,41 @@ def test_adsorption_gradient_is_weights_over_volume():
     # the objective is linear in rho: its value equals g . rho
     from cdft_amd.methods import adsorption_system
     assert g @ rho.reshape(-1) == pytest.approx(adsorption_system(s, rho)[1], rel=1e-13)
+
+
+def test_uniform2d_structure_and_weights():
+    from cdft_amd import Uniform2DCart
+    with pytest.raises(ValueError):
+        Uniform2DCart((1e5, 200.0), [1.0], [[0, 1], [0, 1], [0, 1]], 16)      # 3 axes given
+    st = Uniform2DCart((1e5, 200.0), [1.0], [[0, 1.5], [-1, 2]], (16, 8))
+    assert st.ndim == 2 and st.npoints == 128 and st.volume() == pytest.approx(4.5)
+    rng = np.random.default_rng(1)
+    fx, fy = rng.random(16), rng.random(8)
+    F = (fy[:, None] * fx[None, :]).reshape(-1)
+    assert integrate(st, F) == pytest.approx(simpson(fx, 1.5 / 16) * simpson(fy, 3 / 8), rel=1e-13)
+
+
+def test_hard_disk_matrix_placement_and_mask():
+    box = 10.0
+    m = HardSphereMatrix.random(6, 0.8, 1.6, box, seed=4, ndim=2)
+    c, r = m.centres, m.radii
+    assert c.shape == (6, 2)
+    for i in range(len(r)):
+        for j in range(i):
+            d = c[i] - c[j]
+            d -= box * np.round(d / box)
+            assert np.sqrt(d @ d) >= r[i] + r[j]
+    from cdft_amd import Uniform2DCart
+    st = Uniform2DCart((1e5, 200.0), [1.0], [[0, box]] * 2, 256)
+    frac = np.pi * np.sum(r ** 2) / box ** 2
+    assert m.mask(st).mean() == pytest.approx(frac, rel=0.03)
+    with pytest.raises(ValueError):
+        m.mask(Uniform3DCart((1e5, 200.0), [1.0], [[0, box]] * 3, 16))
+
+
+def test_lamellar_seed_replicates_the_1d_seed():
+    s1, r1 = configs.c1c10_interface(ngrid=128, box=10.0)
+    s2, r2 = configs.c1c10_lamellar(ngrid=(128, 16), box=10.0)
+    rows = r2.reshape(2, 16, 128)
+    for j in range(16):
+        assert np.array_equal(rows[:, j, :], r1)
