@@ -27,7 +27,7 @@ EXPORTS = [
     "cdft_converge_batch", "cdft_converge_batch_device", "cdft_converge_batch_device_noalloc",
     "cdft_last_kernel_ms",
     "cdft_surface_tension", "cdft_adsorption", "cdft_phi_vjp", "cdft_hvp", "cdft_adjoint_solve",
-    "cdft_ddft_rhs",
+    "cdft_ddft_rhs", "cdft_system_tune", "cdft_system_get_tuning", "cdft_system_set_tuning",
 ]
 
 
@@ -60,6 +60,23 @@ class Result(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("lines_per_tile", C.c_int * 2), ("fused_z", C.c_int * 2), ("ms", C.c_double * 2),
+                ("tuned", C.c_int)]
+
+    def as_dict(self):
+        return {"lines_per_tile": list(self.lines_per_tile), "fused_z": list(self.fused_z),
+                "ms": list(self.ms), "tuned": self.tuned}
+
+    @classmethod
+    def make(cls, lines_per_tile, fused_z, tuned=0):
+        t = cls()
+        t.lines_per_tile[0], t.lines_per_tile[1] = lines_per_tile
+        t.fused_z[0], t.fused_z[1] = fused_z
+        t.tuned = tuned
+        return t
+
+
 class CDFTError(RuntimeError):
     pass
 
@@ -86,6 +103,9 @@ def load():
     L.cdft_system_npoints.argtypes = [vp]
     L.cdft_functional_eval.argtypes = [vp, _dp, _dp, _dp, _dp]
     L.cdft_functional_eval_fp32.argtypes = [vp, _dp, _dp]
+    L.cdft_system_tune.argtypes = [vp, C.c_int, C.POINTER(Tuning)]
+    L.cdft_system_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.cdft_system_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.cdft_converge.argtypes = [vp, _dp, C.POINTER(SolverOpts), C.POINTER(Result)]
     L.cdft_converge_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(_dp),
                                       C.POINTER(SolverOpts), C.POINTER(Result)]
@@ -205,6 +225,21 @@ class DeviceSystem:
         check(load().cdft_functional_eval(self.handle, ptr(rho), ptr(out), ptr(fres),
                                           ptr(fval) if want_fval else None), self.ctx.handle)
         return out, float(fres[0]), fval
+
+    def tune(self, force=False):
+        """Measure the line-FFT variants (2D/3D) and apply the fastest; returns the choice."""
+        t = Tuning()
+        check(load().cdft_system_tune(self.handle, int(bool(force)), C.byref(t)), self.ctx.handle)
+        return t.as_dict()
+
+    def get_tuning(self):
+        t = Tuning()
+        check(load().cdft_system_get_tuning(self.handle, C.byref(t)), self.ctx.handle)
+        return t.as_dict()
+
+    def set_tuning(self, lines_per_tile, fused_z):
+        t = Tuning.make(lines_per_tile, fused_z)
+        check(load().cdft_system_set_tuning(self.handle, C.byref(t)), self.ctx.handle)
 
     def functional_eval_fp32(self, rho):
         """delta F_res / delta rho from the FP32 map of a mixed-precision system."""

@@ -110,25 +110,42 @@ UpdSrc upd_src(const cdft_sys *s)
     return {s->dphi, s->part, s->F};
 }
 
-// ---- 3D passes: one line FFT along axis AX over tiles of TB3 lines
-template <typename T, int AX, int IN, int OUT, int EPT>
+// ---- 3D passes: one line FFT along axis AX over tiles of TB lines (TB: the
+// system's tuned lines per tile for precision T)
+template <typename T, int AX, int IN, int OUT, int EPT, int TB>
 void fft3_launch(const cdft_sys *s, const Pass3 &p)
 {
     const Grid3 &G = s->g3;
     const int L = G.n[AX];
-    const size_t lds = (size_t)L * TB3 * sizeof(cx<T>);
+    const size_t lds = (size_t)L * TB * sizeof(cx<T>);
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_fft3<T, AX, IN, OUT, EPT>,
+        (void)hipFuncSetAttribute((const void *)k_fft3<T, AX, IN, OUT, EPT, TB>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
     long nblk;
-    if (AX == 0) nblk = (G.N / G.n[0]) / TB3;
-    else if (AX == 1) nblk = (long)(G.n[0] / TB3) * G.n[2];
-    else nblk = (long)(G.n[0] / TB3) * G.n[1];
-    hipLaunchKernelGGL((k_fft3<T, AX, IN, OUT, EPT>), dim3((unsigned)nblk), dim3(256), lds, s->ctx->stream, G, p);
+    if (AX == 0) nblk = (G.N / G.n[0]) / TB;
+    else if (AX == 1) nblk = (long)(G.n[0] / TB) * G.n[2];
+    else nblk = (long)(G.n[0] / TB) * G.n[1];
+    hipLaunchKernelGGL((k_fft3<T, AX, IN, OUT, EPT, TB>), dim3((unsigned)nblk), dim3(256), lds, s->ctx->stream, G, p);
 }
+
+// EPT = L TB / 1024 radix-4 butterflies per thread (0: short lines, L TB <= 512,
+// with guards); the tuner keeps L TB <= 4096.
+template <typename T, int AX, int IN, int OUT, int TB>
+void fft3_launch_tb(const cdft_sys *s, const Pass3 &p)
+{
+    switch (s->g3.n[AX] * TB / 1024) {
+    case 4: fft3_launch<T, AX, IN, OUT, 4, TB>(s, p); break;
+    case 2: fft3_launch<T, AX, IN, OUT, 2, TB>(s, p); break;
+    case 1: fft3_launch<T, AX, IN, OUT, 1, TB>(s, p); break;
+    default: fft3_launch<T, AX, IN, OUT, 0, TB>(s, p); break;
+    }
+}
+
+template <typename T>
+inline int tuned_tb(const cdft_sys *s) { return s->tune.lines_per_tile[sizeof(T) == sizeof(float) ? 1 : 0]; }
 
 template <typename T, int AX, int IN, int OUT>
 void fft3_pass(const cdft_sys *s, Pass3 p, bool inverse)
@@ -136,12 +153,10 @@ void fft3_pass(const cdft_sys *s, Pass3 p, bool inverse)
     p.inverse = inverse ? 1 : 0;
     p.gate = s->gate;
     p.tw = reinterpret_cast<const cx<double> *>(s->ctx->twiddles.at(s->g3.n[AX]));
-    static_assert(TB3 == 8, "EPT = L TB3 / 1024 below");
-    switch (s->g3.n[AX]) {
-    case 512: fft3_launch<T, AX, IN, OUT, 4>(s, p); break;
-    case 256: fft3_launch<T, AX, IN, OUT, 2>(s, p); break;
-    case 128: fft3_launch<T, AX, IN, OUT, 1>(s, p); break;
-    default: fft3_launch<T, AX, IN, OUT, 0>(s, p); break;        // 16, 32, 64
+    switch (tuned_tb<T>(s)) {
+    case 4: fft3_launch_tb<T, AX, IN, OUT, 4>(s, p); break;
+    case 16: fft3_launch_tb<T, AX, IN, OUT, 16>(s, p); break;
+    default: fft3_launch_tb<T, AX, IN, OUT, 8>(s, p); break;
     }
 }
 
@@ -258,7 +273,7 @@ void launch_transposed3d(cdft_sys *s, const T *dphi, double *dtot)
     const int N = s->N, F = s->F;
     const bool plane = (s->g3.n[2] == 1);            // a 2D grid: no z passes
     for (int c = 0; c < s->nc; c++) {
-        if (s->B3x[2]) {
+        if (s->B3x[2] && s->tune.fused_z[sizeof(T) == sizeof(float) ? 1 : 0]) {
             ZPairs3 zp = zpairs_for<T>(s, c);
             double *bq[4] = {s->B3, s->B3x[0], s->B3x[1], s->B3x[2]};
             for (int q = 0; q < 4; q++) {
@@ -701,6 +716,7 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
                 break;
             }
     }
+    for (int pr = 0; pr < 2; pr++) s->tune.fused_z[pr] = s->B3x[2] ? 1 : 0;
     if (d->vext) {
         s->has_vext = true;
         if ((e = alloc(&s->vext, n)) != hipSuccess ||
@@ -1018,8 +1034,147 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     return CDFT_OK;
 }
 
+// ---- autotuner of the spectral path's line-FFT variants
+// One weighted-density + transposed-convolution sequence (FP32 or FP64) with the
+// system's current variant, ms per sequence (mean of reps after one warm-up).
+static int time_spectral_seq(cdft_sys *s, bool fp32, int reps, double *ms)
+{
+    cdft_ctx *ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHECK_CTX(ctx, hipEventCreate(&e0));
+    struct EvFree { hipEvent_t a, b; ~EvFree() { if (a) (void)hipEventDestroy(a); if (b) (void)hipEventDestroy(b); } } ef{e0, nullptr};
+    HIPCHECK_CTX(ctx, hipEventCreate(&e1));
+    ef.b = e1;
+    auto seq = [&]() {
+        if (fp32) {
+            launch_weighted3d<float>(s, s->rho, s->n32);
+            launch_transposed3d<float>(s, s->dphi32, s->part);
+        } else {
+            launch_weighted3d<double>(s, s->rho, s->n);
+            launch_transposed3d<double>(s, s->dphi, s->part);
+        }
+    };
+    seq();
+    HIPCHECK_CTX(ctx, hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++) seq();
+    HIPCHECK_CTX(ctx, hipEventRecord(e1, st));
+    HIPCHECK_CTX(ctx, hipEventSynchronize(e1));
+    HIPCHECK_CTX(ctx, hipGetLastError());
+    float t = 0.0f;
+    HIPCHECK_CTX(ctx, hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    return CDFT_OK;
+}
+
+// A lines-per-tile count every axis of the grid can take: whole tiles, at most 4
+// butterflies per thread, and at most 64 KB of LDS (two workgroups per CU).
+static bool tb_fits(const cdft_sys *s, int tb, size_t elem)
+{
+    if (tb != 4 && tb != 8 && tb != 16) return false;
+    for (int a = 0; a < 3; a++) {
+        const int L = s->g3.n[a];
+        if (L == 1) continue;
+        if ((long)L * tb > 4096 || (size_t)L * tb * elem > 64 * 1024) return false;
+    }
+    return s->g3.n[0] % tb == 0 && (s->g3.N / s->g3.n[0]) % tb == 0;
+}
+
+static std::vector<long> tune_key(const cdft_sys *s)
+{
+    return {s->g3.n[0], s->g3.n[1], s->g3.n[2], s->nc, s->precision};
+}
+
+static int tune_spectral(cdft_sys *s, bool force)
+{
+    if (!spectral(s)) return CDFT_OK;
+    cdft_ctx *ctx = s->ctx;
+    const std::vector<long> key = tune_key(s);
+    if (!force) {
+        if (s->tune.tuned) return CDFT_OK;
+        auto it = ctx->tune_cache.find(key);
+        if (it != ctx->tune_cache.end()) {
+            s->tune = it->second;
+            return CDFT_OK;
+        }
+    }
+    HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
+    cdft_tuning best = s->tune;
+    const int nprec = (s->precision == CDFT_MIXED) ? 2 : 1;
+    const int reps = s->N >= (1L << 22) ? 2 : 4;
+    for (int pr = 0; pr < nprec; pr++) {
+        const size_t elem = pr ? sizeof(cx<float>) : sizeof(cx<double>);
+        double tbest = -1.0;
+        for (int tb : {8, 4, 16}) {
+            if (!tb_fits(s, tb, elem)) continue;
+            for (int fz = (s->B3x[2] ? 1 : 0); fz >= 0; fz--) {
+                s->tune.lines_per_tile[pr] = tb;
+                s->tune.fused_z[pr] = fz;
+                double ms = 0.0;
+                int rc = time_spectral_seq(s, pr == 1, reps, &ms);
+                if (rc != CDFT_OK) {
+                    s->tune = best;
+                    return rc;
+                }
+                if (tbest < 0 || ms < tbest) {
+                    tbest = ms;
+                    best.lines_per_tile[pr] = tb;
+                    best.fused_z[pr] = fz;
+                    best.ms[pr] = ms;
+                }
+            }
+        }
+        s->tune = best;
+    }
+    best.tuned = 1;
+    s->tune = best;
+    ctx->tune_cache[key] = best;
+    return CDFT_OK;
+}
+
+int cdft_system_tune(cdft_sys *s, int force, cdft_tuning *out)
+{
+    if (!s) return fail(nullptr, CDFT_ERR_ARG, "cdft_system_tune: NULL system");
+    int rc = tune_spectral(s, force != 0);
+    if (rc != CDFT_OK) return rc;
+    if (out) *out = s->tune;
+    return CDFT_OK;
+}
+
+int cdft_system_get_tuning(const cdft_sys *s, cdft_tuning *out)
+{
+    if (!s || !out) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_system_get_tuning: NULL argument");
+    *out = s->tune;
+    return CDFT_OK;
+}
+
+int cdft_system_set_tuning(cdft_sys *s, const cdft_tuning *in)
+{
+    if (!s || !in) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_system_set_tuning: NULL argument");
+    cdft_tuning t = *in;
+    for (int pr = 0; pr < 2; pr++) {
+        if (spectral(s) && !tb_fits(s, t.lines_per_tile[pr], pr ? sizeof(cx<float>) : sizeof(cx<double>)))
+            return fail(s->ctx, CDFT_ERR_ARG, "cdft_system_set_tuning: lines per tile the grid cannot take");
+        if (t.fused_z[pr] && !s->B3x[2])
+            return fail(s->ctx, CDFT_ERR_ARG, "cdft_system_set_tuning: no all-pairs z pass on this grid");
+    }
+    t.tuned = 1;                               // an explicit choice: cdft_converge keeps it
+    s->tune = t;
+    return CDFT_OK;
+}
+
+static bool autotune_enabled()
+{
+    const char *e = std::getenv("CDFT_AUTOTUNE");
+    return !(e && e[0] == '0');
+}
+
 int cdft_converge(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, cdft_result *r)
 {
+    if (s && spectral(s) && s->N >= (1L << 16) && autotune_enabled()) {
+        int rc = tune_spectral(s, false);
+        if (rc != CDFT_OK) return rc;
+    }
     return aasol_run(s, rho_inout, o, r, nullptr);
 }
 

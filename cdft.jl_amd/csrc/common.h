@@ -42,6 +42,7 @@ struct cdft_ctx {
     std::string err;
     double last_kernel_ms = 0.0;
     std::map<int, double *> twiddles;   // FFT ring length M -> W_M^t table (re, im pairs)
+    std::map<std::vector<long>, cdft_tuning> tune_cache;   // (grid, nc, precision) -> measured choice
     // batched launches: this device's static scratch (system descriptors and results
     // of the allocation-free entry point) and host staging sized at creation
     void *scratch = nullptr;
@@ -102,6 +103,7 @@ struct cdft_sys {
     hipGraphExec_t aa_exec = nullptr;
     const void *aa_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     const int *gate = nullptr;  // gate of the kernels being enqueued (null: always run)
+    cdft_tuning tune = {{8, 8}, {1, 1}, {0.0, 0.0}, 0};   // line-FFT variant per map precision
 };
 
 #define HIPCHECK_CTX(ctx, call)                                                              \

@@ -28,10 +28,12 @@ struct cx {
     T x, y;
 };
 
+// Complex product with its multiply-adds written out, so that every kernel that
+// inlines it rounds alike (the compiler may otherwise fuse either product).
 template <typename T>
 __device__ __attribute__((always_inline)) inline cx<T> cxmul(cx<T> a, cx<T> b)
 {
-    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+    return {fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x)};
 }
 
 
@@ -77,6 +79,16 @@ __device__ __attribute__((always_inline)) inline cx<double> ddft_map(const Grid3
     if (2 * ii[a] == G.n[a]) return {0.0, 0.0};
     const int ks = (ii[a] < (G.n[a] + 1) / 2) ? ii[a] : ii[a] - G.n[a];
     return {0.0, twopi * (double)ks * G.mult[a]};
+}
+
+// S + conj(Z) A (S ignored when first), with the multiply-adds written out so that
+// every pass that accumulates (k_fft3's OUT_ACCUM, k_fft3z_accum) rounds alike.
+template <typename T>
+__device__ __attribute__((always_inline)) inline cx<T> accum_conj(cx<T> s, cx<T> v, cx<T> z, bool first)
+{
+    const T tx = fma(v.y, z.y, v.x * z.x);
+    const T ty = fma(-v.x, z.y, v.y * z.x);
+    return first ? cx<T>{tx, ty} : cx<T>{s.x + tx, s.y + ty};
 }
 
 // Z = H1 + i H2 at one frequency from the tabulated G (H = G or i G).
@@ -140,8 +152,10 @@ __device__ inline cx<T> omega3(const Grid3 &G, int c, int f, int i0, int i1, int
     return {T(0), T(W)};
 }
 
-// Lines per LDS tile of the line-FFT passes: 32 KB per workgroup in FP64 at 256
-// points, so several workgroups per CU keep loads in flight.
+// Lines per LDS tile of the line-FFT passes (the default; the autotuner picks 4, 8
+// or 16 per system): 32 KB per workgroup in FP64 at 256 points, so several
+// workgroups per CU keep loads in flight.  The arithmetic of a line does not depend
+// on it, so neither do the results.
 constexpr int TB3 = 8;
 
 // Radix-4 Stockham passes (+ a final radix-2 when log2 L is odd) along j over a tile
@@ -249,11 +263,10 @@ __device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const c
 // are contiguous (LDS [b][j], j fastest); along y and z a tile is TB consecutive x
 // values (LDS [j][b], b fastest), so global accesses are coalesced and LDS accesses
 // of consecutive lanes are consecutive.  EPT = radix-4 butterflies per thread.
-template <typename T, int AX, int IN, int OUT, int EPT>
+template <typename T, int AX, int IN, int OUT, int EPT, int TB = TB3>
 __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
 {
     if (p.gate && *p.gate) return;
-    constexpr int TB = TB3;
     constexpr int EPR = EPT > 0 ? EPT : 1;     // radix-4 butterflies per thread and pass
     extern __shared__ __align__(16) unsigned char smem3[];
     cx<T> *a = reinterpret_cast<cx<T> *>(smem3);
@@ -319,7 +332,7 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
         }
     }
     __syncthreads();
-    tile_fft<T, AX, EPT>(a, p.tw, L, lg, p.inverse != 0);
+    tile_fft<T, AX, EPT, TB>(a, p.tw, L, lg, p.inverse != 0);
     // ---- store
 #pragma unroll
     for (int q = 0; q < EL; q++) {
@@ -343,15 +356,8 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
             const double ge = p.scale * double(v.x), gm = p.scale * double(v.y);
             reinterpret_cast<double *>(p.out)[g] += ge * ge + gm * ge;
         } else {  // OUT_ACCUM: S += conj(Z) A
-            const cx<T> z = pair_z<T>(p, g);
-            const cx<T> t = {v.x * z.x + v.y * z.y, v.y * z.x - v.x * z.y};
             cx<T> *S = reinterpret_cast<cx<T> *>(p.out);
-            if (p.first) {
-                S[g] = t;
-            } else {
-                const cx<T> sv = S[g];
-                S[g] = {sv.x + t.x, sv.y + t.y};
-            }
+            S[g] = accum_conj<T>(p.first ? cx<T>{T(0), T(0)} : S[g], v, pair_z<T>(p, g), p.first != 0);
         }
     }
 }
@@ -420,15 +426,7 @@ __global__ __launch_bounds__(256) void k_fft3z_accum(Grid3 G, ZPairs3 p)
             int l;
             long g;
             at(tid + q * 256, l, g);
-            const cx<T> v = a1[l];
-            const cx<T> z = zpair_z<T>(p, k, g);
-            const cx<T> t = {v.x * z.x + v.y * z.y, v.y * z.x - v.x * z.y};
-            if (k == 0) {
-                a0[l] = t;
-            } else {
-                const cx<T> sv = a0[l];
-                a0[l] = {sv.x + t.x, sv.y + t.y};
-            }
+            a0[l] = accum_conj<T>(k == 0 ? cx<T>{T(0), T(0)} : a0[l], a1[l], zpair_z<T>(p, k, g), k == 0);
         }
         __syncthreads();                                 // a1 is rewritten by the next pair; a0 complete
     }

@@ -176,6 +176,31 @@ int cdft_converge_batch_device_noalloc(cdft_ctx *ctx, int nsys, cdft_sys *const 
  * kernel of cdft_converge / cdft_adjoint_solve (their result's device_ms too). */
 double cdft_last_kernel_ms(const cdft_ctx *ctx);
 
+/* Tuning of the spectral path's line-FFT passes (2D and 3D grids), per map
+ * precision: lines per workgroup tile and, for z lines of 128 or 256 points, the
+ * all-pairs z pass of the transposed convolutions.  Every choice gives the same
+ * results bit for bit; only the time differs.  cdft_converge tunes a system of at
+ * least 2^16 grid points before its first solve (unless CDFT_AUTOTUNE=0), measuring
+ * each candidate on the device; a shape measured once is reused by every system of
+ * that shape in the context.  (No reference counterpart: FFTW plans are not
+ * tuned there, src/structure/transforms.jl:19-22.) */
+typedef struct {
+    int lines_per_tile[2];   /* [0] FP64 map, [1] FP32 map: 4, 8 or 16               */
+    int fused_z[2];          /* 1: the all-pairs z pass                               */
+    double ms[2];            /* one weighted-density + transposed-convolution
+                                sequence with the choice, ms (0: not measured)        */
+    int tuned;               /* 1 once measured for this shape or set explicitly      */
+} cdft_tuning;
+
+/* Measure the candidates (if not yet measured for the shape, or force != 0) and
+ * apply the fastest; out (optional) receives the choice.  A 1D system has nothing
+ * to tune (returns its defaults). */
+int cdft_system_tune(cdft_sys *sys, int force, cdft_tuning *out);
+int cdft_system_get_tuning(const cdft_sys *sys, cdft_tuning *out);
+/* Apply a choice without measuring (CDFT_ERR_ARG for an invalid one); the solves
+ * keep it. */
+int cdft_system_set_tuning(cdft_sys *sys, const cdft_tuning *in);
+
 /* Post-processing of a converged profile.
  * surface_tension: out = F k_B T - sum_i mu_i int rho_i + p int 1 with F = F_ideal +
  * F_res (Simpson, dz = (ub - lb) / N); mu[nc] in J/mol, p in Pa.  Replaces

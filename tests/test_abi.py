@@ -29,7 +29,8 @@ def test_library_exists_and_exports_header_symbols():
 def test_struct_layout_matches_header(tmp_path):
     """ctypes mirrors agree with the C compiler's layout of the header structs."""
     src = tmp_path / "layout.c"
-    fields = {"cdft_system_desc": _lib.SystemDesc, "cdft_solver_opts": _lib.SolverOpts, "cdft_result": _lib.Result}
+    fields = {"cdft_system_desc": _lib.SystemDesc, "cdft_solver_opts": _lib.SolverOpts, "cdft_result": _lib.Result,
+              "cdft_tuning": _lib.Tuning}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cdft_hip.h"', "int main(void){"]
     for cname, py in fields.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
