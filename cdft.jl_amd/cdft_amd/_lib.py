@@ -14,6 +14,7 @@ LIBPATH = os.path.join(_PKG, "libcdft_hip.so")
 CDFT_OK = 0
 CDFT_FP64 = 0
 CDFT_MIXED = 1
+CDFT_CARTESIAN, CDFT_SPHERICAL, CDFT_CYLINDRICAL = 0, 1, 2
 MAXC = 8
 NOALLOC_MAX_SYS = 1024     # CDFT_NOALLOC_MAX_SYS
 
@@ -41,6 +42,7 @@ class SystemDesc(C.Structure):
         ("assoc_nsite", C.c_int), ("assoc_npair", C.c_int),
         ("assoc_site_comp", C.POINTER(C.c_int)), ("assoc_site_count", _dp),
         ("assoc_pair_site", C.POINTER(C.c_int)), ("assoc_pair_eps", _dp), ("assoc_pair_kappa", _dp),
+        ("coord", C.c_int),
     ]
 
 
@@ -171,7 +173,7 @@ class DeviceSystem:
 
     def __init__(self, ctx, *, nc, ndim, ngrid, lb, ub, T, p, L, psi, m, d, sigma, eps,
                  rho_bulk, mu_res, vext, precision, assoc_site_comp=(), assoc_site_count=(),
-                 assoc_pairs=(), assoc_eps=(), assoc_kappa=()):
+                 assoc_pairs=(), assoc_eps=(), assoc_kappa=(), coord=0):
         self.ctx = ctx
         keep = {k: np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
                 for k, v in dict(m=m, d=d, sigma=sigma, eps=eps, rho_bulk=rho_bulk,
@@ -189,6 +191,7 @@ class DeviceSystem:
             keep["vext"] = np.ascontiguousarray(vext, dtype=np.float64).reshape(-1)
             desc.vext = ptr(keep["vext"])
         desc.precision = int(precision)
+        desc.coord = int(coord)
         if len(assoc_pairs):
             keep["site_comp"] = np.ascontiguousarray(assoc_site_comp, dtype=np.int32)
             keep["site_count"] = np.ascontiguousarray(assoc_site_count, dtype=np.float64)

@@ -21,12 +21,14 @@
                   (infinite cylinders), on a 2D grid
   c1c10_lamellar  the methane / n-decane interface as a TwoPhase2DLamCart: the 1D grid
                   along x, uniform along y
+  methane_radial  methane at 150 K, 1e7 Pa inside a spherical (Uniform1DSphr) or
+                  cylindrical (Uniform1DCyl) graphite Steele pore of radius 5 sigma
 """
 import numpy as np
 
 from .bulk import PCSAFT
-from .structure import (HardSphereMatrix, Steele, TwoPhase1DCart, TwoPhase2DLamCart, Uniform1DCart, Uniform2DCart,
-                        Uniform3DCart)
+from .structure import (HardSphereMatrix, Steele, TwoPhase1DCart, TwoPhase2DLamCart, Uniform1DCart, Uniform1DCyl,
+                        Uniform1DSphr, Uniform2DCart, Uniform3DCart)
 from .system import DFTOptions, DFTSystem, initialize_profiles
 
 
@@ -51,6 +53,17 @@ def matrix_2d(options=None, n=256, box=30.0, ndisks=16, seed=0, T=200.0, p=1e7, 
     structure = Uniform2DCart((p, T), x / v, [[0.0, side]] * 2, n)
     field = HardSphereMatrix.random(ndisks, rmin * L, rmax * L, side, seed=seed, ndim=2) if ndisks else None
     system = DFTSystem(model, structure, field, options)
+    return system, initialize_profiles(system)
+
+
+def methane_radial(coord="spherical", options=None, ngrid=151, radius=5.0, wall=True):
+    model = PCSAFT(["methane"])
+    T, p = 150.0, 1e7
+    v = model.volume(p, T, [1.0], phase="liquid")
+    L = model.length_scale()
+    mk = Uniform1DSphr if coord == "spherical" else Uniform1DCyl
+    structure = mk((p, T), [1 / v], [0.0, radius * L], ngrid)
+    system = DFTSystem(model, structure, Steele(["graphite"]) if wall else None, options)
     return system, initialize_profiles(system)
 
 

@@ -6,6 +6,8 @@ Mirrors the reference's constructors:
   Uniform3DCart      src/base/structure.jl:335-352
   TwoPhase1DCart     src/base/structure.jl:491 (``TwoPhase1DCart``)
   TwoPhase2DLamCart  src/base/structure.jl:494-513 (lamellar interface along the first axis)
+  Uniform1DSphr      src/base/structure.jl:388-410 (radial grid of a spherical transform)
+  Uniform1DCyl       src/base/structure.jl:411-433 (radial grid of a cylindrical transform)
   Steele          src/structure/external_fields/steele.jl:31-46 (database/Steele/steele_like.csv)
 and adds a field the reference does not have:
   HardSphereMatrix  a periodic matrix of impenetrable spheres on a 3D grid (disks,
@@ -17,7 +19,7 @@ import numpy as np
 class Structure:
     """A 1D, 2D or 3D Cartesian grid with (p, T) conditions and bulk densities."""
 
-    def __init__(self, conditions, rho_bulk, bounds, ngrid, topology="uniform", rho_bulk2=None, ndim=1):
+    def __init__(self, conditions, rho_bulk, bounds, ngrid, topology="uniform", rho_bulk2=None, ndim=1, coord=0):
         p, T = conditions
         self.conditions = (float(p), float(T))
         self.rho_bulk = np.asarray(rho_bulk, dtype=np.float64).reshape(-1)
@@ -36,6 +38,9 @@ class Structure:
         self.ngrid = tuple(int(n) for n in g)
         self.topology = topology
         self.rho_bulk2 = None if rho_bulk2 is None else np.asarray(rho_bulk2, dtype=np.float64).reshape(-1)
+        self.coord = int(coord)              # 0 Cartesian, 1 spherical, 2 cylindrical
+        if self.coord and (ndim != 1 or self.bounds[0][0] < 0):
+            raise ValueError("radial structures are 1D with bounds 0 <= lb < ub")
 
     @property
     def ndim(self):
@@ -46,7 +51,11 @@ class Structure:
         return int(np.prod(self.ngrid))
 
     def axis_coords(self, a):
-        """LinRange(lb, ub, N) of axis a (get_coords, src/structure/structure.jl:138-151)."""
+        """LinRange(lb, ub, N) of axis a (get_coords, src/structure/structure.jl:138-151);
+        the transform's sample radii on a radial grid (structure.jl:203-214)."""
+        if self.coord:
+            from .radial import grid
+            return grid(self.coord, self.bounds[0][1], self.ngrid[0])[0]
         lb, ub = self.bounds[a]
         N = self.ngrid[a]
         t = np.arange(N, dtype=np.float64) / (N - 1)
@@ -66,7 +75,11 @@ class Structure:
         return tuple((ub - lb) / n for (lb, ub), n in zip(self.bounds, self.ngrid))
 
     def volume(self):
-        """Product of the bound lengths (_domain_volume, src/methods/adsorption.jl:15)."""
+        """_domain_volume (src/methods/adsorption.jl:22-25): the product of the bound
+        lengths, or the sphere's / disk's measure between lb and ub."""
+        if self.coord:
+            from .radial import domain_volume
+            return float(domain_volume(self.coord, *self.bounds[0]))
         return float(np.prod([ub - lb for lb, ub in self.bounds]))
 
 
@@ -91,6 +104,19 @@ def Uniform3DCart(conditions, rho_bulk, bounds, ngrid):
     return Structure(conditions, rho_bulk, bounds, ngrid, "uniform", ndim=3)
 
 
+def Uniform1DSphr(conditions, rho_bulk, bounds, ngrid):
+    """Uniform1DSphr(conditions, rho_bulk, [lb, ub] or ub, ngrid): a radial grid over
+    [0, ub] (the transform's aperture); lb places walls only (base/structure.jl:388-410)."""
+    b = [0.0, float(bounds)] if np.isscalar(bounds) else bounds
+    return Structure(conditions, rho_bulk, b, ngrid, "uniform", coord=1)
+
+
+def Uniform1DCyl(conditions, rho_bulk, bounds, ngrid):
+    """Uniform1DCyl(conditions, rho_bulk, [lb, ub] or ub, ngrid) (base/structure.jl:411-433)."""
+    b = [0.0, float(bounds)] if np.isscalar(bounds) else bounds
+    return Structure(conditions, rho_bulk, b, ngrid, "uniform", coord=2)
+
+
 def TwoPhase1DCart(conditions, rho_bulk, rho_bulk2, bounds, ngrid):
     return Structure(conditions, rho_bulk, bounds, ngrid, "two_phase", rho_bulk2)
 
@@ -113,7 +139,12 @@ class Steele:
         self.width = float(width)
 
     def vext(self, structure, model):
-        """beta V_ext(z) per component, (N, nc) column-major flattened (steele.jl:49-81)."""
+        """beta V_ext(z) per component, (N, nc) column-major flattened (steele.jl:49-81);
+        on a radial grid the single-wall form at the wall distance r - lb (lb > 0: fluid
+        outside a sphere or cylinder) or ub - r (a pore), clamped below at half the
+        smallest sigma_i (steele.jl:111-142)."""
+        if structure.coord:
+            return self._vext_radial(structure, model)
         z = structure.coords()
         T = structure.conditions[1]
         eps_i = np.diag(model.epsilon)
@@ -133,6 +164,23 @@ class Steele:
                     zz = H - z
                     V[i] += pre * (2.0 / 5.0 * (ssi[i] / zz) ** 10 - (ssi[i] / zz) ** 4
                                    - ssi[i] ** 4 / (3.0 * D * (zz + 0.61 * D) ** 3))
+        return (V * (1.0 / T)).reshape(-1)
+
+    def _vext_radial(self, structure, model):
+        r = structure.axis_coords(0)
+        lb, ub = structure.bounds[0]
+        T = structure.conditions[1]
+        eps_i = np.diag(model.epsilon)
+        sig_i = np.diag(model.sigma)
+        zw = np.maximum(r - lb if lb > 0 else ub - r, 0.5 * np.min(sig_i))
+        V = np.zeros((len(eps_i), r.size))
+        for s in range(len(self.surface)):
+            esi = np.sqrt(self.epsilon[s] * eps_i)
+            ssi = (self.sigma[s] + sig_i) / 2.0
+            D = self.delta[s]
+            for i in range(len(eps_i)):
+                V[i] += 2.0 * np.pi * self.rho[s] * esi[i] * D * ssi[i] ** 2 * (
+                    2.0 / 5.0 * (ssi[i] / zw) ** 10 - (ssi[i] / zw) ** 4 - ssi[i] ** 4 / (3.0 * D * (zw + 0.61 * D) ** 3))
         return (V * (1.0 / T)).reshape(-1)
 
 

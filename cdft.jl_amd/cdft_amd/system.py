@@ -92,7 +92,7 @@ class DFTSystem:
                 ub=[b[1] for b in st.bounds], T=T, p=p, L=self.L,
                 psi=PSI, m=self.model.segment, d=self.species.size, sigma=self.model.sigma,
                 eps=self.model.epsilon, rho_bulk=self.species.bulk_density, mu_res=self.species.chempot_res,
-                vext=self.vext, precision=self.options.precision_code, **self.assoc_descriptor())
+                vext=self.vext, precision=self.options.precision_code, coord=st.coord, **self.assoc_descriptor())
         return self._dev
 
     def descriptor(self):
@@ -104,7 +104,7 @@ class DFTSystem:
                     m=self.model.segment.copy(), d=self.species.size.copy(), sigma=self.model.sigma.copy(),
                     eps=self.model.epsilon.copy(), mu_res=self.species.chempot_res.copy(),
                     rho_bulk=self.species.bulk_density.copy(), vext=None if self.vext is None else self.vext.copy(),
-                    **self.assoc_descriptor())
+                    coord=st.coord, **self.assoc_descriptor())
 
     def assoc_descriptor(self):
         """Association sites and bonding pairs (pack_assoc_params, assoc.jl:18-57);
@@ -214,6 +214,9 @@ def simpson(f, dz):
 
 def integration_weights(structure):
     """Weights w with _int(f) = w . f over the whole grid (column-major, x fastest)."""
+    if structure.coord:
+        from .radial import weights
+        return weights(structure.coord, structure.bounds[0][1], structure.ngrid[0])
     w = None
     for n, dz in zip(structure.ngrid, structure.dzs()):
         k = np.arange(1, n + 1)
@@ -226,7 +229,11 @@ def integration_weights(structure):
 
 
 def integrate(structure, f):
-    """_int over every axis of the structure (src/utils/integrals.jl:28-40), f column-major."""
+    """_int over every axis of the structure (src/utils/integrals.jl:28-40), f column-major;
+    on a radial grid 4 pi or 2 pi times the transform's quadrature (integrals.jl:20-22)."""
+    if structure.coord:
+        from .radial import weights
+        return float(weights(structure.coord, structure.bounds[0][1], structure.ngrid[0]) @ np.asarray(f).reshape(-1))
     if structure.ndim == 1:
         return simpson(f, structure.dz())
     f = np.asarray(f, dtype=np.float64).reshape(structure.ngrid[::-1])      # [z, y, x]

@@ -58,7 +58,7 @@ double simpson1d(const double *f, int n, double dz)
 
 void free_sys_buffers(cdft_sys *s)
 {
-    double **bufs[] = {&s->h, &s->vext, &s->cpd, &s->rho, &s->n, &s->dphi, &s->part, &s->fval, &s->out, &s->sol,
+    double **bufs[] = {&s->h, &s->hT, &s->vext, &s->cpd, &s->rho, &s->n, &s->dphi, &s->part, &s->fval, &s->out, &s->sol,
                        &s->gx, &s->df, &s->dg, &s->res, &s->Q, &s->Qalt, &s->DG, &s->red, &s->fin, &s->hddft,
                        &s->xsafe};
     for (double **b : bufs)
@@ -98,6 +98,20 @@ double simpson_nd(const double *f, int nd, const int *ngrid, const double *dz)
 
 // 2D and 3D grids take the spectral path (a 2D grid is a 3D grid with one z plane).
 inline bool spectral(const cdft_sys *s) { return s->nd >= 2; }
+
+// int f dV over the structure (integrals.jl:20-40): Simpson on Cartesian grids, the
+// radial transform's weights (4 pi / 2 pi integrateR) on radial ones.
+double integ_sys(const cdft_sys *s, const double *f)
+{
+    if (s->coord) {
+        double acc = 0.0;
+        for (int i = 0; i < s->N; i++) acc += s->rw[i] * f[i];
+        return acc;
+    }
+    double dz[3];
+    for (int a = 0; a < s->nd; a++) dz[a] = (s->ub[a] - s->lb[a]) / s->ngrid[a];   // structure_dz
+    return simpson_nd(f, s->nd, s->ngrid, dz);
+}
 
 // Arguments of k_update: per-field parts (1D) or the finished transposed sum (3D).
 struct UpdSrc {
@@ -425,6 +439,12 @@ void launch_transposed(cdft_sys *s)
         return;
     }
     int N = s->N, F = s->F;
+    if (s->coord) {        // the scalar slots' operators themselves, the vector slot's transpose
+        hipLaunchKernelGGL(k_dense1d, dim3((N + 63) / 64, s->nc * (F - 1)), dim3(256), 0, s->ctx->stream,
+                           (const double *)s->dphi, (long)F * N, (long)N, (const double *)s->h, (const double *)s->hT,
+                           s->part, (long)(F - 1) * N, (long)N, 1, N, F, 1.0, s->gate);
+        return;
+    }
     hipLaunchKernelGGL(k_circ1d<double>, dim3((N + CIRC_TW - 1) / CIRC_TW, s->nc * (F - 1)), dim3(256), 0, s->ctx->stream, s->dphi,
                        (long)F * N, (long)N, s->h, s->part, (long)(F - 1) * N, (long)N, 1, N, F, 1.0, 1, s->gate);
 }
@@ -437,6 +457,12 @@ void launch_weighted(cdft_sys *s)
         return;
     }
     int N = s->N, F = s->F;
+    if (s->coord) {
+        hipLaunchKernelGGL(k_dense1d, dim3((N + 63) / 64, s->nc * (F - 1)), dim3(256), 0, s->ctx->stream,
+                           (const double *)s->rho, (long)N, 0L, (const double *)s->h, (const double *)nullptr, s->n,
+                           (long)F * N, (long)N, 0, N, F, s->NA, s->gate);
+        return;
+    }
     hipLaunchKernelGGL(k_circ1d<double>, dim3((N + CIRC_TW - 1) / CIRC_TW, s->nc * (F - 1)), dim3(256), 0, s->ctx->stream, s->rho,
                        (long)N, 0L, s->h, s->n, (long)F * N, (long)N, 0, N, F, s->NA, 0, s->gate);
 }
@@ -478,6 +504,10 @@ void launch_hvp(cdft_sys *s, const double *v)
     int N = s->N, F = s->F, nc = s->nc;
     if (spectral(s)) {
         launch_weighted3d<double>(s, v, s->dphi);
+    } else if (s->coord) {
+        hipLaunchKernelGGL(k_dense1d, dim3((N + 63) / 64, nc * (F - 1)), dim3(256), 0, st, v, (long)N, 0L,
+                           (const double *)s->h, (const double *)nullptr, s->dphi, (long)F * N, (long)N, 0, N, F,
+                           s->NA, s->gate);
     } else {
         dim3 g((N + CIRC_TW - 1) / CIRC_TW, nc * (F - 1));
         hipLaunchKernelGGL(k_circ1d<double>, g, dim3(256), 0, st, v, (long)N, 0L, s->h, s->dphi, (long)F * N,
@@ -490,6 +520,146 @@ void launch_hvp(cdft_sys *s, const double *v)
     default: launch_phi_hvp<4>(s, v, st); break;
     }
     launch_transposed(s);
+}
+
+// ---- radial grids (Uniform1DSphr / Uniform1DCyl, base/structure.jl:363-433)
+// The quasi-discrete Hankel transform of Hankel.jl (Guizar-Sicairos and
+// Gutierrez-Vega 2004), order 0: cylindrical = the 2D radial Fourier transform on
+// the zeros j_n of J_0, spherical = the 3D one as the order-1/2 transform of
+// sqrt(r) f on j_n = n pi.  r_n = j_n R / j_(N+1), k_n = j_n / R, and
+// C_mn = 2 / j_(N+1) J_p(j_m j_n / j_(N+1)) / (|J_(p+1)(j_m)| |J_(p+1)(j_n)|).
+static double radial_jp(bool sph, double x) { return sph ? std::sqrt(2.0 / (kPi * x)) * std::sin(x) : ::j0(x); }
+static double radial_jp1(bool sph, double x)
+{
+    return sph ? std::sqrt(2.0 / (kPi * x)) * (std::sin(x) / x - std::cos(x)) : ::j1(x);
+}
+static double radial_zero(bool sph, int s)
+{
+    if (sph) return s * kPi;
+    const double b = (s - 0.25) * kPi;                  // McMahon's expansion, then Newton
+    double x = b + 1.0 / (8.0 * b) - 124.0 / (3.0 * std::pow(8.0 * b, 3));
+    for (int it = 0; it < 50; it++) {
+        const double dx = ::j0(x) / ::j1(x);
+        x += dx;
+        if (std::fabs(dx) < 1e-15 * x) break;
+    }
+    return x;
+}
+
+// The dense operators of a radial system: for every component and field slot the
+// convolution Q^-1 diag(Omega_f) Q (the reference's convolve! with a QDHT,
+// integrals.jl:127-137; scalar kernels at omega = k / 2 pi in reduced units without
+// a k = 0 sample, weighted_densities.jl:92-116), column-major into s->h; the vector
+// slot's T = D Q^-1 diag(H) Q with H in unscaled units and D the radial derivative
+// matrix (weighted_densities.jl:290-316, transforms.jl:43-63) into slot 3 and its
+// transpose into s->hT (the reference adds transpose(T) dPhi/dn_v, :330-343); the
+// integration weights (integrals.jl:20-22) into s->rw.
+static hipError_t radial_setup(cdft_sys *s)
+{
+    const int N = s->N, nc = s->nc, F = s->F;
+    const bool sph = (s->coord == CDFT_SPHERICAL);
+    const double R = s->ub[0];
+    std::vector<double> z(N + 1), r(N), k(N), jp1(N);
+    for (int q = 0; q <= N; q++) z[q] = radial_zero(sph, q + 1);
+    const double S = z[N], K = S / R;
+    for (int q = 0; q < N; q++) {
+        r[q] = z[q] * R / S;
+        k[q] = z[q] / R;
+        jp1[q] = std::fabs(radial_jp1(sph, z[q]));
+    }
+    // forward A[m N + j] (row-major) and backward B[m N + i] = Bop[i][m] (column-major)
+    std::vector<double> A((size_t)N * N), B((size_t)N * N);
+    for (int m = 0; m < N; m++) {
+        const double km = sph ? std::sqrt(kPi / (2.0 * k[m])) : 1.0;
+        for (int j = 0; j < N; j++) {
+            const double C = 2.0 / S * radial_jp(sph, z[m] * z[j] / S) / (jp1[m] * jp1[j]);
+            A[(size_t)m * N + j] = C * (sph ? std::sqrt(r[j]) : 1.0) * R / jp1[j] * jp1[m] / K * km;
+            B[(size_t)m * N + j] = C / km * K / jp1[m] * jp1[j] / R / (sph ? std::sqrt(r[j]) : 1.0);
+        }
+    }
+    // multipliers: w[(c F + f) N + m]; slot 0 unused, slot 3 the unscaled H of the vector weight
+    const double twopi = 2.0 * kPi, fourpi = 4.0 * kPi, tp3 = twopi * twopi * twopi;
+    std::vector<double> w((size_t)nc * F * N, 0.0);
+    for (int c = 0; c < nc; c++) {
+        const double dred = s->pc.dd[c];
+        for (int f = 1; f < F; f++) {
+            const bool sl = (f == 1 || f == 5);                     // int rho dz (d/2), (d)
+            const double width = f == 6 ? dred * s->psi : (f <= 3 ? 0.5 : 1.0) * dred;
+            const double Rw = twopi * width;
+            for (int m = 0; m < N; m++) {
+                double v;
+                if (f == 3) {
+                    const double wb = k[m] / twopi, Rk = twopi * 0.5 * dred * s->L;
+                    v = fourpi / (wb * wb * wb) * (std::sin(wb * Rk) - Rk * wb * std::cos(wb * Rk)) / tp3;
+                } else {
+                    const double wb = k[m] / twopi * s->L, wbR = wb * Rw;
+                    v = sl ? 2.0 * std::sin(wbR) / wb / twopi
+                           : fourpi / (wb * wb * wb) * (std::sin(wbR) - Rw * wb * std::cos(wbR)) / tp3;
+                }
+                w[((size_t)c * F + f) * N + m] = v;
+            }
+        }
+    }
+    double *dA = nullptr, *dB = nullptr, *dw = nullptr;
+    hipError_t e;
+    struct Free3 { double **p[3]; ~Free3() { for (auto q : p) if (*q) (void)hipFree(*q); } } fr{{&dA, &dB, &dw}};
+    if ((e = hipMalloc((void **)&dA, sizeof(double) * A.size())) != hipSuccess) return e;
+    if ((e = hipMalloc((void **)&dB, sizeof(double) * B.size())) != hipSuccess) return e;
+    if ((e = hipMalloc((void **)&dw, sizeof(double) * w.size())) != hipSuccess) return e;
+    if ((e = hipMemcpy(dA, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(dB, B.data(), sizeof(double) * B.size(), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(dw, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_radial_op, dim3((N + 255) / 256, N, nc * F), dim3(256), 0, s->ctx->stream,
+                       (const double *)dA, (const double *)dB, (const double *)dw, N, s->h);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s->ctx->stream)) != hipSuccess) return e;
+    // T = D M, M the slot-3 operator; D has three entries per row
+    std::vector<double> M((size_t)N * N), T((size_t)N * N), TT((size_t)N * N);
+    std::vector<double> D((size_t)N * 3), Dc((size_t)N);       // row i: columns Dc[i] .. Dc[i] + 2
+    for (int i = 0; i < N; i++) {
+        const int i0 = i == 0 ? 0 : (i == N - 1 ? N - 3 : i - 1);
+        Dc[i] = i0;
+        double *d = &D[(size_t)i * 3];
+        if (i == 0) {
+            const double h1 = r[1] - r[0], h2 = r[2] - r[1];
+            d[0] = -(2 * h1 + h2) / (h1 * (h1 + h2));
+            d[1] = (h1 + h2) / (h1 * h2);
+            d[2] = -h1 / (h2 * (h1 + h2));
+        } else if (i == N - 1) {
+            const double h1 = r[N - 2] - r[N - 3], h2 = r[N - 1] - r[N - 2];
+            d[0] = h2 / (h1 * (h1 + h2));
+            d[1] = -(h1 + h2) / (h1 * h2);
+            d[2] = (2 * h2 + h1) / (h2 * (h1 + h2));
+        } else {
+            const double h1 = r[i] - r[i - 1], h2 = r[i + 1] - r[i];
+            d[0] = -h2 / (h1 * (h1 + h2));
+            d[1] = (h2 - h1) / (h1 * h2);
+            d[2] = h1 / (h2 * (h1 + h2));
+        }
+    }
+    for (int c = 0; c < nc; c++) {
+        double *slot = s->h + ((size_t)c * F + 3) * N * N;
+        if ((e = hipMemcpy(M.data(), slot, sizeof(double) * M.size(), hipMemcpyDeviceToHost)) != hipSuccess) return e;
+        for (int i = 0; i < N; i++)                     // M column-major: M[j N + l] = M_lj
+            for (int j = 0; j < N; j++) {
+                const double *d = &D[(size_t)i * 3];
+                const int l0 = (int)Dc[i];
+                double acc = 0.0;
+                for (int q = 0; q < 3; q++) acc += d[q] * M[(size_t)j * N + l0 + q];
+                T[(size_t)j * N + i] = acc;              // column-major T
+                TT[(size_t)i * N + j] = acc;             // column-major T^T
+            }
+        if ((e = hipMemcpy(slot, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice)) != hipSuccess) return e;
+        if ((e = hipMemcpy(s->hT + (size_t)c * N * N, TT.data(), sizeof(double) * TT.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+            return e;
+    }
+    s->rw.resize(N);
+    for (int q = 0; q < N; q++) {
+        const double wq = 2.0 / (K * K * jp1[q] * jp1[q]);       // int_0^R g r dr quadrature
+        s->rw[q] = sph ? 4.0 * kPi * wq * r[q] : 2.0 * kPi * wq;
+    }
+    return hipSuccess;
 }
 
 }  // namespace
@@ -560,6 +730,16 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
     if (d->nc < 1 || d->nc > 4) return fail(ctx, CDFT_ERR_UNSUPPORTED, "nc must be in 1..4");
     if (d->ndim < 1 || d->ndim > 3) return fail(ctx, CDFT_ERR_UNSUPPORTED, "only 1D, 2D and 3D Cartesian grids are supported");
     if (d->ndim == 1 && d->ngrid[0] < 3) return fail(ctx, CDFT_ERR_ARG, "ngrid must be >= 3");
+    if (d->coord != CDFT_CARTESIAN) {
+        if (d->coord != CDFT_SPHERICAL && d->coord != CDFT_CYLINDRICAL)
+            return fail(ctx, CDFT_ERR_ARG, "unknown coordinate system");
+        if (d->ndim != 1) return fail(ctx, CDFT_ERR_ARG, "radial grids are one-dimensional");
+        if (d->ngrid[0] > CDFT_RADIAL_MAXN)
+            return fail(ctx, CDFT_ERR_UNSUPPORTED, "radial grids of at most CDFT_RADIAL_MAXN points");
+        if (!(d->lb[0] >= 0.0)) return fail(ctx, CDFT_ERR_ARG, "radial bounds need 0 <= lb < ub");
+        if (d->precision != CDFT_FP64)
+            return fail(ctx, CDFT_ERR_UNSUPPORTED, "radial grids run in FP64 only");
+    }
     if (d->ndim >= 2) {
         long N3 = 1;
         for (int a = 0; a < d->ndim; a++) {
@@ -621,6 +801,7 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
     s->psi = d->psi > 0 ? d->psi : 1.3862;
     s->NA = 6.02214076e23 * (s->L * s->L * s->L);   // N_A * density_scale^3 (weighted_densities.jl:123)
     s->precision = d->precision;
+    s->coord = d->coord;
     int nc = s->nc;
     for (int i = 0; i < nc; i++) {
         s->m[i] = d->m[i];
@@ -674,7 +855,9 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
     hipError_t e = hipSuccess;
     double *ddev = nullptr;
     const bool d3 = spectral(s);
-    if ((!d3 && (e = alloc(&s->h, (size_t)nc * F * N)) != hipSuccess) || (e = alloc(&s->rho, n)) != hipSuccess ||
+    const size_t hsize = s->coord ? (size_t)nc * F * N * N : (size_t)nc * F * N;
+    if ((!d3 && (e = alloc(&s->h, hsize)) != hipSuccess) || (s->coord && (e = alloc(&s->hT, (size_t)nc * N * N)) != hipSuccess) ||
+        (e = alloc(&s->rho, n)) != hipSuccess ||
         (e = alloc(&s->n, (size_t)nc * F * N)) != hipSuccess || (e = alloc(&s->dphi, (size_t)nc * F * N)) != hipSuccess ||
         (e = alloc(&s->part, d3 ? (size_t)n : (size_t)nc * (F - 1) * N)) != hipSuccess ||
         (e = alloc(&s->fval, N)) != hipSuccess ||
@@ -758,6 +941,8 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
                                    s->G3 + (size_t)(c * 8 + f - 1) * N,
                                    s->G3f ? s->G3f + (size_t)(c * 8 + f - 1) * N : (float *)nullptr);
         if (e == hipSuccess) e = hipGetLastError();
+    } else if (s->coord) {
+        e = radial_setup(s);
     } else {
         double mult = ((double)N / (s->ub[0] - s->lb[0])) / (double)N;   // fftfreq spacing
         e = hipMemcpy(ddev, dd, sizeof(double) * nc, hipMemcpyHostToDevice);
@@ -780,7 +965,7 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
     }
     // a system the batched kernel can take gets its ring spectra now, so that the
     // allocation-free batched entry point finds it ready
-    if (!d3 && s->nc == 1 && !s->assoc && s->N <= kRegNPT * BT && s->N >= 5) {
+    if (!d3 && !s->coord && s->nc == 1 && !s->assoc && s->N <= kRegNPT * BT && s->N >= 5) {
         int rc = batched_prepare(ctx, s);
         if (rc == CDFT_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = CDFT_ERR_HIP;
         if (rc != CDFT_OK) {
@@ -824,11 +1009,7 @@ int cdft_functional_eval(cdft_sys *s, const double *rho, double *dfdrho, double 
     }
     HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
     if (fval) std::memcpy(fval, fv.data(), sizeof(double) * s->N);
-    if (f_res) {
-        double dz[3];
-        for (int a = 0; a < s->nd; a++) dz[a] = (s->ub[a] - s->lb[a]) / s->ngrid[a];   // structure_dz
-        *f_res = simpson_nd(fv.data(), s->nd, s->ngrid, dz) / (s->L * s->L * s->L);    // F_res, models.jl:86-91
-    }
+    if (f_res) *f_res = integ_sys(s, fv.data()) / (s->L * s->L * s->L);    // F_res, models.jl:86-91
     return CDFT_OK;
 }
 
@@ -1297,6 +1478,7 @@ int cdft_ddft_rhs(cdft_sys *s, const double *eta, double *deta)
 {
     if (!s || !eta || !deta) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_ddft_rhs: NULL argument");
     cdft_ctx *ctx = s->ctx;
+    if (s->coord) return fail(ctx, CDFT_ERR_UNSUPPORTED, "cdft_ddft_rhs: Cartesian grids only (spectral derivatives)");
     hipStream_t st = ctx->stream;
     const long n = (long)s->N * s->nc;
     const int N = s->N, nc = s->nc;
@@ -1379,9 +1561,7 @@ int cdft_surface_tension(cdft_sys *s, const double *rho, const double *mu, doubl
     double fres = 0.0;
     int rc = cdft_functional_eval(s, rho, dfd.data(), &fres, nullptr);
     if (rc != CDFT_OK) return rc;
-    double dz[3];
-    for (int a = 0; a < s->nd; a++) dz[a] = (s->ub[a] - s->lb[a]) / s->ngrid[a];
-    auto integ = [&](const double *f) { return simpson_nd(f, s->nd, s->ngrid, dz); };
+    auto integ = [&](const double *f) { return integ_sys(s, f); };
     std::vector<double> phi(s->N, 0.0), one(s->N, 1.0);
     const double NAv = 6.02214076e23, kB = 1.380649e-23;
     for (int c = 0; c < s->nc; c++)
@@ -1399,12 +1579,13 @@ int cdft_surface_tension(cdft_sys *s, const double *rho, const double *mu, doubl
 int cdft_adsorption(cdft_sys *s, const double *rho, double *out)
 {
     if (!s || !rho || !out) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_adsorption: NULL argument");
-    double V = 1.0, dz[3];
-    for (int a = 0; a < s->nd; a++) {
-        V *= s->ub[a] - s->lb[a];                                   // _domain_volume, adsorption.jl:15
-        dz[a] = (s->ub[a] - s->lb[a]) / s->ngrid[a];
-    }
-    for (int c = 0; c < s->nc; c++) out[c] = simpson_nd(rho + (size_t)c * s->N, s->nd, s->ngrid, dz) / V;   // adsorption.jl:22-32
+    double V = 1.0;                                                  // _domain_volume, adsorption.jl:15-25
+    const double lo = s->lb[0], hi = s->ub[0];
+    if (s->coord == CDFT_SPHERICAL) V = 4.0 * kPi / 3.0 * (hi * hi * hi - lo * lo * lo);
+    else if (s->coord == CDFT_CYLINDRICAL) V = kPi * (hi * hi - lo * lo);
+    else
+        for (int a = 0; a < s->nd; a++) V *= s->ub[a] - s->lb[a];
+    for (int c = 0; c < s->nc; c++) out[c] = integ_sys(s, rho + (size_t)c * s->N) / V;   // adsorption.jl:22-32
     return CDFT_OK;
 }
 
@@ -1488,7 +1669,8 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         if (sys[i]->N > kRegNPT * BT || o->m > kRegAM)
             return fail(ctx, CDFT_ERR_UNSUPPORTED,
                         "batched path: at most 512 grid points and anderson m <= 5 for now; use cdft_converge");
-        if (sys[i]->nd != 1) return fail(ctx, CDFT_ERR_UNSUPPORTED, "the batched path takes 1D systems");
+        if (sys[i]->nd != 1 || sys[i]->coord)
+            return fail(ctx, CDFT_ERR_UNSUPPORTED, "the batched path takes 1D Cartesian systems");
         if (sys[i]->assoc)
             return fail(ctx, CDFT_ERR_UNSUPPORTED,
                         "batched path: associating systems are solved per system; use cdft_converge");

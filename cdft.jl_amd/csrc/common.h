@@ -62,12 +62,15 @@ struct cdft_sys {
     double sigma[CDFT_MAXC * CDFT_MAXC], eps[CDFT_MAXC * CDFT_MAXC];
     double cp[CDFT_MAXC];      // mu_res + ln rho_bulk (converge.jl:121)
     int precision = CDFT_FP64;
+    int coord = CDFT_CARTESIAN;  // radial grids: dense operators in h (and hT), weights rw
     bool has_vext = false;
     int assoc = 0;             // association variant of the pointwise kernels: 0 none, 1 one pair, 2 general
     PhiConst pc;
 
     // device buffers (FP64)
-    double *h = nullptr;       // real-space kernels [(c*F+f)*N + j], f >= 1
+    double *h = nullptr;       // real-space kernels [(c*F+f)*N + j], f >= 1 (radial: dense [(c*F+f)*N*N])
+    double *hT = nullptr;      // radial: transposed vector-slot operator [c*N*N]
+    std::vector<double> rw;    // radial: weights of int f dV (4 pi or 2 pi times the transform's quadrature)
     double *vext = nullptr;    // [c*N + i]
     double *cpd = nullptr;     // device copy of cp [nc]
     double *rho = nullptr;     // current profile

@@ -242,6 +242,59 @@ __global__ __launch_bounds__(256) void k_circ1d(const TS *__restrict__ src, long
     }
 }
 
+// Dense operators of the radial (spherical / cylindrical) grids:
+// dst_(c,f)[i] = scale sum_j Op_(c,f)[i][j] src_(c,f)[j], blockIdx.y = c*(F-1) + f-1,
+// Op stored column-major per (c, f) (op[(c F + f) N N + j N + i]); vop (optional) is
+// the vector slot's (f = 3) operator [c N N] instead.  A workgroup owns 64 outputs,
+// its four waves split the j range (src chunks in LDS, the operator read along i,
+// consecutive per lane) and their sums are added in a fixed order.
+__global__ __launch_bounds__(256) void k_dense1d(const double *__restrict__ src, long s_cstride, long s_fstride,
+                                                 const double *__restrict__ op, const double *__restrict__ vop,
+                                                 double *__restrict__ dst, long d_cstride, long d_fstride, int d_fofs,
+                                                 int N, int F, double scale, const int *gate)
+{
+    if (gate && *gate) return;
+    constexpr int TW = 64, NG = 4, KC = 256;
+    __shared__ double xs[NG][KC];
+    __shared__ double part[NG][TW];
+    const int y = blockIdx.y;
+    const int c = y / (F - 1), f = 1 + y % (F - 1);
+    const double *s = src + c * s_cstride + f * s_fstride;
+    const double *A = (vop && f == 3) ? vop + (size_t)c * N * N : op + (size_t)(c * F + f) * N * N;
+    double *d = dst + c * d_cstride + (f - d_fofs) * d_fstride;
+    const int g = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int i = blockIdx.x * TW + l;
+    const int Kg = (N + NG - 1) / NG;
+    const int kb = g * Kg, ke = min(N, kb + Kg);
+    double acc = 0.0;
+    for (int kc0 = 0; kc0 < Kg; kc0 += KC) {
+        const int kc = kb + kc0;
+        const int kcn = max(0, min(KC, ke - kc));
+        __syncthreads();
+        for (int t = l; t < kcn; t += TW) xs[g][t] = s[kc + t];
+        __syncthreads();
+        if (i < N)
+            for (int kk = 0; kk < kcn; kk++) acc += A[(size_t)(kc + kk) * N + i] * xs[g][kk];
+    }
+    part[g][l] = acc;
+    __syncthreads();
+    if (g == 0 && i < N) d[i] = scale * ((part[0][l] + part[1][l]) + (part[2][l] + part[3][l]));
+}
+
+// Set-up of the radial operators: out[o N N + j N + i] = sum_m B[m N + i] w[o N + m] A[m N + j]
+// (B column-major, A row-major), the convolution Q^-1 diag(w_o) Q of multiplier o.
+__global__ void k_radial_op(const double *__restrict__ A, const double *__restrict__ B, const double *__restrict__ w,
+                            int N, double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, o = blockIdx.z;
+    if (i >= N) return;
+    const double *wo = w + (size_t)o * N;
+    double acc = 0.0;
+    for (int m = 0; m < N; m++) acc += B[(size_t)m * N + i] * wo[m] * A[(size_t)m * N + j];
+    out[(size_t)o * N * N + (size_t)j * N + i] = acc;
+}
+
 // Phi and dPhi/dn at every grid point.
 template <int NC, int ND, int AS = 0>
 __global__ __launch_bounds__(256) void k_phi(const double *__restrict__ rho, const double *__restrict__ n,

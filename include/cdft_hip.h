@@ -78,7 +78,17 @@ typedef struct {
     const int *assoc_pair_site;       /* [2*npair] the two sites (a, b) per pair */
     const double *assoc_pair_eps;     /* [npair] eps_AB / k_B, K                 */
     const double *assoc_pair_kappa;   /* [npair] bonding volume kappa_AB         */
+    /* coordinates (base/structure.jl:363-433): CDFT_CARTESIAN, or CDFT_SPHERICAL /
+     * CDFT_CYLINDRICAL with ndim = 1: a radial grid of ngrid[0] points over
+     * [0, ub[0]] (the sample radii of a quasi-discrete Hankel transform; lb[0]
+     * places walls only), FP64 only, at most CDFT_RADIAL_MAXN points */
+    int coord;
 } cdft_system_desc;
+
+#define CDFT_CARTESIAN 0
+#define CDFT_SPHERICAL 1
+#define CDFT_CYLINDRICAL 2
+#define CDFT_RADIAL_MAXN 2048
 
 /* converge! keywords (src/methods/converge.jl:3-13, :51-60). */
 typedef struct {

@@ -13,6 +13,7 @@
  * table, so any grid size works.
  */
 #define _POSIX_C_SOURCE 200809L
+#define _DEFAULT_SOURCE              /* j0, j1 (glibc's Bessel functions of the first kind) */
 #include "oracle.h"
 
 #include <float.h>
@@ -238,8 +239,15 @@ static void convolve(const or_system *s, const double *src, const double *kre, c
     for (int j = 0; j < N; j++) dst[j] = br[j];
 }
 
+static void radial_weighted(const or_system *s, const double *rho, double *n);
+static void radial_transposed(const or_system *s, const double *df, double *out);
+
 void or_weighted_densities(const or_system *s, const double *rho, double *n)
 {
+    if (s->coord) {
+        radial_weighted(s, rho, n);
+        return;
+    }
     int N = or_npoints(s), F = 6 + s->nd;
     double NA = OR_NA * (s->L * s->L * s->L);
     double *ore = malloc(sizeof(double) * (size_t)N * F * s->nc);
@@ -265,6 +273,10 @@ void or_weighted_densities(const or_system *s, const double *rho, double *n)
  * assigns, others add), :368-383 (vector: subtract). */
 void or_transposed(const or_system *s, const double *dphi, double *out)
 {
+    if (s->coord) {
+        radial_transposed(s, dphi, out);
+        return;
+    }
     int N = or_npoints(s), F = 6 + s->nd;
     double *ore = malloc(sizeof(double) * (size_t)N * F * s->nc);
     double *oim = malloc(sizeof(double) * (size_t)N * F * s->nc);
@@ -651,6 +663,272 @@ void or_assoc_X(const or_system *s, const double *nl, double *X)
     if (s->npair > 0) or_assoc(s, nl, n2, n3, nv2, NULL, X);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Radial (spherical / cylindrical) geometries: base/structure.jl:363-433,     */
+/* weighted_densities.jl:92-116 (scalar), :249-343 (vector), integrals.jl:     */
+/* 127-137 (convolve! with a QDHT), transforms.jl:35-63 (derivative matrix),   */
+/* integrals.jl:20-22 (int = 4 pi / 2 pi integrateR).  The transform itself is  */
+/* Hankel.jl's QDHT (a dependency absent here), restated from its published    */
+/* algorithm (Guizar-Sicairos & Gutierrez-Vega 2004).                          */
+/* ------------------------------------------------------------------------ */
+
+static double bessel_jp(int sph, double x)       /* J_0, or J_{1/2} for the spherical transform */
+{
+    return sph ? sqrt(2.0 / (PI * x)) * sin(x) : j0(x);
+}
+
+static double bessel_jp1(int sph, double x)      /* J_1, or J_{3/2} */
+{
+    return sph ? sqrt(2.0 / (PI * x)) * (sin(x) / x - cos(x)) : j1(x);
+}
+
+static double bessel_zero(int sph, int s)        /* s-th positive zero of J_0 or J_{1/2} */
+{
+    if (sph) return s * PI;
+    double b = (s - 0.25) * PI;                  /* McMahon's expansion, then Newton */
+    double x = b + 1.0 / (8.0 * b) - 124.0 / (3.0 * pow(8.0 * b, 3));
+    for (int it = 0; it < 50; it++) {
+        double dx = j0(x) / j1(x);               /* J0' = -J1 */
+        x += dx;
+        if (fabs(dx) < 1e-15 * x) break;
+    }
+    return x;
+}
+
+or_qdht *or_qdht_create(int coord, double R, int N)
+{
+    or_qdht *q = calloc(1, sizeof(or_qdht));
+    q->N = N;
+    q->sph = (coord == 1);
+    q->R = R;
+    double *zr = malloc(sizeof(double) * (N + 1));
+    for (int n = 0; n < N + 1; n++) zr[n] = bessel_zero(q->sph, n + 1);
+    q->S = zr[N];
+    q->K = q->S / R;
+    q->r = malloc(sizeof(double) * N);
+    q->k = malloc(sizeof(double) * N);
+    q->jp1 = malloc(sizeof(double) * N);
+    q->w = malloc(sizeof(double) * N);
+    q->C = malloc(sizeof(double) * (size_t)N * N);
+    for (int n = 0; n < N; n++) {
+        q->r[n] = zr[n] * R / q->S;
+        q->k[n] = zr[n] / R;
+        q->jp1[n] = fabs(bessel_jp1(q->sph, zr[n]));
+    }
+    for (int m = 0; m < N; m++)
+        for (int n = 0; n < N; n++)
+            q->C[(size_t)m * N + n] = 2.0 / q->S * bessel_jp(q->sph, zr[m] * zr[n] / q->S) / (q->jp1[m] * q->jp1[n]);
+    /* the transform's quadrature int_0^R g(r) r dr = sum g(r_n) 2 / (K^2 J_{p+1}(j_n)^2),
+     * with g = f (cylindrical) or g = f r (spherical), times the angular factor */
+    for (int n = 0; n < N; n++) {
+        double wq = 2.0 / (q->K * q->K * q->jp1[n] * q->jp1[n]);
+        q->w[n] = q->sph ? 4.0 * PI * wq * q->r[n] : 2.0 * PI * wq;
+    }
+    free(zr);
+    return q;
+}
+
+void or_qdht_free(or_qdht *q)
+{
+    if (!q) return;
+    free(q->r); free(q->k); free(q->jp1); free(q->w); free(q->C); free(q);
+}
+
+/* forward: F(k_m) = int f j0(k r) r^2 dr (spherical) or int f J0(k r) r dr (cylindrical),
+ * from the samples: g = f (times sqrt(r) for the order-1/2 form), scaled by R / |J_{p+1}|,
+ * multiplied by C, scaled back by |J_{p+1}| / K (times sqrt(pi / 2k)). */
+void or_qdht_forward(const or_qdht *q, const double *f, double *F)
+{
+    int N = q->N;
+    double *a = malloc(sizeof(double) * N);
+    for (int n = 0; n < N; n++) a[n] = f[n] * (q->sph ? sqrt(q->r[n]) : 1.0) * q->R / q->jp1[n];
+    for (int m = 0; m < N; m++) {
+        double acc = 0.0;
+        for (int n = 0; n < N; n++) acc += q->C[(size_t)m * N + n] * a[n];
+        F[m] = acc * q->jp1[m] / q->K * (q->sph ? sqrt(PI / (2.0 * q->k[m])) : 1.0);
+    }
+    free(a);
+}
+
+void or_qdht_backward(const or_qdht *q, const double *F, double *f)
+{
+    int N = q->N;
+    double *a = malloc(sizeof(double) * N);
+    for (int m = 0; m < N; m++) a[m] = F[m] / (q->sph ? sqrt(PI / (2.0 * q->k[m])) : 1.0) * q->K / q->jp1[m];
+    for (int n = 0; n < N; n++) {
+        double acc = 0.0;
+        for (int m = 0; m < N; m++) acc += q->C[(size_t)n * N + m] * a[m];
+        f[n] = acc * q->jp1[n] / q->R / (q->sph ? sqrt(q->r[n]) : 1.0);
+    }
+    free(a);
+}
+
+/* convolve!(result, profile, kernel, P::QDHT, iP::QDHT, buf), integrals.jl:127-137 */
+static void radial_convolve(const or_qdht *q, const double *src, const double *kern, double *dst)
+{
+    int N = q->N;
+    double *t = malloc(sizeof(double) * N);
+    or_qdht_forward(q, src, t);
+    for (int m = 0; m < N; m++) t[m] *= kern[m];
+    or_qdht_backward(q, t, dst);
+    free(t);
+}
+
+/* radial_derivative_matrix(r), transforms.jl:43-63 (row-major D[i*N+j]) */
+static void radial_derivative_matrix(const double *r, int N, double *D)
+{
+    memset(D, 0, sizeof(double) * (size_t)N * N);
+    for (int i = 1; i < N - 1; i++) {
+        double h1 = r[i] - r[i - 1], h2 = r[i + 1] - r[i];
+        D[(size_t)i * N + i - 1] = -h2 / (h1 * (h1 + h2));
+        D[(size_t)i * N + i] = (h2 - h1) / (h1 * h2);
+        D[(size_t)i * N + i + 1] = h1 / (h2 * (h1 + h2));
+    }
+    double h1 = r[1] - r[0], h2 = r[2] - r[1];
+    D[0] = -(2 * h1 + h2) / (h1 * (h1 + h2));
+    D[1] = (h1 + h2) / (h1 * h2);
+    D[2] = -h1 / (h2 * (h1 + h2));
+    h1 = r[N - 2] - r[N - 3];
+    h2 = r[N - 1] - r[N - 2];
+    D[(size_t)(N - 1) * N + N - 3] = h2 / (h1 * (h1 + h2));
+    D[(size_t)(N - 1) * N + N - 2] = -(h1 + h2) / (h1 * h2);
+    D[(size_t)(N - 1) * N + N - 1] = (2 * h2 + h1) / (h2 * (h1 + h2));
+}
+
+/* The radial kernels of component c: scalar slots sampled at omega = k / 2pi in
+ * reduced units (weighted_densities.jl:92-116, no k = 0 sample), and the vector
+ * slot's operator T = D Q^-1 H Q materialised column by column in unscaled units
+ * (weighted_densities.jl:290-316); kern[f*N + m], T[i*N + j]. */
+static void radial_kernels(const or_system *s, const or_qdht *q, int c, double *kern, double *T)
+{
+    int N = q->N, F = 7;
+    const double twopi = 2.0 * PI, fourpi = 4.0 * PI, tp3 = twopi * twopi * twopi;
+    double dred = s->d[c] / s->L;
+    for (int f = 1; f < F; f++) {
+        int kind, vdir;
+        double wfac;
+        field_desc(1, f, &kind, &wfac, &vdir, s->psi);
+        double width = (f == 6) ? dred * s->psi : wfac * dred;
+        double R = twopi * width;
+        for (int m = 0; m < N; m++) {
+            double wb = q->k[m] / twopi * s->L, wbR = wb * R;
+            double v = 0.0;
+            if (kind == K_S) v = 2.0 * sin(wbR) / wb / twopi;
+            else if (kind == K_V || kind == K_VEC) v = fourpi / (wb * wb * wb) * (sin(wbR) - R * wb * cos(wbR)) / tp3;
+            kern[(size_t)f * N + m] = v;
+        }
+    }
+    double Rk = twopi * 0.5 * dred * s->L;
+    double *H = malloc(sizeof(double) * N), *e = calloc(N, sizeof(double)), *col = malloc(sizeof(double) * N);
+    double *M = malloc(sizeof(double) * (size_t)N * N), *D = malloc(sizeof(double) * (size_t)N * N);
+    for (int m = 0; m < N; m++) {
+        double wb = q->k[m] / twopi;
+        H[m] = fourpi / (wb * wb * wb) * (sin(wb * Rk) - Rk * wb * cos(wb * Rk)) / tp3;
+    }
+    for (int k = 0; k < N; k++) {
+        e[k] = 1.0;
+        radial_convolve(q, e, H, col);
+        e[k] = 0.0;
+        for (int i = 0; i < N; i++) M[(size_t)i * N + k] = col[i];
+    }
+    radial_derivative_matrix(q->r, N, D);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            double acc = 0.0;
+            for (int l = 0; l < N; l++) acc += D[(size_t)i * N + l] * M[(size_t)l * N + j];
+            T[(size_t)i * N + j] = acc;
+        }
+    free(H); free(e); free(col); free(M); free(D);
+}
+
+/* The radial operators of every component: kernels kern[(c F + f) N + m] and the
+ * vector slot's T[c N N + i N + j]. */
+static or_qdht *radial_ops(const or_system *s, double **kern, double **T)
+{
+    int N = s->ngrid[0], F = 7, nc = s->nc;
+    or_qdht *q = or_qdht_create(s->coord, s->ub[0], N);
+    *kern = malloc(sizeof(double) * (size_t)F * N * nc);
+    *T = malloc(sizeof(double) * (size_t)N * N * nc);
+    for (int c = 0; c < nc; c++) radial_kernels(s, q, c, *kern + (size_t)c * F * N, *T + (size_t)c * N * N);
+    return q;
+}
+
+/* evaluate_field! for QDHT plans (weighted_densities.jl:118-138, :318-328) */
+static void radial_weighted(const or_system *s, const double *rho, double *n)
+{
+    int N = s->ngrid[0], F = 7, nc = s->nc;
+    double NA = OR_NA * (s->L * s->L * s->L);
+    double *kern, *T;
+    or_qdht *q = radial_ops(s, &kern, &T);
+    for (int c = 0; c < nc; c++) {
+        const double *rc = rho + (size_t)c * N;
+        const double *Tc = T + (size_t)c * N * N;
+        for (int f = 0; f < F; f++) {
+            double *dst = n + (size_t)(c * F + f) * N;
+            if (f == 0) {
+                for (int j = 0; j < N; j++) dst[j] = rc[j] * NA;
+            } else if (f == 3) {
+                for (int i = 0; i < N; i++) {
+                    double acc = 0.0;
+                    for (int j = 0; j < N; j++) acc += Tc[(size_t)i * N + j] * rc[j];
+                    dst[i] = acc * NA;
+                }
+            } else {
+                radial_convolve(q, rc, kern + (size_t)(c * F + f) * N, dst);
+                for (int j = 0; j < N; j++) dst[j] *= NA;
+            }
+        }
+    }
+    or_qdht_free(q);
+    free(kern); free(T);
+}
+
+/* integrate_field! for QDHT plans (weighted_densities.jl:140-163, :330-343): the rho
+ * slot assigns, the scalar slots add the same convolution, the vector slot adds
+ * transpose(T) dPhi/dn_v. */
+static void radial_transposed(const or_system *s, const double *df, double *out)
+{
+    int N = s->ngrid[0], F = 7, nc = s->nc;
+    double *kern, *T;
+    or_qdht *q = radial_ops(s, &kern, &T);
+    double *tmp = malloc(sizeof(double) * N);
+    for (int c = 0; c < nc; c++) {
+        double *oc = out + (size_t)c * N;
+        const double *Tc = T + (size_t)c * N * N;
+        for (int j = 0; j < N; j++) oc[j] = df[(size_t)(c * F) * N + j];
+        for (int f = 1; f < F; f++) {
+            const double *u = df + (size_t)(c * F + f) * N;
+            if (f == 3) {
+                for (int j = 0; j < N; j++) {
+                    double acc = 0.0;
+                    for (int i = 0; i < N; i++) acc += Tc[(size_t)i * N + j] * u[i];
+                    oc[j] += acc;
+                }
+            } else {
+                radial_convolve(q, u, kern + (size_t)(c * F + f) * N, tmp);
+                for (int j = 0; j < N; j++) oc[j] += tmp[j];
+            }
+        }
+    }
+    or_qdht_free(q);
+    free(kern); free(T); free(tmp);
+}
+
+/* int f dV on the structure: Simpson (Cartesian) or 4 pi / 2 pi integrateR (radial) */
+static double or_integrate(const or_system *s, const double *f)
+{
+    if (s->coord) {
+        or_qdht *q = or_qdht_create(s->coord, s->ub[0], s->ngrid[0]);
+        double acc = 0.0;
+        for (int n = 0; n < q->N; n++) acc += q->w[n] * f[n];
+        or_qdht_free(q);
+        return acc;
+    }
+    double dz[3];
+    for (int a = 0; a < s->nd; a++) dz[a] = (s->ub[a] - s->lb[a]) / s->ngrid[a];   /* structure.jl:187-195 */
+    return or_simpson(f, s->nd, s->ngrid, dz);
+}
+
 /* delta F_res / delta rho (models.jl:131-186): fields, pointwise gradient, transposed. */
 void or_dFdrho_res(const or_system *s, const double *rho, double *dfdrho, double *fval)
 {
@@ -695,10 +973,6 @@ double or_simpson(const double *f, int nd, const int *ngrid, const double *dz)
     return sum * sc;
 }
 
-static void structure_dz(const or_system *s, double *dz)
-{
-    for (int a = 0; a < s->nd; a++) dz[a] = (s->ub[a] - s->lb[a]) / s->ngrid[a];   /* structure.jl:187-195 */
-}
 
 /* F_res = int f_val / L^3 (models.jl:86-91, :73) */
 double or_F_res(const or_system *s, const double *rho)
@@ -707,9 +981,7 @@ double or_F_res(const or_system *s, const double *rho)
     double *fv = malloc(sizeof(double) * N);
     double *tmp = malloc(sizeof(double) * (size_t)N * s->nc);
     or_dFdrho_res(s, rho, tmp, fv);
-    double dz[3];
-    structure_dz(s, dz);
-    double r = or_simpson(fv, s->nd, s->ngrid, dz) / (s->L * s->L * s->L);
+    double r = or_integrate(s, fv) / (s->L * s->L * s->L);
     free(fv);
     free(tmp);
     return r;
@@ -729,9 +1001,7 @@ double or_F_ideal(const or_system *s, const double *rho)
         }
         phi[j] = acc;
     }
-    double dz[3];
-    structure_dz(s, dz);
-    double r = or_simpson(phi, s->nd, s->ngrid, dz);
+    double r = or_integrate(s, phi);
     free(phi);
     return r;
 }
@@ -1167,14 +1437,12 @@ void or_init_two_phase(const or_system *s, const double *rho2, const double *Tc,
 double or_surface_tension(const or_system *s, const double *rho, const double *mu, double p)
 {
     int N = or_npoints(s);
-    double dz[3];
-    structure_dz(s, dz);
     double F = or_F_ideal(s, rho) + or_F_res(s, rho);
     double chem = 0.0;
-    for (int c = 0; c < s->nc; c++) chem += mu[c] * or_simpson(rho + (size_t)c * N, s->nd, s->ngrid, dz);
+    for (int c = 0; c < s->nc; c++) chem += mu[c] * or_integrate(s, rho + (size_t)c * N);
     double *ones = malloc(sizeof(double) * N);
     for (int j = 0; j < N; j++) ones[j] = 1.0;
-    double vol = or_simpson(ones, s->nd, s->ngrid, dz);
+    double vol = or_integrate(s, ones);
     free(ones);
     return F * OR_KB * s->T - chem + p * vol;
 }
@@ -1182,10 +1450,13 @@ double or_surface_tension(const or_system *s, const double *rho, const double *m
 void or_adsorption(const or_system *s, const double *rho, double *out)
 {
     int N = or_npoints(s);
-    double dz[3], V = 1.0;
-    structure_dz(s, dz);
-    for (int a = 0; a < s->nd; a++) V *= (s->ub[a] - s->lb[a]);
-    for (int c = 0; c < s->nc; c++) out[c] = or_simpson(rho + (size_t)c * N, s->nd, s->ngrid, dz) / V;
+    double V = 1.0;                                      /* _domain_volume, adsorption.jl:23-25 */
+    const double lo = s->lb[0], hi = s->ub[0];
+    if (s->coord == 1) V = 4.0 * PI / 3.0 * (hi * hi * hi - lo * lo * lo);
+    else if (s->coord == 2) V = PI * (hi * hi - lo * lo);
+    else
+        for (int a = 0; a < s->nd; a++) V *= (s->ub[a] - s->lb[a]);
+    for (int c = 0; c < s->nc; c++) out[c] = or_integrate(s, rho + (size_t)c * N) / V;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1253,7 +1524,7 @@ void or_plan_destroy(or_plan *p)
 
 or_plan *or_plan_create(const or_system *s)
 {
-    if (s->nd != 1) return NULL;
+    if (s->nd != 1 || s->coord) return NULL;
     or_plan *p = calloc(1, sizeof(or_plan));
     const int N = s->ngrid[0], F = 7, nc = s->nc;
     int M = 1, lg = 0;

@@ -52,6 +52,10 @@ typedef struct {
     int pair_a[OR_MAXP], pair_b[OR_MAXP]; /* global site indices of a pair  */
     double pair_eps[OR_MAXP];        /* eps_AB / k_B, K                     */
     double pair_kappa[OR_MAXP];      /* bonding volume kappa_AB             */
+    /* coordinates (base/structure.jl:363-433): 0 Cartesian; 1 spherical, 2
+     * cylindrical (nd = 1, radial grid of a quasi-discrete Hankel transform
+     * over [0, ub]; lb places walls only) */
+    int coord;
 } or_system;
 
 typedef struct {
@@ -70,6 +74,23 @@ typedef void (*or_gfix)(double *G, const double *x, void *ctx);
 #define OR_NA 6.02214076e23
 #define OR_KB 1.380649e-23
 #define OR_R  8.31446261815324
+
+/* Quasi-discrete Hankel transform of order 0 (Hankel.jl QDHT, restated from
+ * Guizar-Sicairos & Gutierrez-Vega, JOSA A 21, 53 (2004)): cylindrical = the
+ * 2D radial Fourier transform (roots of J0), spherical = the 3D one as the
+ * order-1/2 transform of sqrt(r) f (roots n pi, a sine transform). */
+typedef struct {
+    int N, sph;
+    double R, S, K;                  /* aperture, j_{N+1}, S / R            */
+    double *r, *k;                   /* sample radii and angular wavenumbers */
+    double *jp1;                     /* |J_{p+1}(j_n)|                       */
+    double *C;                       /* C[m*N+n] = 2/S J_p(j_m j_n / S) / (|J_{p+1}(j_m)| |J_{p+1}(j_n)|) */
+    double *w;                       /* integration weights of int f dV (4 pi r^2 dr or 2 pi r dr) */
+} or_qdht;
+or_qdht *or_qdht_create(int coord, double R, int N);
+void   or_qdht_free(or_qdht *q);
+void   or_qdht_forward(const or_qdht *q, const double *f, double *F);   /* F(k_m) ~ Fourier transform */
+void   or_qdht_backward(const or_qdht *q, const double *F, double *f);  /* its inverse                  */
 
 int    or_nfields(int nd);
 int    or_npoints(const or_system *s);

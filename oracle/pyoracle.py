@@ -32,6 +32,7 @@ class OrSystem(C.Structure):
         ("site_comp", C.c_int * MAXS), ("site_count", C.c_double * MAXS),
         ("pair_a", C.c_int * MAXP), ("pair_b", C.c_int * MAXP),
         ("pair_eps", C.c_double * MAXP), ("pair_kappa", C.c_double * MAXP),
+        ("coord", C.c_int),
     ]
 
 
@@ -112,9 +113,10 @@ class Oracle:
 
     def __init__(self, *, nc, nd, ngrid, lb, ub, T, L, m, d, sigma, eps, mu_res, rho_bulk,
                  vext=None, psi=1.3862, assoc_site_comp=(), assoc_site_count=(), assoc_pairs=(),
-                 assoc_eps=(), assoc_kappa=()):
+                 assoc_eps=(), assoc_kappa=(), coord=0):
         s = OrSystem()
         s.nc, s.nd = nc, nd
+        s.coord = int(coord)
         for a in range(nd):
             s.ngrid[a] = int(ngrid[a]); s.lb[a] = float(lb[a]); s.ub[a] = float(ub[a])
         for a in range(nd, 3):
