@@ -11,12 +11,17 @@ in HBM).  Prints one JSON line.  Besides the headline workload the line carries:
                where they differ, and the batched and plain results' largest
                relative differences from the oracle (profile, adsorption, tension);
   c0, c1       the plain FP64 path on the methane wall (201 points) and on the
-               methane / n-decane interface (2048 points);
-  matrix_3d    the 3D hard-sphere-matrix ternary on a 256^3 grid: one mixed-precision
-               solve, stream time per map evaluation and the algorithmic-byte rate;
+               methane / n-decane interface (2048 points)
+               (each with its mixed-precision solve beside the FP64 one);
+  matrix_3d    the 3D hard-sphere-matrix ternary on a 256^3 grid: mixed-precision and
+               FP64 solves, stream time per map evaluation, the algorithmic-byte rate
+               and the weighted-density + transposed passes against the HBM roofline;
+  matrix_2d    the same mixture with hard disks on a 512^2 grid, FP64 and mixed;
+  radial       methane in spherical and cylindrical Steele pores (151 radial points);
   adjoint      surface-tension sensitivities of the methane / n-decane interface
-               (2048 points): adjoint solve versus finite differences of full solves.
-(the last four are skipped with --no-extras).
+               (2048 points) from FP64 and mixed solves versus finite differences of
+               full solves.
+(the extras are skipped with --no-extras).
 """
 import argparse
 import json
@@ -151,7 +156,7 @@ def cpu_oracle_solve(s, rho0, tol, maxit=10000):
     import pyoracle as po
     o = po.Oracle(**s.descriptor())
     t = time.perf_counter()
-    x, res, _ = o.converge(rho0, tol=tol, maxit=maxit, planned=True)
+    x, res, _ = o.converge(rho0, tol=tol, maxit=maxit, planned=not s.structure.coord)
     return o, x, res, time.perf_counter() - t
 
 
@@ -167,6 +172,16 @@ def run_c0(np, configs, converge_, nsolve=20):
     dt = (time.perf_counter() - t) / nsolve
     r = s.last_result
     o, x, ro, secs = cpu_oracle_solve(s, rho0, 1e-8)
+    from cdft_amd import DFTOptions
+    sm, _ = configs.methane_wall(options=DFTOptions(precision="mixed"))
+    xm = rho0.copy()
+    converge_(sm, xm, tol=1e-8)
+    t = time.perf_counter()
+    for _ in range(nsolve):
+        xm = rho0.copy()
+        converge_(sm, xm, tol=1e-8)
+    dtm = (time.perf_counter() - t) / nsolve
+    rm = sm.last_result
     return {"workload": "methane at 150 K, 1e7 Pa, graphite Steele walls 5 sigma apart, 201 points, tol 1e-8; FP64",
             "errcode": r["errcode"], "nevals": r["nevals"], "solve_ms": dt * 1e3, "profiles_per_s": 1.0 / dt,
             "stream_ms": r["device_ms"],
@@ -175,7 +190,9 @@ def run_c0(np, configs, converge_, nsolve=20):
                              "errcode": ro.errcode, "nevals": ro.nevals},
             "max_rel_err_vs_oracle": {"profile": relmax(rho.reshape(-1), x),
                                       "adsorption": relmax(np.asarray(s.device_system.adsorption(rho)),
-                                                           o.adsorption(x))}}
+                                                           o.adsorption(x))},
+            "mixed": {"errcode": rm["errcode"], "nevals": rm["nevals"], "nevals_fp32": rm["nevals_fp32"],
+                      "solve_ms": dtm * 1e3, "profile_max_rel_err_vs_oracle": relmax(xm.reshape(-1), x)}}
 
 
 def run_c1(np, configs, converge_):
@@ -191,6 +208,14 @@ def run_c1(np, configs, converge_):
     gamma = s.device_system.surface_tension(rho, bulk_mu(s), p) / 2
     o, x, ro, secs = cpu_oracle_solve(s, rho0, 1e-8, maxit=20000)
     gamma_o = o.surface_tension(x, bulk_mu(s), p) / 2
+    from cdft_amd import DFTOptions
+    sm, _ = configs.c1c10_interface(options=DFTOptions(precision="mixed"))
+    xm = rho0.copy()
+    t = time.perf_counter()
+    converge_(sm, xm, tol=1e-8, maxit=20000)
+    dtm = time.perf_counter() - t
+    rm = sm.last_result
+    gamma_m = sm.device_system.surface_tension(xm, bulk_mu(sm), p) / 2
     return {"workload": "methane/n-decane interface, x=[0.5,0.5] liquid at its bubble point, T=300 K, 2048 points "
                         "over [-20 L, 20 L], tol 1e-8; FP64",
             "errcode": r["errcode"], "nevals": r["nevals"], "solve_s": dt, "profiles_per_s": 1.0 / dt,
@@ -202,13 +227,16 @@ def run_c1(np, configs, converge_):
                                       "profile_unaligned": relmax(rho.reshape(-1), x),
                                       "note": "a free interface's position is a zero mode: at tol 1e-8 the two "
                                               "solves stop with the interfaces shifted by a fraction of a grid "
-                                              "step, which dominates the unaligned profile difference"}}
+                                              "step, which dominates the unaligned profile difference"},
+            "mixed": {"errcode": rm["errcode"], "nevals": rm["nevals"], "nevals_fp32": rm["nevals_fp32"],
+                      "solve_s": dtm, "surface_tension_rel_err_vs_oracle": abs(gamma_m / gamma_o - 1)}}
 
 
 def run_matrix_3d(np, DFTOptions, configs, converge_):
     from cdft_amd.methods import adsorption_system
     # the FP64 solve first: the reference for the mixed-precision result
     s64, rho64 = configs.matrix_3d(n=256)
+    s64.device_system.tune()                     # the autotuner's measurement stays out of the timing
     t = time.perf_counter()
     converge_(s64, rho64, tol=1e-8, **configs.MATRIX_3D_SOLVER)
     wall64 = time.perf_counter() - t
@@ -216,6 +244,7 @@ def run_matrix_3d(np, DFTOptions, configs, converge_):
     ads64 = adsorption_system(s64, rho64)
     del s64
     s, rho = configs.matrix_3d(n=256, options=DFTOptions(precision="mixed"))
+    s.device_system.tune()
     r = rho.copy()
     t = time.perf_counter()
     converge_(s, r, tol=1e-8, **configs.MATRIX_3D_SOLVER)
@@ -226,6 +255,16 @@ def run_matrix_3d(np, DFTOptions, configs, converge_):
     alg = s.N * s.nc * (n32 * bytes_per_point_eval(4, F=F) + n64 * bytes_per_point_eval(8, F=F))
     ms = res["device_ms"]
     ads = adsorption_system(s, r)
+    tune = s.device_system.get_tuning()
+    # compulsory bytes of one weighted-density and one transposed-convolution pass:
+    # B_WD + B_T = 2 N C (1 + F) s
+    wd = {}
+    for pr, (name, sz) in enumerate((("fp64", 8), ("fp32", 4))):
+        b = 2.0 * s.N * s.nc * (1 + F) * sz
+        tms = tune["ms"][pr]
+        wd[name] = {"ms": tms, "compulsory_bytes": b,
+                    "frac_of_hbm_peak": (b / (tms * 1e-3) / 1e9 / HBM_PEAK_GBS) if tms > 0 else None,
+                    "lines_per_tile": tune["lines_per_tile"][pr], "fused_z": tune["fused_z"][pr]}
     return {
         "workload": "methane/ethane/propane x=[0.6,0.3,0.1], T=200 K, p=1e7 Pa, 30 sigma periodic cube on "
                     "256^3 points, 64 hard spheres r~U[1,2.5] sigma (seed 0); mixed precision, tol 1e-8, "
@@ -238,10 +277,62 @@ def run_matrix_3d(np, DFTOptions, configs, converge_):
                  "adsorption_mol_m3": [float(v) for v in ads64]},
         "adsorption_max_rel_diff_mixed_vs_fp64": float(np.max(np.abs(ads - ads64) / np.abs(ads64))),
         "profile_max_rel_diff_mixed_vs_fp64": relmax(r.reshape(-1), rho64.reshape(-1)),
+        "weighted_densities_and_transposed": wd,
+        "wd_note": "one weighted-density + transposed-convolution sequence per map precision, timed by the "
+                   "autotuner with HIP events for its chosen variant; compulsory bytes per SURVEY 8(d): "
+                   "B_WD + B_T = 2 N C (1+F) s",
     }
 
 
+def run_matrix_2d(np, DFTOptions, configs, converge_):
+    from cdft_amd.methods import adsorption_system
+    kw = dict(n=512, box=30.0, ndisks=16, seed=0)
+    out = {"workload": "methane/ethane/propane x=[0.6,0.3,0.1], T=200 K, p=1e7 Pa, 30 sigma periodic square on "
+                       "512^2 points, 16 hard disks r~U[1,2.5] sigma (seed 0), tol 1e-8, anderson_start 0.5"}
+    ads = {}
+    for prec in ("fp64", "mixed"):
+        s, rho = configs.matrix_2d(options=DFTOptions(precision=prec), **kw)
+        s.device_system.tune()
+        t = time.perf_counter()
+        converge_(s, rho, tol=1e-8, **configs.MATRIX_3D_SOLVER)
+        wall = time.perf_counter() - t
+        res = s.last_result
+        ads[prec] = adsorption_system(s, rho)
+        out[prec] = {"errcode": res["errcode"], "nevals": res["nevals"], "nevals_fp32": res["nevals_fp32"],
+                     "solve_s": wall, "ms_per_eval": res["device_ms"] / max(res["nevals"], 1),
+                     "tuning": s.device_system.get_tuning()}
+        del s
+    out["adsorption_max_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(ads["mixed"] - ads["fp64"]) /
+                                                                np.abs(ads["fp64"])))
+    return out
+
+
+def run_radial(np, configs, converge_):
+    out = {}
+    for coord in ("spherical", "cylindrical"):
+        s, rho0 = configs.methane_radial(coord)
+        r = rho0.copy()
+        converge_(s, r, tol=1e-8)
+        t = time.perf_counter()
+        for _ in range(5):
+            r = rho0.copy()
+            converge_(s, r, tol=1e-8)
+        dt = (time.perf_counter() - t) / 5
+        res = s.last_result
+        o, x, ro, secs = cpu_oracle_solve(s, rho0.reshape(-1), 1e-8)
+        out[coord] = {"workload": f"methane at 150 K, 1e7 Pa in a {coord} graphite Steele pore of radius 5 sigma, "
+                                  "151 radial points, tol 1e-8; FP64",
+                      "errcode": res["errcode"], "nevals": res["nevals"], "solve_ms": dt * 1e3,
+                      "cpu_baseline": {"value": 1.0 / secs, "unit": "profiles/s", "cores": 1, "kind": "port",
+                                       "sample": "the same solve, FP64 C oracle (literal QDHT per field, operators "
+                                                 "rebuilt at every evaluation), 1 thread",
+                                       "errcode": ro.errcode, "nevals": ro.nevals},
+                      "profile_max_rel_err_vs_oracle": relmax(r.reshape(-1), x)}
+    return out
+
+
 def run_adjoint(np, configs, converge_):
+    from cdft_amd import DFTOptions
     from cdft_amd import adjoint as adj
     s, rho0 = configs.c1c10_interface(ngrid=2048, box=20.0)
     r = rho0.copy()
@@ -252,28 +343,44 @@ def run_adjoint(np, configs, converge_):
     t = time.perf_counter()
     sens, lam, res = adj.sensitivities(s, rho, "gamma", tol=1e-10)
     t_adj = time.perf_counter() - t
-    # finite differences of full solves for two of the six parameters
+    # the mixed-precision leg: sensitivities at the profile of a mixed solve
+    sm, _ = configs.c1c10_interface(ngrid=2048, box=20.0, options=DFTOptions(precision="mixed"))
+    rm = rho0.copy()
+    converge_(sm, rm, tol=1e-12, maxit=20000)
+    sens_m, _, res_m = adj.sensitivities(sm, rm.reshape(-1), "gamma", tol=1e-10)
+    # central differences of full FP64 solves for two of the six parameters, at the
+    # configured step (1e-6 relative) and at 1e-5
     fd = {}
     t = time.perf_counter()
     for key in (("epsilon", 0), ("sigma", 1)):
         prm, i = key
         b0 = adj.model_params(s.model)[s.model.components[i]][{"segment": 0, "sigma": 1, "epsilon": 2}[prm]]
-        h = 1e-5 * abs(b0)
-        vals = []
-        for sg in (+1, -1):
-            sp = adj.perturbed_system(s, i, prm, sg * h)
-            rr = rho.copy().reshape(s.nc, s.N)
-            converge_(sp, rr, tol=1e-12, maxit=20000)
-            vals.append(adj.objective(sp, rr, "gamma"))
-        fd[key] = (vals[0] - vals[1]) / (2 * h)
-    t_fd = (time.perf_counter() - t) / len(fd)
+        for rel_h in (1e-6, 1e-5):
+            h = rel_h * abs(b0)
+            vals = []
+            for sg in (+1, -1):
+                sp = adj.perturbed_system(s, i, prm, sg * h)
+                rr = rho.copy().reshape(s.nc, s.N)
+                converge_(sp, rr, tol=1e-12, maxit=20000)
+                vals.append(adj.objective(sp, rr, "gamma"))
+                del sp
+            fd[key + (rel_h,)] = (vals[0] - vals[1]) / (2 * h)
+    t_fd = (time.perf_counter() - t) / 4
+
+    def diff(sv, rel_h):
+        return max(abs(sv[k[:2]] - v) / abs(v) for k, v in fd.items() if k[2] == rel_h)
     return {
         "workload": "d(surface tension)/d(sigma_i, eps_i, m_i), methane/n-decane interface, 2048 points, "
-                    "40 sigma box, T=300 K; FP64",
+                    "40 sigma box, T=300 K; FP64 and mixed-precision forward solves, FP64 adjoint",
         "solve_s": t_solve, "adjoint_and_partials_s": t_adj, "adjoint_errcode": res["errcode"],
         "adjoint_nevals": res["nevals"], "fd_full_solves_s_per_parameter": t_fd,
         "dgamma": {f"{k[0]}_{k[1]}": v for k, v in sens.items()},
-        "max_rel_diff_vs_fd": max(abs(sens[k] - v) / abs(v) for k, v in fd.items()),
+        "dgamma_mixed": {f"{k[0]}_{k[1]}": v for k, v in sens_m.items()},
+        "mixed_forward": {"errcode": res_m["errcode"], "adjoint_nevals": res_m["nevals"]},
+        "max_rel_diff_vs_fd": {"fp64_h1e-6": diff(sens, 1e-6), "fp64_h1e-5": diff(sens, 1e-5),
+                               "mixed_h1e-6": diff(sens_m, 1e-6), "mixed_h1e-5": diff(sens_m, 1e-5)},
+        "fd_note": "the full solves stop on the interface's residual floor (about 1e-9, its translation mode "
+                   "barely contracts): about 1e-4 relative noise in differences with h = 1e-6, 1e-5 with h = 1e-5",
     }
 
 
@@ -425,6 +532,8 @@ def main():
         out["c0"] = run_c0(np, configs, converge_)
         out["c1"] = run_c1(np, configs, converge_)
         out["matrix_3d"] = run_matrix_3d(np, DFTOptions, configs, converge_)
+        out["matrix_2d"] = run_matrix_2d(np, DFTOptions, configs, converge_)
+        out["radial"] = run_radial(np, configs, converge_)
         out["adjoint"] = run_adjoint(np, configs, converge_)
     print(json.dumps(out))
 

@@ -849,7 +849,7 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
 
     int N = s->N, F = s->F;
     long n = (long)N * nc;
-    s->nblk = nblocks(n);
+    s->nblk = std::min(nblocks(n), 2048);   // reducing kernels: grid-stride over at most 2048 workgroups
     (void)hipSetDevice(ctx->device);
     auto alloc = [&](double **p, size_t count) { return hipMalloc((void **)p, sizeof(double) * count); };
     hipError_t e = hipSuccess;
@@ -1058,7 +1058,8 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     if (o->maxit < 0) return fail(ctx, CDFT_ERR_ARG, "maxit must be >= 0");
     hipStream_t st = ctx->stream;
     const long n = (long)s->N * s->nc;
-    const int nb = s->nblk;
+    const int nb = s->nblk;                      // reducing kernels (grid-stride)
+    const int ne = nblocks(n);                   // elementwise kernels
     HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
     if (!s->ctl) HIPCHECK_CTX(ctx, hipMalloc((void **)&s->ctl, sizeof(AACtl)));
     const int hcap = (o->hist && o->hist_cap > 0) ? o->hist_cap : 0;
@@ -1101,10 +1102,10 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     const double *pmask = adj ? adj->mask : nullptr;
     if (adj) {
         HIPCHECK_CTX(ctx, hipMemcpyAsync(s->sol, rho_inout, sizeof(double) * n, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_set_point, dim3(nb), dim3(256), 0, st, s->sol, s->sol, ptgt, (int)n, pmask);
+        hipLaunchKernelGGL(k_set_point, dim3(ne), dim3(256), 0, st, s->sol, s->sol, ptgt, (int)n, pmask);
     } else {
         HIPCHECK_CTX(ctx, hipMemcpyAsync(s->out, rho_inout, sizeof(double) * n, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_init_x, dim3(nb), dim3(256), 0, st, s->out, s->sol, s->rho, (int)n);
+        hipLaunchKernelGGL(k_init_x, dim3(ne), dim3(256), 0, st, s->out, s->sol, s->rho, (int)n);
     }
 
     AACtl *c = s->ctl;
@@ -1123,7 +1124,7 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
                            (const double *)(s->fin + CDFT_NQ), 1, s->red, nb);
         hipLaunchKernelGGL(k_aa_fin<1>, dim3(2), dim3(256), 0, st, (const double *)s->red, nb, s->fin + 2 * CDFT_NQ, c, 1,
                            s->fin, s->dhist);
-        hipLaunchKernelGGL(k_aa_point, dim3(nb), dim3(256), 0, st, (int)n, (const AACtl *)c, (const double *)s->df,
+        hipLaunchKernelGGL(k_aa_point, dim3(ne), dim3(256), 0, st, (int)n, (const AACtl *)c, (const double *)s->df,
                            s->Q, s->Qalt, s->sol, (const double *)s->gx, (const double *)s->dg, s->DG, s->xsafe, ptgt,
                            pmask);
         s->gate = &c->g_f64;
@@ -1189,9 +1190,9 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
         bodies += K;
     }
     // a restore left pending by the last step (back to the last finite FP32 point)
-    hipLaunchKernelGGL(k_aa_point, dim3(nb), dim3(256), 0, st, (int)n, (const AACtl *)c, (const double *)s->df, s->Q,
+    hipLaunchKernelGGL(k_aa_point, dim3(ne), dim3(256), 0, st, (int)n, (const AACtl *)c, (const double *)s->df, s->Q,
                        s->Qalt, s->sol, (const double *)s->gx, (const double *)s->dg, s->DG, s->xsafe, ptgt, pmask);
-    if (!adj) hipLaunchKernelGGL(k_exp, dim3(nb), dim3(256), 0, st, s->sol, s->out, (int)n);
+    if (!adj) hipLaunchKernelGGL(k_exp, dim3(ne), dim3(256), 0, st, s->sol, s->out, (int)n);
     HIPCHECK_CTX(ctx, hipGetLastError());
     HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev1, st));
     HIPCHECK_CTX(ctx, hipMemcpyAsync(rho_inout, adj ? s->sol : s->out, sizeof(double) * n, hipMemcpyDeviceToHost, st));

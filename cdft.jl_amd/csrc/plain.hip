@@ -454,7 +454,8 @@ __global__ void k_exp(const double *x, double *rho, int n)
 }
 
 // Exponent assembly (get_new_profile!, converge.jl:108-153) fused with the
-// Anderson book-keeping that follows each map evaluation.
+// Anderson book-keeping that follows each map evaluation.  Grid-stride: a thread
+// sums its points' partials in order, so a capped grid keeps the partials few.
 //  mode 0: out = dF_res/drho (no field, no clamp)                functional only
 //  mode 1: gx = betafix(lnG); res = gx - sol; partial |res|^2     first/Picard step
 //  mode 2: aa_point! (anderson.jl:470-481); partials |res|^2, |df|^2, df.res
@@ -476,10 +477,10 @@ __global__ __launch_bounds__(256) void k_update(const double *__restrict__ dphi,
         mode = ctl->umode;
         beta = ctl->beta_u;
     }
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
-    if (t < N * nc) {
-        int c = t / N, i = t - c * N;
+    const long nt = (long)N * nc, stride = (long)gridDim.x * blockDim.x;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += stride) {
+        int c = (int)(t / N), i = (int)(t - (long)c * N);
         double s = dphi[(size_t)(c * F) * N + i];
         for (int f = 1; f < F; f++) s += part[(size_t)(c * (F - 1) + f - 1) * N + i];
         if (mode == 0) {
@@ -499,7 +500,7 @@ __global__ __launch_bounds__(256) void k_update(const double *__restrict__ dphi,
                 gx[t] = g;
                 double r = g - x;
                 res[t] = r;
-                v[0] = r * r;
+                v[0] += r * r;
             } else {
                 double gold = gx[t], rold = res[t];
                 dg[t] = -gold + g;
@@ -508,9 +509,9 @@ __global__ __launch_bounds__(256) void k_update(const double *__restrict__ dphi,
                 res[t] = r;
                 double d = -rold + r;
                 df[t] = d;
-                v[0] = r * r;
-                v[1] = d * d;
-                v[2] = d * r;
+                v[0] += r * r;
+                v[1] += d * d;
+                v[2] += d * r;
             }
         }
     }
@@ -830,10 +831,10 @@ __global__ __launch_bounds__(256) void k_aa_proj1(int n, const double *__restric
     const int downdate = ctl->down, m = ctl->m, kq = ctl->kqq;
     const double *Q = ctl->qsel ? Q1 : Q0;
     double *Qout = ctl->qsel ? const_cast<double *>(Q0) : Q1;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[2 * CDFT_MAXM];
     for (int q = 0; q < 2 * CDFT_MAXM; q++) v[q] = 0.0;
-    if (i < n) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         double qv[CDFT_MAXM];
         if (downdate) {
             double qo[CDFT_MAXM];
@@ -850,8 +851,8 @@ __global__ __launch_bounds__(256) void k_aa_proj1(int n, const double *__restric
         }
         double dv = df[i], rv = res[i];
         for (int j = 0; j < kq; j++) {
-            v[j] = qv[j] * dv;
-            v[CDFT_MAXM + j] = qv[j] * rv;
+            v[j] += qv[j] * dv;
+            v[CDFT_MAXM + j] += qv[j] * rv;
         }
     }
     block_partials<2 * CDFT_MAXM>(v, red, nblk);
@@ -868,10 +869,10 @@ __global__ __launch_bounds__(256) void k_aa_proj2(int n, const double *__restric
     if (ctl->g_proj) return;
     const int kq = ctl->kqq;
     const double *Q = (ctl->qsel ^ ctl->down) ? Q1 : Q0;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[CDFT_MAXM];
     for (int q = 0; q < CDFT_MAXM; q++) v[q] = 0.0;
-    if (i < n) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         double qv[CDFT_MAXM];
         double a = 0.0;
         for (int j = 0; j < kq; j++) {
@@ -881,10 +882,10 @@ __global__ __launch_bounds__(256) void k_aa_proj2(int n, const double *__restric
         double d = df[i] - a;
         df[i] = d;
         if (!last) {
-            for (int j = 0; j < kq; j++) v[j] = qv[j] * d;
+            for (int j = 0; j < kq; j++) v[j] += qv[j] * d;
         } else {
-            v[0] = d * d;
-            v[1] = d * res[i];
+            v[0] += d * d;
+            v[1] += d * res[i];
         }
     }
     block_partials<CDFT_MAXM>(v, red, nblk);
