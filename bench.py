@@ -14,8 +14,9 @@ in HBM).  Prints one JSON line.  Besides the headline workload the line carries:
                methane / n-decane interface (2048 points)
                (each with its mixed-precision solve beside the FP64 one);
   matrix_3d    the 3D hard-sphere-matrix ternary on a 256^3 grid: mixed-precision and
-               FP64 solves, stream time per map evaluation, the algorithmic-byte rate
-               and the weighted-density + transposed passes against the HBM roofline;
+               FP64 solves, stream time per map evaluation, the algorithmic-byte rate,
+               the weighted-density + transposed passes against the HBM roofline, and
+               the adjoint's d(N_i)/d(eps_j) with one entry checked by full solves;
   matrix_2d    the same mixture with hard disks on a 512^2 grid, FP64 and mixed;
   radial       methane in spherical and cylindrical Steele pores (151 radial points);
   adjoint      surface-tension sensitivities of the methane / n-decane interface
@@ -242,6 +243,31 @@ def run_matrix_3d(np, DFTOptions, configs, converge_):
     wall64 = time.perf_counter() - t
     res64 = s64.last_result
     ads64 = adsorption_system(s64, rho64)
+    # the workload's sensitivity d(N_i)/d(eps_j) through the adjoint (three adjoint
+    # solves, one per component), and one entry against central differences of full
+    # FP64 solves (h = 1e-5 relative)
+    from cdft_amd import adjoint as adj
+    t = time.perf_counter()
+    jac, _, jres = adj.jacobian(s64, rho64.reshape(-1), "adsorption", [0, 1, 2], params=("epsilon",), tol=1e-10)
+    t_jac = time.perf_counter() - t
+    J = [[jac[(c, "epsilon", j)] for j in range(3)] for c in range(3)]
+    h = 1e-5 * adj.model_params(s64.model)[s64.model.components[0]][2]
+    fd_ads = []
+    t = time.perf_counter()
+    for sg in (+1, -1):
+        sp = adj.perturbed_system(s64, 0, "epsilon", sg * h)
+        rr = rho64.copy()
+        converge_(sp, rr, tol=1e-10, **configs.MATRIX_3D_SOLVER)
+        fd_ads.append(adsorption_system(sp, rr)[0])
+        del sp
+    t_fd = time.perf_counter() - t
+    fd00 = (fd_ads[0] - fd_ads[1]) / (2 * h)
+    adjoint = {"dN_deps_mol_m3_per_K": J, "adjoint_errcodes": [int(jres[c]["errcode"]) for c in range(3)],
+               "adjoint_nevals": [int(jres[c]["nevals"]) for c in range(3)], "seconds": t_jac,
+               "fd_dN0_deps0": fd00, "rel_diff_vs_fd": abs(J[0][0] - fd00) / abs(fd00),
+               "fd_two_full_solves_s": t_fd,
+               "note": "FP64 adjoint at the FP64 profile; explicit partials by central differences of one "
+                       "functional evaluation each; FD of full solves from the converged profile at tol 1e-10"}
     del s64
     s, rho = configs.matrix_3d(n=256, options=DFTOptions(precision="mixed"))
     s.device_system.tune()
@@ -278,8 +304,9 @@ def run_matrix_3d(np, DFTOptions, configs, converge_):
         "adsorption_max_rel_diff_mixed_vs_fp64": float(np.max(np.abs(ads - ads64) / np.abs(ads64))),
         "profile_max_rel_diff_mixed_vs_fp64": relmax(r.reshape(-1), rho64.reshape(-1)),
         "weighted_densities_and_transposed": wd,
+        "adjoint_dN_deps": adjoint,
         "wd_note": "one weighted-density + transposed-convolution sequence per map precision, timed by the "
-                   "autotuner with HIP events for its chosen variant; compulsory bytes per SURVEY 8(d): "
+                   "autotuner with HIP events for its chosen variant; compulsory bytes (each field read or written once): "
                    "B_WD + B_T = 2 N C (1+F) s",
     }
 

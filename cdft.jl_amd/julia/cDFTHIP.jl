@@ -35,6 +35,10 @@ struct CdftSystemDesc
     m::Ptr{Cdouble}; d::Ptr{Cdouble}; sigma::Ptr{Cdouble}; eps::Ptr{Cdouble}
     rho_bulk::Ptr{Cdouble}; mu_res::Ptr{Cdouble}; vext::Ptr{Cdouble}
     precision::Cint
+    assoc_nsite::Cint; assoc_npair::Cint
+    assoc_site_comp::Ptr{Cint}; assoc_site_count::Ptr{Cdouble}; assoc_pair_site::Ptr{Cint}
+    assoc_pair_eps::Ptr{Cdouble}; assoc_pair_kappa::Ptr{Cdouble}
+    coord::Cint                    # 0 Cartesian, 1 spherical, 2 cylindrical
 end
 
 struct CdftSolverOpts
@@ -66,6 +70,11 @@ _check(rc, ctx) = rc == 0 || error("cdft error $rc: ", unsafe_string(ccall((:cdf
 function _system(system, ctx)
     model = system.model
     st = system.structure
+    # association sites go through the descriptor's assoc_* fields, which this binding
+    # does not fill yet: refuse associating models rather than drop their term
+    if hasproperty(model.params, :bondvol) && !isempty(model.params.bondvol.values.values)
+        error("cDFTHIP: associating models are not passed through this binding yet")
+    end
     nd = cDFT.dimension(st)
     (p, T) = st.conditions
     ngrid = ntuple(i -> i <= nd ? Cint(st.ngrid[i]) : Cint(1), 3)
@@ -83,10 +92,13 @@ function _system(system, ctx)
         vext = sum(c[end] for c in cache)          # beta V_ext (steele.jl:49-81)
     end
     prec = cDFT.fptype(system.options) == Float64 ? Cint(0) : Cint(1)
+    coord = st isa cDFT.DFTStructByCoord{cDFT.Spherical} ? Cint(1) :
+            st isa cDFT.DFTStructByCoord{cDFT.Cylindrical} ? Cint(2) : Cint(0)
     GC.@preserve m d sigma eps rhob mu vext begin
         desc = CdftSystemDesc(length(model), nd, ngrid, lb, ub, T, p, cDFT.length_scale(model), 1.3862,
                               pointer(m), pointer(d), pointer(sigma), pointer(eps), pointer(rhob), pointer(mu),
-                              vext === nothing ? Ptr{Cdouble}(C_NULL) : pointer(vext), prec)
+                              vext === nothing ? Ptr{Cdouble}(C_NULL) : pointer(vext), prec,
+                              0, 0, C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, coord)
         h = Ref{Ptr{Cvoid}}(C_NULL)
         _check(ccall((:cdft_system_create, libcdft), Cint, (Ptr{Cvoid}, Ref{CdftSystemDesc}, Ref{Ptr{Cvoid}}), ctx, desc, h), ctx)
         return h[]
