@@ -12,6 +12,7 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBPATH = os.path.join(_PKG, "libcdft_hip.so")
 
 CDFT_OK = 0
+CDFT_ERR_STOPPED = -5      # a progress callback stopped the solve
 CDFT_FP64 = 0
 CDFT_MIXED = 1
 CDFT_CARTESIAN, CDFT_SPHERICAL, CDFT_CYLINDRICAL = 0, 1, 2
@@ -46,10 +47,15 @@ class SystemDesc(C.Structure):
     ]
 
 
+# int (*)(void *user, int iter, const double *rho, long n)
+ProgressFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, _dp, C.c_long)
+
+
 class SolverOpts(C.Structure):
     _fields_ = [("maxit", C.c_int), ("beta", C.c_double), ("tol", C.c_double),
                 ("anderson_start", C.c_double), ("m", C.c_int), ("hist_cap", C.c_int),
-                ("hist", _dp)]
+                ("hist", _dp), ("progress", ProgressFn), ("progress_user", C.c_void_p),
+                ("progress_every", C.c_int)]
 
 
 class Result(C.Structure):
@@ -253,15 +259,34 @@ class DeviceSystem:
         return out
 
     def converge(self, rho_inout, *, maxit=10000, beta=1e-2, tol=1e-4, anderson_start=1e-1,
-                 m=5, hist_cap=0):
+                 m=5, hist_cap=0, progress=None, progress_every=0):
+        """cdft_converge.  progress(iter, rho) (rho: a float64 copy of the profile the
+        iter-th map evaluation used) is called after every progress_every-th evaluation;
+        an exception it raises stops the solve and propagates."""
         assert rho_inout.dtype == np.float64 and rho_inout.flags.c_contiguous
         assert rho_inout.size == self.N * self.nc
         hist = np.zeros(max(hist_cap, 1))
         o = SolverOpts(int(maxit), float(beta), float(tol), float(anderson_start), int(m),
                        int(hist_cap), ptr(hist) if hist_cap else None)
+        raised = []
+        if progress is not None:
+            if int(progress_every) < 1:
+                raise ValueError("progress_every must be >= 1")
+
+            def hook(_user, it, rp, n):
+                try:
+                    progress(int(it), np.ctypeslib.as_array(rp, shape=(int(n),)).copy())
+                    return 0
+                except BaseException as e:          # carried over the C frames, re-raised below
+                    raised.append(e)
+                    return 1
+            o.progress = ProgressFn(hook)           # referenced by o for the whole call
+            o.progress_every = int(progress_every)
         r = Result()
-        check(load().cdft_converge(self.handle, ptr(rho_inout), C.byref(o), C.byref(r)),
-              self.ctx.handle)
+        rc = load().cdft_converge(self.handle, ptr(rho_inout), C.byref(o), C.byref(r))
+        if raised:
+            raise raised[0]
+        check(rc, self.ctx.handle)
         out = r.as_dict()
         if hist_cap:
             out["history"] = hist[:min(r.nhist, hist_cap)].copy()

@@ -175,12 +175,25 @@ def converge_(system, rho, maxit=10000, beta=1e-2, tol=1e-4, anderson_start=1e-1
         raise ValueError("log_interval and save_interval must be >= 0")
     if save_interval > 0 and save_callback is None:
         raise ValueError("save_interval > 0 needs a save_callback")
-    if save_interval > 0 or log_interval > 0:
-        raise NotImplementedError("per-iteration callbacks are not available: the iteration runs on the device")
     if not (isinstance(rho, np.ndarray) and rho.dtype == np.float64 and rho.flags.c_contiguous):
         raise TypeError("rho must be a C-contiguous float64 numpy array of shape (nc, N)")
+    progress, every = None, 0
+    if log_interval > 0 or save_interval > 0:
+        # GFix_logger (converge.jl:256-267): after evaluation it, the free energy of the
+        # profile that evaluation used every log_interval, save_callback(it, rho) every
+        # save_interval; the device solve pauses at multiples of their gcd
+        every = math.gcd(log_interval, save_interval)
+        shape = rho.shape
+
+        def progress(it, r):
+            r = r.reshape(shape)
+            if log_interval > 0 and it % log_interval == 0:
+                print(f"DFT iter {it:5d} | F = {free_energy(system, r):.6g}")
+            if save_interval > 0 and it % save_interval == 0:
+                save_callback(it, r)
     res = system.device_system.converge(rho.reshape(-1), maxit=maxit, beta=beta, tol=tol,
-                                        anderson_start=anderson_start, m=anderson_m, hist_cap=history)
+                                        anderson_start=anderson_start, m=anderson_m, hist_cap=history,
+                                        progress=progress, progress_every=every)
     system.last_result = res
     if verbose:
         msg = "DFT converged" if res["idid"] else "DFT did not converge"

@@ -1056,6 +1056,9 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     if (m < 0 || m > CDFT_MAXM - 1) return fail(ctx, CDFT_ERR_ARG, "anderson m must be in 0..7");
     if (!(std::fabs(beta) > 0 && std::fabs(beta) <= 1)) return fail(ctx, CDFT_ERR_ARG, "abs(beta) must be in (0,1]");
     if (o->maxit < 0) return fail(ctx, CDFT_ERR_ARG, "maxit must be >= 0");
+    // the progress hook belongs to converge! (the adjoint's linear solve has none)
+    const bool prog = (!adj && o->progress);
+    if (prog && o->progress_every < 1) return fail(ctx, CDFT_ERR_ARG, "progress_every must be >= 1");
     hipStream_t st = ctx->stream;
     const long n = (long)s->N * s->nc;
     const int nb = s->nblk;                      // reducing kernels (grid-stride)
@@ -1095,6 +1098,7 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     hc->g_f64 = hc->fp32;
     hc->g_f32 = !hc->fp32;
     hc->evfp32 = hc->fp32;
+    hc->pause_every = prog ? o->progress_every : 0;
     for (int j = 0; j < CDFT_MAXM; j++) hc->ring[j] = j;
     HIPCHECK_CTX(ctx, hipMemcpyAsync(s->ctl, hc, sizeof(AACtl), hipMemcpyHostToDevice, st));
 
@@ -1150,15 +1154,31 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     // maxit + picard_maxit, plus the first ones at the seed and at the switch
     const long bound = 4L * (2L * o->maxit + 4) + 2L * K;
 
+    std::vector<double> prho(prog ? n : 0);       // the paused evaluation's profile
     HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev0, st));
     body();                                       // eager first body
     HIPCHECK_CTX(ctx, hipGetLastError());
-    long bodies = 1;
+    long bodies = 1, pauses = 0;
     while (true) {
         HIPCHECK_CTX(ctx, hipMemcpyAsync(hc, c, sizeof(AACtl), hipMemcpyDeviceToHost, st));
         HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+        if (hc->paused) {
+            // the evaluation just finished used rho = exp(x) in s->rho, and nothing
+            // has run since (every gate closed at the pause)
+            HIPCHECK_CTX(ctx, hipMemcpyAsync(prho.data(), s->rho, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+            HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+            if (o->progress(o->progress_user, hc->nevals, prho.data(), n) != 0) {
+                std::memcpy(rho_inout, prho.data(), sizeof(double) * n);
+                return fail(ctx, CDFT_ERR_STOPPED, "cdft_converge: stopped by the progress callback");
+            }
+            hipLaunchKernelGGL(k_aa_resume, dim3(1), dim3(1), 0, st, c);
+            HIPCHECK_CTX(ctx, hipGetLastError());
+            pauses++;
+            continue;
+        }
         if (hc->phase == AA_DONE) break;
-        if (bodies > bound) return fail(ctx, CDFT_ERR_HIP, "aasol: the device-side solver did not terminate");
+        // a pause leaves at most one chunk's bodies unused
+        if (bodies > bound + pauses * K) return fail(ctx, CDFT_ERR_HIP, "aasol: the device-side solver did not terminate");
         if (use_graph) {
             bool same = s->aa_exec != nullptr;
             for (int j = 0; j < 5 && same; j++) same = (s->aa_key[j] == key[j]);

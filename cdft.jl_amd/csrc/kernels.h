@@ -33,6 +33,7 @@ struct AACtl {
     int g_proj;        // no Gram-Schmidt projections in this body
     int g_f64;         // this body's map is not the FP64 one
     int g_f32;         // this body's map is not the FP32 one
+    int paused;        // stopped after an evaluation for the host's progress callback
     // work of the body in flight
     int umode;         // k_update mode (1 first/Picard, 2 Anderson)
     int pmode;         // point kernel: 0 none, 1 sol = gx, 2 sol = xsafe, 3 Anderson point
@@ -48,6 +49,7 @@ struct AACtl {
     int phase, k, ka, it, nh, nevals, nevals32;
     int fp32, fp32_failed, toosoon, since_best, restore, errcode, idid;
     int m, maxit, picard_maxit, hist_cap, adjoint;
+    int pause_every;   // pause after every this many evaluations (0: never)
     double r, dfn2, dfres, tol, up_bd, picard_tol, best;
     double rtol, atol, picard_rtol, picard_atol, beta, picard_beta;
     double R[CDFT_MAXM * CDFT_MAXM];

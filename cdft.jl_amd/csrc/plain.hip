@@ -757,6 +757,18 @@ __device__ inline void aa_pre(AACtl &c)
     c.evfp32 = !done && c.fp32;
 }
 
+// Pause after an evaluation (converge!'s per-evaluation logger, converge.jl:256-267):
+// every gate closes, so the bodies still queued do nothing, and the work of the next
+// body (aa_pre) waits for k_aa_resume.  The state aa_post left is untouched.
+__device__ inline void aa_pause(AACtl &c)
+{
+    c.paused = 1;
+    c.g_done = 1;
+    c.g_proj = 1;
+    c.g_f64 = 1;
+    c.g_f32 = 1;
+}
+
 // After the projections (or in aa_pre when there were none): the new R column
 // and Q^T res, then the triangular solve theta = R \ (Q^T res).
 __device__ inline void aa_mid(AACtl &c, const double *fin)
@@ -818,8 +830,17 @@ __global__ __launch_bounds__(256) void k_aa_fin(const double *part, int nblk, do
     if (STEP == 1) aa_mid(*ctl, fin);
     else {
         aa_post(*ctl, fin, hist);
-        aa_pre(*ctl);
+        if (ctl->pause_every > 0 && ctl->nevals % ctl->pause_every == 0) aa_pause(*ctl);
+        else aa_pre(*ctl);
     }
+}
+
+// Resumes a paused solve: the deferred aa_pre of the next body.  One thread.
+__global__ void k_aa_resume(AACtl *ctl)
+{
+    if (!ctl->paused) return;
+    ctl->paused = 0;
+    aa_pre(*ctl);
 }
 
 // pass 1: optional downdate Q' = Q Qx, then h1 = Q'^T df and t = Q'^T res.
@@ -903,7 +924,7 @@ __global__ __launch_bounds__(256) void k_aa_point(int n, const AACtl *__restrict
                                                   const double *__restrict__ mask)
 {
     const int pm = ctl->pmode, keep = ctl->keep;
-    if (pm == 0 && !keep) return;
+    if (ctl->paused || (pm == 0 && !keep)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (keep) xsafe[i] = sol[i];

@@ -44,7 +44,10 @@ end
 struct CdftSolverOpts
     maxit::Cint; beta::Cdouble; tol::Cdouble; anderson_start::Cdouble
     m::Cint; hist_cap::Cint; hist::Ptr{Cdouble}
+    progress::Ptr{Cvoid}; progress_user::Ptr{Cvoid}; progress_every::Cint
 end
+CdftSolverOpts(maxit, beta, tol, anderson_start, m) =
+    CdftSolverOpts(maxit, beta, tol, anderson_start, m, 0, C_NULL, C_NULL, C_NULL, 0)
 
 struct CdftResult
     idid::Cint; errcode::Cint; iters::Cint; nevals::Cint; nhist::Cint
@@ -107,18 +110,68 @@ end
 
 const HIPSystem = cDFT.DFTSystem{<:Clapeyron.PCSAFTModel,<:Any,<:Any,<:Any,<:Any,<:Any,<:cDFT.DFTOptions{HIPNative}}
 
+# F_ideal on the host (cDFT's own, BasicIdeal.jl:10-34) + F_res from the library
+function _free_energy(system, sys, ctx, r)
+    fres = Ref{Cdouble}(0.0)
+    out = similar(r, Float64)
+    _check(ccall((:cdft_functional_eval, libcdft), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Ref{Cdouble}, Ptr{Cdouble}),
+                 sys, r, out, fres, C_NULL), ctx)
+    return cDFT.F_ideal(system, r) + fres[]
+end
+
+# converge!'s GFix_logger (src/methods/converge.jl:256-267), called by the library
+# after every gcd(log_interval, save_interval)-th map evaluation with the profile
+# that evaluation used
+mutable struct _Progress
+    prob::Any; sys::Ptr{Cvoid}; ctx::Ptr{Cvoid}; dims::Tuple; err::Any
+end
+
+function _progress(user::Ptr{Cvoid}, it::Cint, rho::Ptr{Cdouble}, n::Clong)::Cint
+    st = unsafe_pointer_to_objref(user)::_Progress
+    try
+        r = reshape(copy(unsafe_wrap(Array, rho, Int(n))), st.dims)
+        prob = st.prob
+        if prob.log_interval > 0 && it % prob.log_interval == 0
+            H = _free_energy(prob.system, st.sys, st.ctx, r)
+            @info "DFT iter $(lpad(it, 5)) | F = $(round(H; sigdigits=6))"
+        end
+        if prob.save_interval > 0 && it % prob.save_interval == 0 && prob.save_callback !== nothing
+            prob.save_callback(Int(it), r)
+        end
+        return Cint(0)
+    catch e
+        st.err = e                     # re-thrown once the library has returned
+        return Cint(1)
+    end
+end
+
 function cDFT.converge!(prob::cDFT.DFTProblem{<:HIPSystem}, method::cDFT.AASol, rho::AbstractArray)
     system = prob.system
     ctx = _ctx(system.options.device.device)
     sys = _system(system, ctx)
     try
-        opts = CdftSolverOpts(method.maxit, method.beta, method.rtol, method.picard_rtol, method.m, 0, C_NULL)
+        every = prob.log_interval > 0 && prob.save_interval > 0 ? gcd(prob.log_interval, prob.save_interval) :
+                max(prob.log_interval, prob.save_interval)
+        st = _Progress(prob, sys, ctx, size(rho), nothing)
+        cb = @cfunction(_progress, Cint, (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Clong))
         res = Ref{CdftResult}()
         buf = Array{Float64}(rho)
-        _check(ccall((:cdft_converge, libcdft), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ref{CdftSolverOpts}, Ref{CdftResult}),
-                     sys, buf, opts, res), ctx)
+        GC.@preserve st begin
+            opts = CdftSolverOpts(method.maxit, method.beta, method.rtol, method.picard_rtol, method.m, 0, C_NULL,
+                                  every > 0 ? cb : C_NULL, every > 0 ? pointer_from_objref(st) : C_NULL, every)
+            rc = ccall((:cdft_converge, libcdft), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ref{CdftSolverOpts}, Ref{CdftResult}),
+                       sys, buf, opts, res)
+        end
+        st.err === nothing || throw(st.err)
+        _check(rc, ctx)
         copyto!(rho, buf)
         LAST[] = res[]
+        if method.verbose                 # converge.jl:247-252
+            r = res[]
+            msg = r.idid == 1 ? "DFT converged" : "DFT did not converge"
+            H = _free_energy(system, sys, ctx, buf)
+            @info "$msg after $(r.nevals) iterations: err = $(round(r.resnorm; sigdigits=3)) | F = $(round(H; sigdigits=6))"
+        end
     finally
         ccall((:cdft_system_destroy, libcdft), Cint, (Ptr{Cvoid},), sys)
     end
@@ -134,7 +187,7 @@ function converge_batch!(systems::AbstractVector, rhos::AbstractVector; maxit = 
     hs = [_system(s, ctx) for s in systems]
     bufs = [Array{Float64}(r) for r in rhos]
     res = Vector{CdftResult}(undef, length(systems))
-    opts = CdftSolverOpts(maxit, beta, tol, anderson_start, anderson_m, 0, C_NULL)
+    opts = CdftSolverOpts(maxit, beta, tol, anderson_start, anderson_m)
     try
         GC.@preserve bufs begin
             ptrs = [pointer(b) for b in bufs]
@@ -170,7 +223,7 @@ function adjoint_solve(system, rho::AbstractArray, g::AbstractArray; maxit = 10_
     sys = _system(system, ctx)
     lam = zeros(Float64, length(rho))
     res = Ref{CdftResult}()
-    opts = CdftSolverOpts(maxit, beta, tol, anderson_start, anderson_m, 0, C_NULL)
+    opts = CdftSolverOpts(maxit, beta, tol, anderson_start, anderson_m)
     try
         _check(ccall((:cdft_adjoint_solve, libcdft), Cint,
                      (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ref{CdftSolverOpts}, Ref{CdftResult}),

@@ -36,6 +36,7 @@ extern "C" {
 #define CDFT_ERR_HIP (-2)
 #define CDFT_ERR_UNSUPPORTED (-3)
 #define CDFT_ERR_NOMEM (-4)
+#define CDFT_ERR_STOPPED (-5)   /* a progress callback asked the solve to stop */
 
 /* precision modes (DFTOptions(...; precision), src/base/devices.jl:29-40) */
 #define CDFT_FP64 0   /* everything in FP64 (the reference's default)                */
@@ -90,6 +91,19 @@ typedef struct {
 #define CDFT_CYLINDRICAL 2
 #define CDFT_RADIAL_MAXN 2048
 
+/* Per-evaluation progress hook: converge!'s logger GFix_logger
+ * (src/methods/converge.jl:256-267), which the reference calls after every map
+ * evaluation with the evaluation count and the profile rho = exp(x) that evaluation
+ * used, to log the free energy every log_interval evaluations and to call
+ * save_callback(iter, rho) every save_interval.  Here the solve pauses on the device
+ * after every `every`-th evaluation and calls progress(user, iter, rho, n) on the
+ * host with that profile (n = N nc doubles, layout as rho_inout, valid during the
+ * call).  The callback may evaluate the functional of the same system
+ * (cdft_functional_eval); it must not start another solve of it.  A nonzero return
+ * stops the solve: cdft_converge then returns CDFT_ERR_STOPPED with that profile in
+ * rho_inout. */
+typedef int (*cdft_progress_fn)(void *user, int iter, const double *rho, long n);
+
 /* converge! keywords (src/methods/converge.jl:3-13, :51-60). */
 typedef struct {
     int maxit;             /* 10000                                               */
@@ -99,6 +113,9 @@ typedef struct {
     int m;                 /* 5 Anderson depth (0 = damped Picard)                */
     int hist_cap;          /* capacity of hist (0: no history)                    */
     double *hist;          /* residual-norm history out (host), or NULL           */
+    cdft_progress_fn progress; /* or NULL (the batched path ignores it)           */
+    void *progress_user;
+    int progress_every;    /* >= 1 when progress is set                            */
 } cdft_solver_opts;
 
 /* aasol's outcome (AndersonOK, src/utils/anderson.jl:727-737). */
@@ -148,8 +165,9 @@ int cdft_functional_eval_fp32(cdft_sys *sys, const double *rho, double *dfdrho);
 /* Euler-Lagrange fixed point: rho is overwritten with the converged profile.
  * Plain path: one system, per-operator kernels; the whole Anderson iteration,
  * its control flow included, runs on the device from a captured hipGraph and the
- * host reads the solver state once per 16 map evaluations.  Replaces converge!(prob, ::AASol, rho)
- * (src/methods/converge.jl:220-254) and aasol (src/utils/anderson.jl:301-443). */
+ * host reads the solver state once per 16 map evaluations (and at each pause for
+ * opts->progress).  Replaces converge!(prob, ::AASol, rho) (src/methods/converge.jl:220-267)
+ * and aasol (src/utils/anderson.jl:301-443). */
 int cdft_converge(cdft_sys *sys, double *rho_inout, const cdft_solver_opts *opts, cdft_result *result);
 
 /* Batched path: nsys independent systems (any mix of 1D grid sizes), one
@@ -157,7 +175,7 @@ int cdft_converge(cdft_sys *sys, double *rho_inout, const cdft_solver_opts *opts
  * on the device; systems retire independently.  Each system follows exactly the
  * same algorithm as cdft_converge.  (No batched counterpart in the reference: it
  * replaces a loop of converge! calls, e.g. over adsorption(...) state points,
- * src/methods/adsorption.jl:48-66.)  hist in opts is ignored.  One-component
+ * src/methods/adsorption.jl:48-66.)  hist and progress in opts are ignored.  One-component
  * CDFT_MIXED systems of at most 512 points with anderson m <= 5 only for now: other
  * batches return CDFT_ERR_UNSUPPORTED (use cdft_converge per system). */
 int cdft_converge_batch(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_inout,
