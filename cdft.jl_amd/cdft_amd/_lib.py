@@ -29,6 +29,7 @@ EXPORTS = [
     "cdft_converge_batch", "cdft_converge_batch_device", "cdft_converge_batch_device_noalloc",
     "cdft_last_kernel_ms",
     "cdft_surface_tension", "cdft_adsorption", "cdft_phi_vjp", "cdft_hvp", "cdft_adjoint_solve",
+    "cdft_adjoint_params",
     "cdft_ddft_rhs", "cdft_system_tune", "cdft_system_get_tuning", "cdft_system_set_tuning",
 ]
 
@@ -128,6 +129,8 @@ def load():
     L.cdft_phi_vjp.argtypes = [vp, _dp, _dp, _dp]
     L.cdft_hvp.argtypes = [vp, _dp, _dp, _dp]
     L.cdft_adjoint_solve.argtypes = [vp, _dp, _dp, _dp, C.POINTER(SolverOpts), C.POINTER(Result)]
+    L.cdft_adjoint_params.argtypes = [vp, _dp, C.c_int, _dp, _dp, C.c_int, C.POINTER(vp), C.POINTER(vp), _dp,
+                                      C.POINTER(SolverOpts), _dp, C.POINTER(Result)]
     L.cdft_ddft_rhs.argtypes = [vp, _dp, _dp]
     _lib = L
     return L
@@ -336,6 +339,34 @@ class DeviceSystem:
         check(load().cdft_adjoint_solve(self.handle, ptr(rho), ptr(g), ptr(lam), C.byref(o), C.byref(r)),
               self.ctx.handle)
         return lam, r.as_dict()
+
+    def adjoint_params(self, rho, lam, g=None, pairs=(), h=(), *, maxit=10000, beta=1e-2, tol=1e-10,
+                       anderson_start=1e-1, m=5):
+        """cdft_adjoint_params for nobj = len(lam) objectives: with g (nobj rows) the
+        multipliers are solved first, in place in lam (nobj x n float64); then for each
+        (plus, minus) device-system pair and step h[p] the implicit sensitivity parts,
+        shape (nobj, nparam).  Returns (out, results or None)."""
+        rho = self._vec(rho)
+        n = self.N * self.nc
+        assert isinstance(lam, np.ndarray) and lam.dtype == np.float64 and lam.flags.c_contiguous
+        nobj = lam.size // n
+        assert nobj >= 1 and lam.size == nobj * n
+        gp = None
+        if g is not None:
+            g = np.ascontiguousarray(g, dtype=np.float64).reshape(-1)
+            assert g.size == nobj * n
+            gp = ptr(g)
+        npar = len(pairs)
+        hs = np.ascontiguousarray(h, dtype=np.float64).reshape(-1)
+        assert hs.size == npar
+        plus = (C.c_void_p * max(npar, 1))(*[a.handle.value for a, _ in pairs])
+        minus = (C.c_void_p * max(npar, 1))(*[b.handle.value for _, b in pairs])
+        out = np.zeros((nobj, max(npar, 1)))
+        res = (Result * nobj)()
+        o = SolverOpts(int(maxit), float(beta), float(tol), float(anderson_start), int(m), 0, None)
+        check(load().cdft_adjoint_params(self.handle, ptr(rho), nobj, gp, ptr(lam), npar, plus, minus,
+                                         ptr(hs) if npar else None, C.byref(o), ptr(out), res), self.ctx.handle)
+        return out[:, :npar], ([r.as_dict() for r in res] if g is not None else None)
 
 
 def converge_batch(ctx, systems, rhos, *, maxit=10000, beta=1e-2, tol=1e-4, anderson_start=1e-1,

@@ -7,8 +7,9 @@ profile rho* = exp(x*), x* = G(x*, theta) the undamped Euler-Lagrange map
 
     dJ/dtheta = dJ/dtheta|_rho + lambda . dG/dtheta|_x,   (I - dG/dx)^T lambda = rho * dJ/drho,
 
-with lambda from cdft_adjoint_solve (Hessian-vector products of F_res on the device,
-the same damped Picard / Anderson iteration as converge!).  The two explicit
+with lambda and the products lambda . dG/dtheta from cdft_adjoint_params (Hessian-vector
+products of F_res on the device, the same damped Picard / Anderson iteration as
+converge!).  The two explicit
 partials at fixed profile are central differences of one functional evaluation
 each, on systems rebuilt with the perturbed parameter over the same grid: two
 evaluations per parameter instead of two full solves.
@@ -139,13 +140,19 @@ def jacobian(system, rho, kind, comps, params=PARAMS, rel_h=1e-6, tol=1e-10, max
     if deflate is None:
         deflate = system.vext is None
     t = translation_mode(system, rho) if deflate else None
-    lams, results = {}, {}
-    for c in comps:
-        g = rho * objective_grad_rho(system, rho, kind, c)
+    dev = system.device_system
+    comps = list(comps)
+    g = np.empty((len(comps), rho.size))
+    for k, c in enumerate(comps):
+        gk = rho * objective_grad_rho(system, rho, kind, c)
         if deflate:
-            g = g - (g @ t) / (t @ t) * t
-        lams[c], results[c] = system.device_system.adjoint_solve(rho, g, maxit=maxit, beta=beta, tol=tol,
-                                                                 anderson_start=anderson_start, m=m)
+            gk = gk - (gk @ t) / (t @ t) * t
+        g[k] = gk
+    # the multipliers of every objective (cdft_adjoint_params, nparam = 0)
+    lam = np.zeros_like(g)
+    _, res = dev.adjoint_params(rho, lam, g=g, maxit=maxit, beta=beta, tol=tol, anderson_start=anderson_start, m=m)
+    lams = {c: lam[k] for k, c in enumerate(comps)}
+    results = {c: res[k] for k, c in enumerate(comps)}
     out = {}
     rows = model_params(system.model)
     for i, name in enumerate(system.model.components):
@@ -154,10 +161,11 @@ def jacobian(system, rho, kind, comps, params=PARAMS, rel_h=1e-6, tol=1e-10, max
             h = rel_h * abs(base)
             sp = perturbed_system(system, i, prm, +h)
             sm = perturbed_system(system, i, prm, -h)
-            dG = (log_map(sp, rho) - log_map(sm, rho)) / (2 * h)
-            for c in comps:
+            # lambda_k . dG/dtheta on the device, for all objectives at once
+            imp, _ = dev.adjoint_params(rho, lam, pairs=[(sp.device_system, sm.device_system)], h=[h])
+            for k, c in enumerate(comps):
                 dJ = (objective(sp, rho, kind, c) - objective(sm, rho, kind, c)) / (2 * h)
-                out[(c, prm, i)] = dJ + float(lams[c] @ dG)
+                out[(c, prm, i)] = dJ + float(imp[k, 0])
             del sp, sm                # release their device systems before the next pair
     return out, lams, results
 

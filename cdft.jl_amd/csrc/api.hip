@@ -1495,6 +1495,68 @@ int cdft_adjoint_solve(cdft_sys *s, const double *rho, const double *g, double *
     return aasol_run(s, lambda_inout, o, r, &am);     // s->n keeps W rho* for every product
 }
 
+int cdft_adjoint_params(cdft_sys *s, const double *rho, int nobj, const double *g, double *lambda_inout,
+                        int nparam, cdft_sys *const *sys_plus, cdft_sys *const *sys_minus, const double *h,
+                        const cdft_solver_opts *o, double *out, cdft_result *results)
+{
+    cdft_ctx *ctx = s ? s->ctx : nullptr;
+    if (!s || !rho || !lambda_inout || nobj < 1 || nparam < 0 || (g && (!o || !results)) ||
+        (nparam > 0 && (!sys_plus || !sys_minus || !h || !out)))
+        return fail(ctx, CDFT_ERR_ARG, "cdft_adjoint_params: NULL or invalid argument");
+    const long n = (long)s->N * s->nc;
+    for (int p = 0; p < nparam; p++) {
+        const cdft_sys *q[2] = {sys_plus[p], sys_minus[p]};
+        for (const cdft_sys *x : q) {
+            if (!x || x == s) return fail(ctx, CDFT_ERR_ARG, "cdft_adjoint_params: a perturbed system is NULL or sys");
+            if (x->ctx != ctx || x->nc != s->nc || x->N != s->N || x->nd != s->nd || x->coord != s->coord)
+                return fail(ctx, CDFT_ERR_ARG, "cdft_adjoint_params: perturbed systems must share the context and grid");
+        }
+        if (!(h[p] != 0.0) || !std::isfinite(h[p])) return fail(ctx, CDFT_ERR_ARG, "cdft_adjoint_params: h must be finite and nonzero");
+    }
+    // lambda_k for every objective (the same solve as cdft_adjoint_solve)
+    if (g)
+        for (int k = 0; k < nobj; k++) {
+            const int rc = cdft_adjoint_solve(s, rho, g + (size_t)k * n, lambda_inout + (size_t)k * n, o, results + k);
+            if (rc != CDFT_OK) return rc;
+        }
+    if (nparam == 0) return CDFT_OK;
+    hipStream_t st = ctx->stream;
+    HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
+    const int nb = std::min(nblocks(n), 2048);
+    DevBufs tmp;
+    double *lam = nullptr, *part = nullptr;
+    HIPCHECK_CTX(ctx, tmp.get(&lam, (size_t)nobj * n));
+    HIPCHECK_CTX(ctx, tmp.get(&part, (size_t)nobj * nb));
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(lam, lambda_inout, sizeof(double) * nobj * n, hipMemcpyHostToDevice, st));
+    std::vector<double> hp((size_t)nobj * nb);
+    for (int p = 0; p < nparam; p++) {
+        // dF_res/drho of both perturbed systems at the common profile
+        cdft_sys *q[2] = {sys_plus[p], sys_minus[p]};
+        for (cdft_sys *x : q) {
+            HIPCHECK_CTX(ctx, hipMemcpyAsync(x->rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, st));
+            launch_functional(x);
+            hipLaunchKernelGGL(k_update, dim3(nblocks(n)), dim3(256), 0, st, upd_src(x).dphi, upd_src(x).part,
+                               (const double *)nullptr, x->N, x->nc, upd_src(x).F, (const double *)nullptr, x->out,
+                               (const double *)nullptr, (double *)nullptr, (double *)nullptr, (double *)nullptr,
+                               (double *)nullptr, 1.0, 0, (double *)nullptr, 0, (const double *)nullptr,
+                               (const double *)nullptr, (const AACtl *)nullptr);
+        }
+        hipLaunchKernelGGL(k_adj_param, dim3(nb), dim3(256), 0, st, (const double *)q[0]->out,
+                           (const double *)q[0]->vext, (const double *)q[0]->cpd, (const double *)q[1]->out,
+                           (const double *)q[1]->vext, (const double *)q[1]->cpd, (const double *)lam, s->N, s->nc,
+                           nobj, part, nb);
+        HIPCHECK_CTX(ctx, hipGetLastError());
+        HIPCHECK_CTX(ctx, hipMemcpyAsync(hp.data(), part, sizeof(double) * nobj * nb, hipMemcpyDeviceToHost, st));
+        HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+        for (int k = 0; k < nobj; k++) {
+            double acc = 0.0;                          // fixed order: reproducible
+            for (int b = 0; b < nb; b++) acc += hp[(size_t)k * nb + b];
+            out[(size_t)k * nparam + p] = acc / (2.0 * h[p]);
+        }
+    }
+    return CDFT_OK;
+}
+
 int cdft_ddft_rhs(cdft_sys *s, const double *eta, double *deta)
 {
     if (!s || !eta || !deta) return fail(s ? s->ctx : nullptr, CDFT_ERR_ARG, "cdft_ddft_rhs: NULL argument");

@@ -421,6 +421,32 @@ __device__ __forceinline__ void block_partials(double (&v)[NQ], double *part, in
     }
 }
 
+// Block partials of lambda_k . (G+ - G-) for k < nobj: G = clamp(cp_c - dF/drho - V,
+// -100, 100), the undamped map (get_new_profile!, converge.jl:108-153) of two systems
+// at one profile, from their dF_res/drho (dp, dm), fields and chemical potentials.
+// part[k nblk + block]; grid-stride over the n = N nc points.
+__global__ __launch_bounds__(256) void k_adj_param(const double *__restrict__ dp, const double *__restrict__ vp,
+                                                   const double *__restrict__ cpp, const double *__restrict__ dm,
+                                                   const double *__restrict__ vm, const double *__restrict__ cpm,
+                                                   const double *__restrict__ lam, int N, int nc, int nobj,
+                                                   double *part, int nblk)
+{
+    const long n = (long)N * nc, stride = (long)gridDim.x * blockDim.x;
+    for (int k = 0; k < nobj; k++) {
+        double v[1] = {0.0};
+        for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+            const int c = (int)(t / N);
+            double sp = dp[t], sm = dm[t];
+            if (vp) sp += vp[t];
+            if (vm) sm += vm[t];
+            const double gp = clamp_lng(cpp[c] - sp), gm = clamp_lng(cpm[c] - sm);
+            v[0] += lam[(size_t)k * n + t] * (gp - gm);
+        }
+        block_partials<1>(v, part + (size_t)k * nblk, nblk);
+        __syncthreads();                       // the partials' LDS slots are reused for k + 1
+    }
+}
+
 // sol = log(rho_in); rho = exp(sol)   (converge.jl:244 and GFix! :236)
 __global__ void k_init_x(const double *rho_in, double *sol, double *rho, int n)
 {

@@ -235,6 +235,35 @@ function adjoint_solve(system, rho::AbstractArray, g::AbstractArray; maxit = 10_
     return reshape(lam, size(rho))
 end
 
-export HIPNative, converge_batch!, phi_vjp, adjoint_solve
+"""Parameter sensitivities through the adjoint (cdft_adjoint_params): for objectives with
+gradients g[:, k] = rho .* dJ_k/drho (one column each) at the converged rho of `system`,
+and pairs (plus, minus) of the same system rebuilt with theta_p +- h[p], returns the
+multipliers (one column per objective) and the implicit parts lambda_k . dG/dtheta_p as a
+(objective, parameter) matrix; add the explicit partials dJ_k/dtheta_p at fixed rho."""
+function adjoint_params(system, rho::AbstractArray, g::AbstractMatrix, pairs::AbstractVector, h::AbstractVector;
+                        maxit = 10_000, beta = 1e-2, tol = 1e-10, anderson_start = 1e-1, anderson_m = 5)
+    ctx = _ctx(system.options.device.device)
+    sys = _system(system, ctx)
+    hp = [_system(p, ctx) for (p, _) in pairs]
+    hm = [_system(m, ctx) for (_, m) in pairs]
+    nobj = size(g, 2)
+    lam = zeros(Float64, length(rho), nobj)
+    out = zeros(Float64, max(length(pairs), 1), nobj)           # out[p, k] = C's out[k nparam + p]
+    res = Vector{CdftResult}(undef, nobj)
+    opts = CdftSolverOpts(maxit, beta, tol, anderson_start, anderson_m)
+    try
+        _check(ccall((:cdft_adjoint_params, libcdft), Cint,
+                     (Ptr{Cvoid}, Ptr{Cdouble}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}},
+                      Ptr{Cdouble}, Ref{CdftSolverOpts}, Ptr{Cdouble}, Ptr{CdftResult}),
+                     sys, Array{Float64}(vec(rho)), nobj, Array{Float64}(g), lam, length(pairs), hp, hm,
+                     Array{Float64}(h), opts, out, res), ctx)
+    finally
+        foreach(x -> ccall((:cdft_system_destroy, libcdft), Cint, (Ptr{Cvoid},), x), vcat(hp, hm, [sys]))
+    end
+    LAST[] = res
+    return lam, permutedims(out[1:length(pairs), :])
+end
+
+export HIPNative, converge_batch!, phi_vjp, adjoint_solve, adjoint_params
 
 end # module

@@ -257,6 +257,21 @@ int cdft_hvp(cdft_sys *sys, const double *rho, const double *v, double *out);
 int cdft_adjoint_solve(cdft_sys *sys, const double *rho, const double *g, double *lambda_inout,
                        const cdft_solver_opts *opts, cdft_result *result);
 
+/* adjoint_params: the device part of parameter sensitivities at a converged rho of
+ * sys, for nobj objectives J_k at once.  With g (nobj x n, g_k = rho * dJ_k/drho)
+ * it first solves lambda_k as cdft_adjoint_solve does (lambda_inout[k n ..] the
+ * initial guess in, lambda_k out; results[k] its outcome); with g == NULL
+ * lambda_inout already holds the multipliers.  Then for every parameter p, from two
+ * systems built with theta_p + h[p] and theta_p - h[p] over the same grid
+ * (sys_plus[p], sys_minus[p], same context; their own bulk state and external
+ * field), out[k nparam + p] = lambda_k . (G_plus(x) - G_minus(x)) / (2 h[p]) at
+ * x = log rho, G the undamped map above: the implicit part of dJ_k/dtheta_p, to which
+ * the caller adds the explicit partial dJ_k/dtheta_p at fixed rho.  nparam = 0 only
+ * solves.  (No reference counterpart: the reference differentiates no solve.) */
+int cdft_adjoint_params(cdft_sys *sys, const double *rho, int nobj, const double *g, double *lambda_inout,
+                        int nparam, cdft_sys *const *sys_plus, cdft_sys *const *sys_minus, const double *h,
+                        const cdft_solver_opts *opts, double *out, cdft_result *results);
+
 /* Dynamic DFT: the right-hand side d(eta)/dt of the log-density form, eta = ln rho
  * (nc*N values, same layout as rho): rho = exp(clamp(eta, -50, 30)),
  * mu = dF_res/drho + beta V_ext, and per component

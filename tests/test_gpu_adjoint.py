@@ -157,3 +157,51 @@ def test_surface_tension_and_adsorption_abi(wall):
     a_dev = s.device_system.adsorption(rho)
     a_host = adsorption_system(s, rho)
     assert np.max(np.abs(a_dev - a_host)) <= 1e-14 * np.max(np.abs(a_host))
+
+
+def test_adjoint_params_abi_matches_its_restatement():
+    """cdft_adjoint_params on the binary interface (two objectives, Gamma_0 and
+    Gamma_1, at once; two parameter pairs): its multipliers equal
+    cdft_adjoint_solve's for each objective, and each output equals
+    lambda_k . (G_plus - G_minus) / (2 h) assembled on the host from the two
+    perturbed systems' log maps.  A multiplier given back (g = NULL) gives the same
+    outputs without a solve; invalid pairs are refused."""
+    from cdft_amd._lib import CDFTError
+    say("cdft_adjoint_params: two objectives, two parameter pairs, methane/n-decane N=256")
+    s, rho0 = configs.c1c10_interface(ngrid=256, box=10.0)
+    rho = _converged(s, rho0.reshape(-1), tol=1e-12, floor=1e-9)
+    n = rho.size
+    w = adj.integration_weights(s.structure) / s.structure.volume()
+    g = np.zeros((2, n))
+    for k in range(2):
+        g[k, k * s.N:(k + 1) * s.N] = rho[k * s.N:(k + 1) * s.N] * w
+    t = adj.translation_mode(s, rho)
+    g = g - np.outer(g @ t / (t @ t), t)
+    lam = np.zeros((2, n))
+    dev = s.device_system
+    _, res = dev.adjoint_params(rho, lam, g=g, tol=1e-10)
+    assert [r["errcode"] for r in res] == [0, 0], res
+    for k in range(2):
+        lk, rk = dev.adjoint_solve(rho, g[k], tol=1e-10)
+        assert rk["nevals"] == res[k]["nevals"]
+        assert np.array_equal(lk, lam[k])
+    pairs, hs, want = [], [], []
+    for i, prm in ((0, "epsilon"), (1, "sigma")):
+        base = adj.model_params(s.model)[s.model.components[i]][{"sigma": 1, "epsilon": 2}[prm]]
+        h = 1e-6 * base
+        sp, sm = adj.perturbed_system(s, i, prm, h), adj.perturbed_system(s, i, prm, -h)
+        pairs.append((sp, sm))
+        hs.append(h)
+        dG = (adj.log_map(sp, rho) - adj.log_map(sm, rho)) / (2 * h)
+        want.append(lam @ dG)
+    want = np.array(want).T                       # (objective, parameter)
+    got, _ = dev.adjoint_params(rho, lam, pairs=[(a.device_system, b.device_system) for a, b in pairs], h=hs)
+    say(f"  implicit parts:\n{got}\n  host restatement:\n{want}")
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12 * np.max(np.abs(want)))
+    with pytest.raises(CDFTError):
+        dev.adjoint_params(rho, lam, pairs=[(pairs[0][0].device_system, dev)], h=[hs[0]])
+    with pytest.raises(CDFTError):
+        dev.adjoint_params(rho, lam, pairs=[(pairs[0][0].device_system, pairs[0][1].device_system)], h=[0.0])
+    other, _ = configs.c1c10_interface(ngrid=128, box=10.0)
+    with pytest.raises(CDFTError):
+        dev.adjoint_params(rho, lam, pairs=[(pairs[0][0].device_system, other.device_system)], h=[hs[0]])
