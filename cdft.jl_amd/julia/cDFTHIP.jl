@@ -70,14 +70,36 @@ end
 
 _check(rc, ctx) = rc == 0 || error("cdft error $rc: ", unsafe_string(ccall((:cdft_last_error, libcdft), Cstring, (Ptr{Cvoid},), ctx)))
 
+# Wertheim association data in the descriptor's layout: pack_assoc_params
+# (src/models/DFT/assoc.jl:18-57) with 0-based global site indices
+function _assoc(model)
+    site_comp = Cint[]; site_count = Float64[]
+    pair_site = Cint[]; pair_eps = Float64[]; pair_kappa = Float64[]
+    if hasproperty(model.params, :bondvol) && !isempty(model.params.bondvol.values.values)
+        sites = Clapeyron.getsites(model)
+        nper = [length(sites.n_sites[i]) for i in 1:length(model)]
+        off = cumsum([0; nper])
+        for i in 1:length(model), a in 1:nper[i]
+            push!(site_comp, i - 1)
+            push!(site_count, sites.n_sites[i][a])
+        end
+        ev = model.params.epsilon_assoc.values
+        kv = model.params.bondvol.values
+        for idx in 1:length(ev.values)
+            i, j = ev.outer_indices[idx]
+            a, b = ev.inner_indices[idx]
+            push!(pair_site, off[i] + a - 1, off[j] + b - 1)
+            push!(pair_eps, ev.values[idx])
+            push!(pair_kappa, kv.values[idx])
+        end
+    end
+    return site_comp, site_count, pair_site, pair_eps, pair_kappa
+end
+
 function _system(system, ctx)
     model = system.model
     st = system.structure
-    # association sites go through the descriptor's assoc_* fields, which this binding
-    # does not fill yet: refuse associating models rather than drop their term
-    if hasproperty(model.params, :bondvol) && !isempty(model.params.bondvol.values.values)
-        error("cDFTHIP: associating models are not passed through this binding yet")
-    end
+    site_comp, site_count, pair_site, pair_eps, pair_kappa = _assoc(model)
     nd = cDFT.dimension(st)
     (p, T) = st.conditions
     ngrid = ntuple(i -> i <= nd ? Cint(st.ngrid[i]) : Cint(1), 3)
@@ -97,11 +119,17 @@ function _system(system, ctx)
     prec = cDFT.fptype(system.options) == Float64 ? Cint(0) : Cint(1)
     coord = st isa cDFT.DFTStructByCoord{cDFT.Spherical} ? Cint(1) :
             st isa cDFT.DFTStructByCoord{cDFT.Cylindrical} ? Cint(2) : Cint(0)
-    GC.@preserve m d sigma eps rhob mu vext begin
+    npair = length(pair_eps)
+    GC.@preserve m d sigma eps rhob mu vext site_comp site_count pair_site pair_eps pair_kappa begin
         desc = CdftSystemDesc(length(model), nd, ngrid, lb, ub, T, p, cDFT.length_scale(model), 1.3862,
                               pointer(m), pointer(d), pointer(sigma), pointer(eps), pointer(rhob), pointer(mu),
                               vext === nothing ? Ptr{Cdouble}(C_NULL) : pointer(vext), prec,
-                              0, 0, C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, coord)
+                              npair > 0 ? length(site_comp) : 0, npair,
+                              npair > 0 ? pointer(site_comp) : Ptr{Cint}(C_NULL),
+                              npair > 0 ? pointer(site_count) : Ptr{Cdouble}(C_NULL),
+                              npair > 0 ? pointer(pair_site) : Ptr{Cint}(C_NULL),
+                              npair > 0 ? pointer(pair_eps) : Ptr{Cdouble}(C_NULL),
+                              npair > 0 ? pointer(pair_kappa) : Ptr{Cdouble}(C_NULL), coord)
         h = Ref{Ptr{Cvoid}}(C_NULL)
         _check(ccall((:cdft_system_create, libcdft), Cint, (Ptr{Cvoid}, Ref{CdftSystemDesc}, Ref{Ptr{Cvoid}}), ctx, desc, h), ctx)
         return h[]
