@@ -370,14 +370,15 @@ class DeviceSystem:
 
 
 def converge_batch(ctx, systems, rhos, *, maxit=10000, beta=1e-2, tol=1e-4, anderson_start=1e-1,
-                   m=5, device_ptrs=False, noalloc=False):
+                   m=5, device_ptrs=False, noalloc=False, hist_cap=0):
     """Batched persistent solve of many one-component mixed-precision systems of at
     most 512 points, m <= 5 (cdft_converge_batch[_device]); other batches raise
     CDFTError.
 
     rhos: list of float64 numpy arrays (in/out), or device pointers (ints) when
     device_ptrs is True.  noalloc (device pointers only): the allocation-free entry
-    point cdft_converge_batch_device_noalloc."""
+    point cdft_converge_batch_device_noalloc.  hist_cap > 0: each result carries
+    "history", the residual norm of each of its first hist_cap map evaluations."""
     if noalloc and not device_ptrs:
         raise ValueError("the allocation-free batched entry point takes device pointers")
     n = len(systems)
@@ -388,11 +389,17 @@ def converge_batch(ctx, systems, rhos, *, maxit=10000, beta=1e-2, tol=1e-4, ande
         for r, s in zip(rhos, systems):
             assert r.dtype == np.float64 and r.flags.c_contiguous and r.size == s.N * s.nc
         ps = (_dp * n)(*[ptr(r) for r in rhos])
-    o = SolverOpts(int(maxit), float(beta), float(tol), float(anderson_start), int(m), 0, None)
+    hist = np.zeros(max(int(hist_cap), 1) * max(n, 1))
+    o = SolverOpts(int(maxit), float(beta), float(tol), float(anderson_start), int(m), int(hist_cap),
+                   ptr(hist) if hist_cap > 0 else None)
     res = (Result * n)()
     if noalloc:
         fn = load().cdft_converge_batch_device_noalloc
     else:
         fn = load().cdft_converge_batch_device if device_ptrs else load().cdft_converge_batch
     check(fn(ctx.handle, n, hs, ps, C.byref(o), res), ctx.handle)
-    return [r.as_dict() for r in res]
+    out = [r.as_dict() for r in res]
+    if hist_cap > 0 and not noalloc:
+        for i, d in enumerate(out):
+            d["history"] = hist[i * hist_cap:i * hist_cap + min(d["nhist"], hist_cap)].copy()
+    return out

@@ -1835,6 +1835,8 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         b.res = s->res;
         b.Q = s->Q;
         b.DG = s->DG;
+        b.hist = nullptr;
+        b.hist_cap = 0;
         b.pc = s->pc;
     }
     // device copies of the descriptors and the results: this device's static scratch
@@ -1853,6 +1855,18 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         f1.p = db;
         HIPCHECK_CTX(ctx, hipMalloc((void **)&dr, sizeof(BResult) * nsys));
         f2.p = dr;
+    }
+    // residual histories (allocating entry points): system i's at o->hist + i hist_cap
+    const int hcap = (!noalloc && o->hist && o->hist_cap > 0) ? o->hist_cap : 0;
+    Free f3;
+    if (hcap > 0) {
+        double *dh = nullptr;
+        HIPCHECK_CTX(ctx, hipMalloc((void **)&dh, sizeof(double) * (size_t)hcap * nsys));
+        f3.p = dh;
+        for (int q = 0; q < nsys; q++) {
+            hb[q].hist = dh + (size_t)q * hcap;
+            hb[q].hist_cap = hcap;
+        }
     }
     HIPCHECK_CTX(ctx, hipMemcpyAsync(db, hb, sizeof(BSys) * nsys, hipMemcpyHostToDevice, st));
     BOpts bo;
@@ -1876,6 +1890,10 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         for (int c = 0; c < sys[i]->nc; c++) chain_free = chain_free && sys[i]->m[c] == 1.0;
     auto kern = (bt == BT) ? (chain_free ? k_batched<1, true, BT, true> : k_batched<1, true, BT, false>)
                            : (chain_free ? k_batched<1, true, 2 * BT, true> : k_batched<1, true, 2 * BT, false>);
+    if (hcap > 0)
+        kern = (bt == BT) ? (chain_free ? k_batched<1, true, BT, true, true> : k_batched<1, true, BT, false, true>)
+                          : (chain_free ? k_batched<1, true, 2 * BT, true, true>
+                                        : k_batched<1, true, 2 * BT, false, true>);
     HIPCHECK_CTX(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev0, st));
     hipLaunchKernelGGL(kern, dim3(nsys), dim3(bt), lds, st, db, bo, dr);
@@ -1889,6 +1907,10 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
             HIPCHECK_CTX(ctx, hipMemcpyAsync(rho[order[q]], s->rho, sizeof(double) * s->N * s->nc,
                                              hipMemcpyDeviceToHost, st));
         }
+    if (hcap > 0)
+        for (int q = 0; q < nsys; q++)
+            HIPCHECK_CTX(ctx, hipMemcpyAsync(o->hist + (size_t)order[q] * hcap, hb[q].hist, sizeof(double) * hcap,
+                                             hipMemcpyDeviceToHost, st));
     HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
     float ms = 0.0f;
     HIPCHECK_CTX(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -1901,7 +1923,7 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
         r.errcode = hr[q].errcode;
         r.iters = hr[q].iters;
         r.nevals = hr[q].nevals;
-        r.nhist = 0;
+        r.nhist = hcap > 0 ? hr[q].nhist : 0;
         r.resnorm = hr[q].resnorm;
         r.tol = hr[q].tol;
         r.nevals_fp32 = hr[q].nevals32;

@@ -45,6 +45,8 @@ struct BSys {
     const double *tw64;        // W_M^t, t < M, (re, im) pairs
     double *rho;               // in: seed, out: converged profile  [c*N + i]
     double *sol, *gx, *df, *dg, *res, *Q, *DG;   // Anderson storage, n = N*nc each (Q, DG: CDFT_MAXM cols)
+    double *hist;              // residual norm of every evaluation (AASol's history), or null
+    int hist_cap;
     PhiConst pc;
 };
 
@@ -58,7 +60,7 @@ struct BOpts {
 };
 
 struct BResult {
-    int idid, errcode, iters, nevals, nevals32, pad;
+    int idid, errcode, iters, nevals, nevals32, nhist;
     double resnorm, tol;
     unsigned long long ticks;   // wall_clock64 ticks spent in the workgroup
 };
@@ -851,7 +853,9 @@ __host__ __device__ inline size_t batched_lds_layout(int N, int M, int nc, bool 
            (dg_lds ? (size_t)kRegAM * kRegPoints * 8 : 0);
 }
 
-template <int NC, bool REG, int BTT = BT, bool CF = false>
+// HIST: the variant that records every evaluation's residual norm (a separate
+// instantiation: the store's mere presence costs the recording-free kernel about 2 %)
+template <int NC, bool REG, int BTT = BT, bool CF = false, bool HIST = false>
 __global__ __launch_bounds__(BTT) void k_batched(const BSys *__restrict__ systems, BOpts o, BResult *results)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -885,6 +889,8 @@ __global__ __launch_bounds__(BTT) void k_batched(const BSys *__restrict__ system
     A.gDG = D->DG;
     A.n = (long)S.N * NC;
     const double *__restrict__ twg = D->tw64;
+    double *const hist = HIST ? D->hist : nullptr;
+    const int hist_cap = HIST && hist ? D->hist_cap : 0;
     {
         const double *src = reinterpret_cast<const double *>(&D->pc);
         double *dst = reinterpret_cast<double *>(&pcs);
@@ -1114,6 +1120,8 @@ __global__ __launch_bounds__(BTT) void k_batched(const BSys *__restrict__ system
         }
         nevals++;
         resnorm = sqrt(fo[0]);
+        if constexpr (HIST)
+            if (tid == 0 && nevals <= hist_cap) hist[nevals - 1] = resnorm;   // every evaluation is recorded
 
         // ---------------- book-keeping after it
         bool check_loop = false;
@@ -1187,7 +1195,7 @@ __global__ __launch_bounds__(BTT) void k_batched(const BSys *__restrict__ system
         r.iters = k;
         r.nevals = nevals;
         r.nevals32 = nevals32;
-        r.pad = 0;
+        r.nhist = HIST ? nevals : 0;
         r.resnorm = resnorm;
         r.tol = tol;
         r.ticks = wall_clock64() - t_start;

@@ -175,7 +175,10 @@ int cdft_converge(cdft_sys *sys, double *rho_inout, const cdft_solver_opts *opts
  * on the device; systems retire independently.  Each system follows exactly the
  * same algorithm as cdft_converge.  (No batched counterpart in the reference: it
  * replaces a loop of converge! calls, e.g. over adsorption(...) state points,
- * src/methods/adsorption.jl:48-66.)  hist and progress in opts are ignored.  One-component
+ * src/methods/adsorption.jl:48-66.)  opts->hist, when set, holds nsys x hist_cap
+ * residual norms (system i's from hist + i hist_cap, one per map evaluation,
+ * result[i].nhist of them recorded; the allocation-free entry point ignores it);
+ * progress is ignored.  One-component
  * CDFT_MIXED systems of at most 512 points with anderson m <= 5 only for now: other
  * batches return CDFT_ERR_UNSUPPORTED (use cdft_converge per system). */
 int cdft_converge_batch(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_inout,

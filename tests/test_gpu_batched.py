@@ -303,3 +303,33 @@ def test_batched_results_independent_of_earlier_launches(monkeypatch):
         outs.append(([r["nevals"] for r in res], np.concatenate([r.reshape(-1) for r in rm])))
     assert outs[0][0] == outs[2][0]
     assert np.array_equal(outs[0][1], outs[2][1], equal_nan=True)
+
+
+def test_batched_residual_history():
+    """opts.hist on the batched path: one residual norm per map evaluation for every
+    system of the batch (AASol's history, anderson.jl:327-337), in input order.  The
+    recording leaves the solve unchanged (same bits as without); each history ends at
+    the reported residual; its first entry (the FP32 map at the seed) agrees with the
+    FP64 oracle's first residual to FP32 accuracy, and its Picard phase follows the
+    oracle's to the same accuracy."""
+    mixed = configs.slit_pores(n=512, seed=0, indices=MIXED_PORES, options=DFTOptions(precision="mixed"))
+    say(f"batched residual history, {len(mixed)} pores, tol=1e-8")
+    sm = [s for s, _ in mixed]
+    r0 = [r.copy() for _, r in mixed]
+    res0 = batch(sm, r0, tol=1e-8)
+    r1 = [r.copy() for _, r in mixed]
+    res1 = batch(sm, r1, tol=1e-8, hist_cap=30000)
+    for i, (s, a, b, x, y) in enumerate(zip(sm, r0, r1, res0, res1)):
+        assert np.array_equal(a, b) and x["nevals"] == y["nevals"] and x["errcode"] == y["errcode"]
+        h = y["history"]
+        assert y["nhist"] == y["nevals"] == len(h)
+        assert h[-1] == y["resnorm"]
+        _, ro, ho = oracle_of(s).converge(mixed[i][1].reshape(-1), tol=1e-8, maxit=20, planned=True)
+        k = min(10, len(ho))
+        say(f"  pore {MIXED_PORES[i]} N={s.N}: {len(h)} entries, first {h[0]:.6e} (oracle {ho[0]:.6e})")
+        np.testing.assert_allclose(h[:k], ho[:k], rtol=2e-5)
+    # a small cap keeps the first entries
+    r2 = [r.copy() for _, r in mixed]
+    res2 = batch(sm, r2, tol=1e-8, hist_cap=5)
+    for y, z in zip(res1, res2):
+        assert z["nhist"] == z["nevals"] and np.array_equal(z["history"], y["history"][:5])
