@@ -1039,10 +1039,52 @@ static bool aa_use_graph()
     return !(e && e[0] == '0');
 }
 
+// The single-workgroup body (k_small_body) serves small 1D Cartesian FP64 solves of
+// converge!; CDFT_SMALL_BODY=0 keeps them on the multi-kernel body.
+static bool small_body_ok(const cdft_sys *s, bool adjoint, bool mixed)
+{
+    if (adjoint || mixed || s->nd != 1 || s->coord || s->nc > 2 || s->F != SB_F || (long)s->N * s->nc > SB_MAXN)
+        return false;
+    const char *e = std::getenv("CDFT_SMALL_BODY");
+    return !(e && e[0] == '0');
+}
+
+extern "C++" {
+// Dynamic LDS above the default needs the kernel's attribute raised first (to what
+// this system needs; the static LDS comes on top of it).
+template <int NC, int AS>
+static hipError_t small_body_launch(const cdft_sys *s, const SBArgs &a, int nbody)
+{
+    static size_t attr = 0;
+    const size_t lds = sb_lds_bytes(s->N, s->nc);
+    if (lds > attr) {
+        const hipError_t e = hipFuncSetAttribute((const void *)k_small_body<NC, AS>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attr = lds;
+    }
+    hipLaunchKernelGGL((k_small_body<NC, AS>), dim3(1), dim3(SB_T), lds, s->ctx->stream, a, s->pc, nbody);
+    return hipSuccess;
+}
+}  // extern "C++"
+
+static hipError_t small_body(const cdft_sys *s, AACtl *c, int nbody)
+{
+    const SBArgs a{c,     s->h,  s->vext, s->cpd,   s->sol, s->gx,    s->res, s->df,  s->dg,
+                   s->Q,  s->Qalt, s->DG, s->xsafe, s->rho, s->dhist, s->N,   s->nc,  s->NA};
+    const int as = s->assoc < 2 ? s->assoc : 2;
+    if (s->nc == 1)
+        return as == 0 ? small_body_launch<1, 0>(s, a, nbody)
+                       : (as == 1 ? small_body_launch<1, 1>(s, a, nbody) : small_body_launch<1, 2>(s, a, nbody));
+    return as == 0 ? small_body_launch<2, 0>(s, a, nbody)
+                   : (as == 1 ? small_body_launch<2, 1>(s, a, nbody) : small_body_launch<2, 2>(s, a, nbody));
+}
+
 // aasol (src/utils/anderson.jl:301-443) entirely on the device: every N-length
 // operation and the m x m QR downdate and triangular solve run in kernels, the
 // solver's control flow in one thread of the kernels that finish the reductions
-// (k_aa_fin: aa_mid, aa_post, aa_pre), twelve launches per body in 1D.  The host
+// (k_aa_fin: aa_mid, aa_post, aa_pre), twelve launches per body in 1D (one, the
+// single-workgroup k_small_body, for small 1D FP64 systems).  The host
 // enqueues bodies of one map evaluation each — replayed from a hipGraph of
 // aa_chunk() bodies captured once per system and kind of map — and reads the
 // solver state once per chunk.  adj == nullptr: the Euler-Lagrange map of
@@ -1113,7 +1155,14 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     }
 
     AACtl *c = s->ctl;
+    const bool small = small_body_ok(s, adj != nullptr, mixed);
+    hipError_t sb_err = hipSuccess;
     auto body = [&]() {
+        if (small) {                              // one body (the first, eager)
+            const hipError_t e = small_body(s, c, 1);
+            if (e != hipSuccess) sb_err = e;
+            return;
+        }
         hipLaunchKernelGGL(k_aa_proj1, dim3(nb), dim3(256), 0, st, (int)n, (const double *)s->Q, s->Qalt,
                            (const AACtl *)c, (const double *)s->df, (const double *)s->res, s->red, nb);
         hipLaunchKernelGGL(k_aa_fin<0>, dim3(2 * CDFT_MAXM), dim3(256), 0, st, (const double *)s->red, nb, s->fin, c, 0,
@@ -1157,6 +1206,7 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     std::vector<double> prho(prog ? n : 0);       // the paused evaluation's profile
     HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev0, st));
     body();                                       // eager first body
+    HIPCHECK_CTX(ctx, sb_err);
     HIPCHECK_CTX(ctx, hipGetLastError());
     long bodies = 1, pauses = 0;
     while (true) {
@@ -1171,7 +1221,8 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
                 std::memcpy(rho_inout, prho.data(), sizeof(double) * n);
                 return fail(ctx, CDFT_ERR_STOPPED, "cdft_converge: stopped by the progress callback");
             }
-            hipLaunchKernelGGL(k_aa_resume, dim3(1), dim3(1), 0, st, c);
+            if (small) hipLaunchKernelGGL(k_aa_resume<true>, dim3(1), dim3(64), 0, st, c);
+            else hipLaunchKernelGGL(k_aa_resume<false>, dim3(1), dim3(1), 0, st, c);
             HIPCHECK_CTX(ctx, hipGetLastError());
             pauses++;
             continue;
@@ -1179,7 +1230,10 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
         if (hc->phase == AA_DONE) break;
         // a pause leaves at most one chunk's bodies unused
         if (bodies > bound + pauses * K) return fail(ctx, CDFT_ERR_HIP, "aasol: the device-side solver did not terminate");
-        if (use_graph) {
+        if (small) {                              // a chunk of bodies in one launch
+            HIPCHECK_CTX(ctx, small_body(s, c, K));
+            HIPCHECK_CTX(ctx, hipGetLastError());
+        } else if (use_graph) {
             bool same = s->aa_exec != nullptr;
             for (int j = 0; j < 5 && same; j++) same = (s->aa_key[j] == key[j]);
             if (!same) {

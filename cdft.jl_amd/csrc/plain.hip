@@ -700,10 +700,14 @@ __device__ inline void aa_post(AACtl &c, const double *fin, double *hist)
 __device__ inline void aa_mid(AACtl &c, const double *fin);
 
 // Before a map evaluation: the work of the next body (which point, which map,
-// which update), with BuildDG!'s ring and the QR downdate of aa_qr_update!.
-__device__ inline void aa_pre(AACtl &c)
+// which update), with BuildDG!'s ring and the QR downdate of aa_qr_update!.  With
+// DEFER the downdate is left to the caller (a whole wave, wave_downdate); the
+// return value says whether one is due.
+template <bool DEFER = false>
+__device__ inline bool aa_pre(AACtl &c)
 {
 #pragma clang fp contract(off)
+    bool dd = false;
     c.g_proj = 1;
     c.pmode = 0;
     c.umode = 1;
@@ -762,7 +766,8 @@ __device__ inline void aa_pre(AACtl &c)
         } else {
             for (int j = 0; j < CDFT_MAXM * CDFT_MAXM; j++) c.qx.a[j] = 0.0;
             c.down = (kq > m - 1);
-            if (c.down) householder_downdate(c.R, m, c.qx.a);
+            if constexpr (DEFER) dd = c.down;
+            else if (c.down) householder_downdate(c.R, m, c.qx.a);
             c.kqq = min(kq, m - 1);
             c.g_proj = 0;
         }
@@ -781,6 +786,7 @@ __device__ inline void aa_pre(AACtl &c)
     c.g_f64 = done || c.fp32;
     c.g_f32 = done || !c.fp32;
     c.evfp32 = !done && c.fp32;
+    return dd;
 }
 
 // Pause after an evaluation (converge!'s per-evaluation logger, converge.jl:256-267):
@@ -861,12 +867,27 @@ __global__ __launch_bounds__(256) void k_aa_fin(const double *part, int nblk, do
     }
 }
 
-// Resumes a paused solve: the deferred aa_pre of the next body.  One thread.
+// The Anderson QR downdate by one whole wave (batched.hip).
+template <int MX>
+__device__ inline void wave_downdate(double *R, int m, double *Qx);
+
+// Resumes a paused solve: the deferred aa_pre of the next body, with the downdate
+// of the body that paused (WAVE: k_small_body's whole-wave one; launch one wave),
+// so that a pause changes no bit of the solve.
+template <bool WAVE>
 __global__ void k_aa_resume(AACtl *ctl)
 {
-    if (!ctl->paused) return;
-    ctl->paused = 0;
-    aa_pre(*ctl);
+    __shared__ int dd;
+    if (threadIdx.x == 0) {
+        dd = 0;
+        if (ctl->paused) {
+            ctl->paused = 0;
+            if constexpr (WAVE) dd = aa_pre<true>(*ctl);
+            else aa_pre(*ctl);
+        }
+    }
+    __syncthreads();
+    if (WAVE && dd) wave_downdate<CDFT_MAXM>(ctl->R, ctl->m, ctl->qx.a);
 }
 
 // pass 1: optional downdate Q' = Q Qx, then h1 = Q'^T df and t = Q'^T res.
@@ -971,6 +992,339 @@ __global__ __launch_bounds__(256) void k_aa_point(int n, const AACtl *__restrict
     }
     sol[i] = x;
     rho[i] = mask ? mask[i] * x : exp(x);
+}
+
+// ---------------------------------------------------------------------------
+// Bodies of the plain path in a single workgroup, for small 1D Cartesian FP64
+// systems (n = N nc <= SB_MAXN, nc <= 2): the Gram-Schmidt passes, the new point,
+// the weighted densities, Phi, the transposed convolutions, the update and the
+// control step of k_aa_proj1 / k_aa_proj2 / k_aa_point / k_circ1d / k_phi /
+// k_update / k_aa_fin, in that order and with the same arithmetic per point, for
+// up to nbody bodies in one launch instead of twelve launches per body (on grids
+// this small each launch costs more than its work).  The control state is copied
+// into LDS, advanced by the same aa_mid / aa_post / aa_pre in thread 0 (the QR
+// downdate by wave 0, wave_downdate) and written back at the end.  Sums run in a
+// fixed order (per point over taps in order; over the workgroup by waves in order),
+// so a body is reproducible; they are not the multi-kernel path's orders, so the two
+// paths agree to rounding.
+constexpr int SB_T = 512;          // threads
+constexpr int SB_MAXN = 512;       // n = N nc at most
+constexpr int SB_F = 7;            // 1D field slots
+
+struct SBArgs {
+    AACtl *ctl;
+    const double *h;               // circulant kernels h[(c F + f) N + k]
+    const double *vext, *cp;
+    double *sol, *gx, *res, *df, *dg, *Q0, *Q1, *DG, *xsafe, *rho, *hist;
+    int N, nc;
+    double NA;
+};
+
+// Extended periodic source ue[j] = u[j mod N], j < Le, stored decimated by four:
+// ue[j] at ud[(j & 3) Lq + (j >> 2)], Lq = Le / 4, so that the sliding windows of
+// four consecutive outputs per lane read consecutive addresses across a wave.
+__host__ __device__ inline int sb_le(int N) { return 4 * ((2 * N + 4 + 3) / 4); }
+
+__device__ __forceinline__ void sb_put(double *ud, int Lq, int N, int i, double v)
+{
+    for (int j = i; j < 4 * Lq; j += N) ud[(j & 3) * Lq + (j >> 2)] = v;
+}
+
+// out_r = sum_{k < N} h[k] ue[4b + r - k + N], r < 4, taps in order (a circular
+// convolution at the outputs 4b .. 4b + 3).
+__device__ __forceinline__ void sb_conv4(const double *__restrict__ ud, int Lq, const double *__restrict__ hk, int N,
+                                         int b, double (&a)[4])
+{
+    auto U = [&](int j) __attribute__((always_inline)) { return ud[(j & 3) * Lq + (j >> 2)]; };
+    const int j0 = 4 * b + N;
+    double w0 = U(j0), w1 = U(j0 + 1), w2 = U(j0 + 2), w3 = U(j0 + 3);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    // ue[j0 - 1 - s - 4q] for s < 4 sits at p_s[-q]: stepping k by four keeps j & 3
+    const double *p0 = ud + ((j0 - 1) & 3) * Lq + ((j0 - 1) >> 2);
+    const double *p1 = ud + ((j0 - 2) & 3) * Lq + ((j0 - 2) >> 2);
+    const double *p2 = ud + ((j0 - 3) & 3) * Lq + ((j0 - 3) >> 2);
+    const double *p3 = ud + ((j0 - 4) & 3) * Lq + ((j0 - 4) >> 2);
+    int k = 0;
+    for (int q = 0; k + 4 <= N; k += 4, q++) {
+        const double h0 = hk[k], h1 = hk[k + 1], h2 = hk[k + 2], h3 = hk[k + 3];
+        a0 += h0 * w0; a1 += h0 * w1; a2 += h0 * w2; a3 += h0 * w3;
+        const double x = p0[-q];                       // ue[j0 - k - 1]
+        a0 += h1 * x; a1 += h1 * w0; a2 += h1 * w1; a3 += h1 * w2;
+        const double y = p1[-q];                       // ue[j0 - k - 2]
+        a0 += h2 * y; a1 += h2 * x; a2 += h2 * w0; a3 += h2 * w1;
+        const double z = p2[-q];                       // ue[j0 - k - 3]
+        a0 += h3 * z; a1 += h3 * y; a2 += h3 * x; a3 += h3 * w0;
+        const double u = p3[-q];                       // ue[j0 - k - 4], j0 - k - 4 >= 4 b >= 0
+        w3 = x; w2 = y; w1 = z; w0 = u;
+    }
+    for (; k < N; k++) {
+        const double hv = hk[k];
+        a0 += hv * w0; a1 += hv * w1; a2 += hv * w2; a3 += hv * w3;
+        w3 = w2; w2 = w1; w1 = w0;
+        w0 = U(j0 - k - 1);                  // >= 4 b >= 0
+    }
+    a[0] = a0; a[1] = a1; a[2] = a2; a[3] = a3;
+}
+
+// Workgroup sums of NQ values into out[0..NQ) (LDS), waves added in order.
+template <int NQ>
+__device__ __forceinline__ void sb_sum(double (&v)[NQ], double *wred, double *out)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+        for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) wred[wid * 16 + q] = v[q];
+    __syncthreads();
+    if ((int)threadIdx.x < NQ) {
+        double t = 0.0;
+        for (int w = 0; w < SB_T / 64; w++) t += wred[w * 16 + threadIdx.x];
+        out[threadIdx.x] = t;
+    }
+    __syncthreads();
+}
+
+template <int NC, int AS>
+__global__ __launch_bounds__(SB_T) void k_small_body(SBArgs A, PhiConst P, int nbody)
+{
+    constexpr int F = SB_F;
+    extern __shared__ __align__(16) double sbm[];
+    __shared__ AACtl C;
+    __shared__ double wred[(SB_T / 64) * 16];
+    __shared__ double fin[2 * CDFT_NQ + 4];
+    const int tid = threadIdx.x;
+    const int N = A.N, n = N * NC;
+    {
+        const int *src = reinterpret_cast<const int *>(A.ctl);
+        int *dst = reinterpret_cast<int *>(&C);
+        for (int i = tid; i < (int)(sizeof(AACtl) / sizeof(int)); i += SB_T) dst[i] = src[i];
+    }
+    __syncthreads();
+    if (C.g_done) return;                      // the solve has ended (or is paused)
+    const int Le = sb_le(N), Lq = Le / 4;
+    double *fld = sbm;                         // [NC F N]
+    double *hk = fld + NC * F * N;             // [NC (F-1) N]
+    double *ud = hk + NC * (F - 1) * N;        // [NC (F-1) Le]
+    for (int t = tid; t < NC * (F - 1) * N; t += SB_T) {
+        const int cf = t / N, k = t - cf * N, c = cf / (F - 1), f = 1 + cf % (F - 1);
+        hk[t] = A.h[(size_t)(c * F + f) * N + k];
+    }
+
+    // up to nbody bodies, until the solve ends or pauses (C stays in LDS between them)
+    for (int body = 0; body < nbody; body++) {
+        if (body > 0 && C.g_done) break;
+        // ---- Gram-Schmidt passes (k_aa_proj1, k_aa_proj2 x 2) and aa_mid
+        // (loops over history columns unrolled to CDFT_MAXM with the live ones selected,
+        // so every per-thread array stays in registers)
+        if (!C.g_proj) {
+            constexpr int MX = CDFT_MAXM;
+            const int m = C.m, kq = C.kqq, down = C.down;
+            const double *Q = C.qsel ? A.Q1 : A.Q0;
+            double *Qout = C.qsel ? A.Q0 : A.Q1;
+            double v1[2 * MX];
+#pragma unroll
+            for (int q = 0; q < 2 * MX; q++) v1[q] = 0.0;
+            for (int i = tid; i < n; i += SB_T) {
+                double qv[MX];
+                if (down) {
+                    double qo[MX];
+#pragma unroll
+                    for (int l = 0; l < MX; l++) qo[l] = (l < m) ? Q[(size_t)l * n + i] : 0.0;
+#pragma unroll
+                    for (int j = 0; j < MX - 1; j++) {
+                        double a = 0.0;
+#pragma unroll
+                        for (int l = 0; l < MX; l++)
+                            if (l < m) a += qo[l] * C.qx.a[j * m + l];
+                        qv[j] = a;
+                        if (j < m - 1) Qout[(size_t)j * n + i] = a;
+                    }
+                    qv[MX - 1] = 0.0;
+                    Qout[(size_t)(m - 1) * n + i] = 0.0;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < MX; j++) qv[j] = (j < kq) ? Q[(size_t)j * n + i] : 0.0;
+                }
+                const double dv = A.df[i], rv = A.res[i];
+#pragma unroll
+                for (int j = 0; j < MX; j++)
+                    if (j < kq) {
+                        v1[j] += qv[j] * dv;
+                        v1[MX + j] += qv[j] * rv;
+                    }
+            }
+            sb_sum<2 * MX>(v1, wred, fin);
+            const double *Q2 = (C.qsel ^ C.down) ? A.Q1 : A.Q0;
+            for (int pass = 0; pass < 2; pass++) {
+                const double *hc = pass == 0 ? fin : fin + CDFT_NQ;
+                double v2[MX];
+#pragma unroll
+                for (int q = 0; q < MX; q++) v2[q] = 0.0;
+                for (int i = tid; i < n; i += SB_T) {
+                    double qv[MX];
+                    double a = 0.0;
+#pragma unroll
+                    for (int j = 0; j < MX; j++) {
+                        qv[j] = (j < kq) ? Q2[(size_t)j * n + i] : 0.0;
+                        if (j < kq) a += qv[j] * hc[j];
+                    }
+                    const double d = A.df[i] - a;
+                    A.df[i] = d;
+                    if (pass == 0) {
+#pragma unroll
+                        for (int j = 0; j < MX; j++)
+                            if (j < kq) v2[j] += qv[j] * d;
+                    } else {
+                        v2[0] += d * d;
+                        v2[1] += d * A.res[i];
+                    }
+                }
+                if (pass == 0) sb_sum<MX>(v2, wred, fin + CDFT_NQ);
+                else {
+                    double v3[2] = {v2[0], v2[1]};
+                    sb_sum<2>(v3, wred, fin + 2 * CDFT_NQ);
+                }
+            }
+            if (tid == 0) aa_mid(C, fin);
+            __syncthreads();
+        }
+
+        // ---- the new point (k_aa_point) and rho; rho N_A into slot 0, rho as the
+        // weighted densities' source (slot 0 of each component's source block)
+        {
+            const int pm = C.pmode, keep = C.keep;
+            double *Qc = C.qsel ? A.Q1 : A.Q0;
+            for (int t = tid; t < n; t += SB_T) {
+                if (keep) A.xsafe[t] = A.sol[t];
+                double r;
+                if (pm != 0) {
+                    double x;
+                    if (pm == 1) {
+                        x = A.gx[t];
+                    } else if (pm == 2) {
+                        x = A.xsafe[t];
+                    } else {
+                        A.DG[(size_t)C.slot * n + t] = A.dg[t];
+                        Qc[(size_t)C.qcol * n + t] = A.df[t] / C.nrm;
+                        double a = 0.0;
+                        for (int j = 0; j < C.mk; j++) a += A.DG[(size_t)C.th.col[j] * n + t] * C.th.a[j];
+                        x = A.gx[t] - a;
+                    }
+                    A.sol[t] = x;
+                    r = exp(x);
+                    A.rho[t] = r;
+                } else {
+                    r = A.rho[t];
+                }
+                const int c = t / N, i = t - c * N;
+                fld[(c * F) * N + i] = r * A.NA;
+                sb_put(ud + (size_t)c * (F - 1) * Le, Lq, N, i, r);
+            }
+        }
+        __syncthreads();
+
+        // ---- weighted densities n_f = N_A h_f (*) rho (k_circ1d)
+        const int NB = (N + 3) / 4, NI = NC * (F - 1) * NB;
+        for (int it = tid; it < NI; it += SB_T) {
+            const int cf = it / NB, b = it - cf * NB, c = cf / (F - 1), f = 1 + cf % (F - 1);
+            double a[4];
+            sb_conv4(ud + (size_t)c * (F - 1) * Le, Lq, hk + (size_t)cf * N, N, b, a);
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (4 * b + r < N) fld[(c * F + f) * N + 4 * b + r] = A.NA * a[r];
+        }
+        __syncthreads();
+
+        // ---- Phi and dPhi/dn in place (k_phi)
+        for (int i = tid; i < N; i += SB_T) {
+            double nl[NC][F], g[NC][F];
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int f = 0; f < F; f++) nl[c][f] = fld[(c * F + f) * N + i];
+            (void)phi_grad<double, double, NC, 1, true, AS>(P, nl, g);
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int f = 0; f < F; f++) fld[(c * F + f) * N + i] = g[c][f];
+        }
+        __syncthreads();
+        for (int t = tid; t < NC * (F - 1) * N; t += SB_T) {
+            const int cf = t / N, i = t - cf * N, c = cf / (F - 1), f = 1 + cf % (F - 1);
+            sb_put(ud + (size_t)cf * Le, Lq, N, i, fld[(c * F + f) * N + i]);
+        }
+        __syncthreads();
+
+        // ---- transposed convolutions p_f = +-h_f (*) dPhi/dn_f (k_circ1d, minus on the
+        // vector slot) into the slots f >= 1
+        for (int it = tid; it < NI; it += SB_T) {
+            const int cf = it / NB, b = it - cf * NB, c = cf / (F - 1), f = 1 + cf % (F - 1);
+            double a[4];
+            sb_conv4(ud + (size_t)cf * Le, Lq, hk + (size_t)cf * N, N, b, a);
+            const double sg = (f == 3) ? -1.0 : 1.0;
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (4 * b + r < N) fld[(c * F + f) * N + 4 * b + r] = sg * a[r];
+        }
+        __syncthreads();
+
+        // ---- the update (k_update with the solver's mode and damping)
+        {
+            const int mode = C.umode;
+            const double beta = C.beta_u;
+            double v[3] = {0.0, 0.0, 0.0};
+            for (int t = tid; t < n; t += SB_T) {
+                const int c = t / N, i = t - c * N;
+                double s = fld[(c * F) * N + i];
+                for (int f = 1; f < F; f++) s += fld[(c * F + f) * N + i];
+                if (A.vext) s += A.vext[t];
+                const double lg = clamp_lng(A.cp[c] - s);
+                const double x = A.sol[t];
+                const double g = (beta == 1.0) ? lg : (1.0 - beta) * x + beta * lg;
+                if (mode == 1) {
+                    A.gx[t] = g;
+                    const double r = g - x;
+                    A.res[t] = r;
+                    v[0] += r * r;
+                } else {
+                    const double gold = A.gx[t], rold = A.res[t];
+                    A.dg[t] = -gold + g;
+                    A.gx[t] = g;
+                    const double r = g - x;
+                    A.res[t] = r;
+                    const double d = -rold + r;
+                    A.df[t] = d;
+                    v[0] += r * r;
+                    v[1] += d * d;
+                    v[2] += d * r;
+                }
+            }
+            sb_sum<3>(v, wred, fin);
+        }
+        __shared__ int dd;
+        if (tid == 0) {
+            aa_post(C, fin, A.hist);
+            dd = 0;
+            if (C.pause_every > 0 && C.nevals % C.pause_every == 0) aa_pause(C);
+            else dd = aa_pre<true>(C);
+        }
+        __syncthreads();
+        if (dd && tid < 64) wave_downdate<CDFT_MAXM>(C.R, C.m, C.qx.a);
+        __syncthreads();
+    }
+    {
+        const int *src = reinterpret_cast<const int *>(&C);
+        int *dst = reinterpret_cast<int *>(A.ctl);
+        for (int i = tid; i < (int)(sizeof(AACtl) / sizeof(int)); i += SB_T) dst[i] = src[i];
+    }
+}
+
+// Dynamic LDS bytes of k_small_body.
+__host__ __device__ inline size_t sb_lds_bytes(int N, int nc)
+{
+    return sizeof(double) * ((size_t)nc * SB_F * N + (size_t)nc * (SB_F - 1) * N + (size_t)nc * (SB_F - 1) * sb_le(N));
 }
 
 }  // namespace cdft

@@ -57,7 +57,9 @@ def test_graph_wall_matches_oracle(monkeypatch):
         say(f"  maxit {maxit}: GPU errcode {r['errcode']} evals {r['nevals']} | oracle {ro.errcode} {ro.nevals}")
         assert r["errcode"] == ro.errcode and r["nevals"] == ro.nevals and r["iters"] == ro.k
         np.testing.assert_allclose(r["history"], hist_o, rtol=1e-8)
-        np.testing.assert_allclose(rho.reshape(-1), rho_o, rtol=1e-9)
+        # near the walls rho ~ 1e-44 .. 1e-20 = exp(x) with x ~ -100: the sum orders of the
+        # convolutions move x by ~1e-9, i.e. rho by as much relative
+        np.testing.assert_allclose(rho.reshape(-1), rho_o, rtol=1e-8)
 
 
 def test_graph_matches_eager_3d_mixed(monkeypatch):
