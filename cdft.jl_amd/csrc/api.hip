@@ -1039,6 +1039,31 @@ static bool aa_use_graph()
     return !(e && e[0] == '0');
 }
 
+// The device solver's state at the start of aasol (converge!'s keywords,
+// converge.jl:3-13): first evaluation at the seed, then Picard, then Anderson.
+static void aa_ctl_init(AACtl *hc, const cdft_solver_opts *o, bool mixed, int hcap, bool adjoint, int pause_every)
+{
+    std::memset(hc, 0, sizeof(AACtl));
+    hc->m = o->m;
+    hc->maxit = o->maxit;
+    hc->picard_maxit = o->maxit;
+    hc->hist_cap = hcap;
+    hc->adjoint = adjoint ? 1 : 0;
+    hc->rtol = hc->atol = o->tol;
+    hc->picard_rtol = hc->picard_atol = o->anderson_start;
+    hc->beta = hc->picard_beta = o->beta;
+    hc->fp32 = mixed ? 1 : 0;
+    hc->phase = AA_FIRST;
+    hc->umode = 1;
+    hc->beta_u = o->beta;
+    hc->g_proj = 1;
+    hc->g_f64 = hc->fp32;
+    hc->g_f32 = !hc->fp32;
+    hc->evfp32 = hc->fp32;
+    hc->pause_every = pause_every;
+    for (int j = 0; j < CDFT_MAXM; j++) hc->ring[j] = j;
+}
+
 // The single-workgroup body (k_small_body) serves small 1D Cartesian FP64 solves of
 // converge!; CDFT_SMALL_BODY=0 keeps them on the multi-kernel body.
 static bool small_body_ok(const cdft_sys *s, bool adjoint, bool mixed)
@@ -1123,25 +1148,7 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
     // floor (4 tol, or no halving in 40 iterations), then the FP64 map with a fresh
     // Anderson history; the reported residual and termination are the FP64 ones.
     const bool mixed = (!adj && s->precision == CDFT_MIXED);
-    std::memset(hc, 0, sizeof(AACtl));
-    hc->m = m;
-    hc->maxit = o->maxit;
-    hc->picard_maxit = o->maxit;
-    hc->hist_cap = hcap;
-    hc->adjoint = adj ? 1 : 0;
-    hc->rtol = hc->atol = o->tol;
-    hc->picard_rtol = hc->picard_atol = o->anderson_start;
-    hc->beta = hc->picard_beta = beta;
-    hc->fp32 = mixed ? 1 : 0;
-    hc->phase = AA_FIRST;
-    hc->umode = 1;
-    hc->beta_u = beta;
-    hc->g_proj = 1;
-    hc->g_f64 = hc->fp32;
-    hc->g_f32 = !hc->fp32;
-    hc->evfp32 = hc->fp32;
-    hc->pause_every = prog ? o->progress_every : 0;
-    for (int j = 0; j < CDFT_MAXM; j++) hc->ring[j] = j;
+    aa_ctl_init(hc, o, mixed, hcap, adj != nullptr, prog ? o->progress_every : 0);
     HIPCHECK_CTX(ctx, hipMemcpyAsync(s->ctl, hc, sizeof(AACtl), hipMemcpyHostToDevice, st));
 
     double *ptgt = adj ? adj->vbuf : s->rho;              // where a new point is expanded
@@ -1769,6 +1776,123 @@ static int batched_prepare(cdft_ctx *ctx, cdft_sys *s)
     return CDFT_OK;
 }
 
+// FP64 batches of small 1D systems (one or two components, N nc <= SB_MAXN, any
+// association variant shared by the batch): one workgroup per system running
+// k_small_body's bodies to the end of its solve (k_small_batch), each with its own
+// control state; the same per-body arithmetic as cdft_converge of that system.
+static bool small_batch_ok(const cdft_sys *s)
+{
+    return s->precision == CDFT_FP64 && small_body_ok(s, false, false);
+}
+
+extern "C++" {
+template <int NC, int AS>
+static void small_batch_launch(int nsys, size_t lds, hipStream_t st, const SBSys *db, int maxbodies,
+                               unsigned long long *ticks)
+{
+    hipLaunchKernelGGL((k_small_batch<NC, AS>), dim3(nsys), dim3(SB_T), lds, st, db, maxbodies, ticks);
+}
+}  // extern "C++"
+
+static int small_batch_run(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho,
+                           const cdft_solver_opts *o, cdft_result *results, bool device_ptrs)
+{
+    hipStream_t st = ctx->stream;
+    const int nc = sys[0]->nc, as = std::min(sys[0]->assoc, 2);
+    for (int i = 1; i < nsys; i++)
+        if (std::min(sys[i]->assoc, 2) != as)
+            return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched FP64 path: the systems must share the association variant");
+    const int hcap = (o->hist && o->hist_cap > 0) ? o->hist_cap : 0;
+    size_t lds = 0;
+    for (int i = 0; i < nsys; i++) lds = std::max(lds, sb_lds_bytes(sys[i]->N, nc));
+    // the largest systems first, as in the register-state launch
+    std::vector<int> order(nsys);
+    for (int i = 0; i < nsys; i++) order[i] = i;
+    std::sort(order.begin(), order.end(),
+              [&](int a, int b) { return sys[a]->N != sys[b]->N ? sys[a]->N > sys[b]->N : a < b; });
+    DevBufs tmp;
+    double *dctl_d = nullptr, *dhist = nullptr, *dsys_d = nullptr, *dticks_d = nullptr;
+    const size_t ctl_words = (sizeof(AACtl) + sizeof(double) - 1) / sizeof(double);
+    const size_t sys_words = (sizeof(SBSys) + sizeof(double) - 1) / sizeof(double);
+    HIPCHECK_CTX(ctx, tmp.get(&dctl_d, ctl_words * nsys));
+    HIPCHECK_CTX(ctx, tmp.get(&dsys_d, sys_words * nsys));
+    HIPCHECK_CTX(ctx, tmp.get(&dticks_d, nsys));
+    if (hcap > 0) HIPCHECK_CTX(ctx, tmp.get(&dhist, (size_t)hcap * nsys));
+    AACtl *dctl = reinterpret_cast<AACtl *>(dctl_d);
+    SBSys *dsys = reinterpret_cast<SBSys *>(dsys_d);
+    unsigned long long *dticks = reinterpret_cast<unsigned long long *>(dticks_d);
+    std::vector<AACtl> hc(nsys);
+    std::vector<SBSys> hs(nsys);
+    for (int q = 0; q < nsys; q++) {
+        cdft_sys *s = sys[order[q]];
+        const long n = (long)s->N * s->nc;
+        aa_ctl_init(&hc[q], o, false, hcap, false, 0);
+        if (device_ptrs)
+            HIPCHECK_CTX(ctx, hipMemcpyAsync(s->rho, rho[order[q]], sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        else
+            HIPCHECK_CTX(ctx, hipMemcpyAsync(s->rho, rho[order[q]], sizeof(double) * n, hipMemcpyHostToDevice, st));
+        hs[q].a = SBArgs{dctl + q, s->h,   s->vext, s->cpd,   s->sol, s->gx,
+                         s->res,   s->df,  s->dg,   s->Q,     s->Qalt, s->DG,
+                         s->xsafe, s->rho, hcap > 0 ? dhist + (size_t)q * hcap : nullptr, s->N, s->nc, s->NA};
+        hs[q].pc = s->pc;
+    }
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(dctl, hc.data(), sizeof(AACtl) * nsys, hipMemcpyHostToDevice, st));
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(dsys, hs.data(), sizeof(SBSys) * nsys, hipMemcpyHostToDevice, st));
+    // every path through aasol takes at most 2 evaluations per iteration counted in
+    // maxit + picard_maxit, plus the first ones (as aasol_run's bound)
+    const long bound = 4L * (2L * o->maxit + 4) + 32;
+    const int maxbodies = (int)std::min<long>(bound, 0x7fffffff);
+    auto attr = [&](const void *k) { return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); };
+    hipError_t e = hipSuccess;
+    HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev0, st));
+    if (nc == 1) {
+        if (as == 0) { e = attr((const void *)k_small_batch<1, 0>); if (e == hipSuccess) small_batch_launch<1, 0>(nsys, lds, st, dsys, maxbodies, dticks); }
+        else if (as == 1) { e = attr((const void *)k_small_batch<1, 1>); if (e == hipSuccess) small_batch_launch<1, 1>(nsys, lds, st, dsys, maxbodies, dticks); }
+        else { e = attr((const void *)k_small_batch<1, 2>); if (e == hipSuccess) small_batch_launch<1, 2>(nsys, lds, st, dsys, maxbodies, dticks); }
+    } else {
+        if (as == 0) { e = attr((const void *)k_small_batch<2, 0>); if (e == hipSuccess) small_batch_launch<2, 0>(nsys, lds, st, dsys, maxbodies, dticks); }
+        else if (as == 1) { e = attr((const void *)k_small_batch<2, 1>); if (e == hipSuccess) small_batch_launch<2, 1>(nsys, lds, st, dsys, maxbodies, dticks); }
+        else { e = attr((const void *)k_small_batch<2, 2>); if (e == hipSuccess) small_batch_launch<2, 2>(nsys, lds, st, dsys, maxbodies, dticks); }
+    }
+    HIPCHECK_CTX(ctx, e);
+    HIPCHECK_CTX(ctx, hipGetLastError());
+    HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev1, st));
+    std::vector<unsigned long long> ticks(nsys);
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(hc.data(), dctl, sizeof(AACtl) * nsys, hipMemcpyDeviceToHost, st));
+    HIPCHECK_CTX(ctx, hipMemcpyAsync(ticks.data(), dticks, sizeof(unsigned long long) * nsys, hipMemcpyDeviceToHost, st));
+    for (int q = 0; q < nsys; q++) {
+        cdft_sys *s = sys[order[q]];
+        const long n = (long)s->N * s->nc;
+        HIPCHECK_CTX(ctx, hipMemcpyAsync(rho[order[q]], s->rho, sizeof(double) * n,
+                                         device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+        if (hcap > 0)
+            HIPCHECK_CTX(ctx, hipMemcpyAsync(o->hist + (size_t)order[q] * hcap, dhist + (size_t)q * hcap,
+                                             sizeof(double) * hcap, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+    float ms = 0.0f;
+    HIPCHECK_CTX(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_kernel_ms = ms;
+    int wall_rate_khz = 0;
+    HIPCHECK_CTX(ctx, hipDeviceGetAttribute(&wall_rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
+    for (int q = 0; q < nsys; q++) {
+        const AACtl &c = hc[q];
+        if (c.phase != AA_DONE) return fail(ctx, CDFT_ERR_HIP, "batched FP64 path: a solve did not terminate");
+        cdft_result &r = results[order[q]];
+        r.idid = c.idid;
+        r.errcode = c.errcode;
+        r.iters = c.k;
+        r.nevals = c.nevals;
+        r.nhist = hcap > 0 ? c.nh : 0;
+        r.resnorm = c.r;
+        r.tol = c.tol;
+        r.nevals_fp32 = 0;
+        r.reserved = 0;
+        r.device_ms = wall_rate_khz > 0 ? (double)ticks[q] / (double)wall_rate_khz : 0.0;
+    }
+    return CDFT_OK;
+}
+
 static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho, const cdft_solver_opts *o,
                       cdft_result *results, bool device_ptrs, bool noalloc)
 {
@@ -1784,22 +1908,39 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
     if (o->maxit < 0) return fail(ctx, CDFT_ERR_ARG, "maxit must be >= 0");
     const int nc = sys[0] ? sys[0]->nc : 0;
     const int prec = sys[0] ? sys[0]->precision : 0;
+    // FP64 batches of small 1D systems: the single-workgroup bodies, one workgroup per
+    // system (the allocating entry points)
+    bool small = !noalloc;
+    for (int i = 0; i < nsys && small; i++) {
+        if (!sys[i] || !rho[i]) return fail(ctx, CDFT_ERR_ARG, std::string(who) + ": NULL system or profile");
+        if (sys[i]->ctx != ctx) return fail(ctx, CDFT_ERR_ARG, "all systems must belong to the context");
+        for (int j = 0; j < i; j++)
+            if (sys[j] == sys[i]) return fail(ctx, CDFT_ERR_ARG, "a system appears twice in the batch");
+        small = small_batch_ok(sys[i]) && sys[i]->nc == nc;
+    }
+    if (small) {
+        HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
+        return small_batch_run(ctx, nsys, sys, rho, o, results, device_ptrs);
+    }
     for (int i = 0; i < nsys; i++) {
         if (!sys[i] || !rho[i]) return fail(ctx, CDFT_ERR_ARG, std::string(who) + ": NULL system or profile");
         if (sys[i]->ctx != ctx) return fail(ctx, CDFT_ERR_ARG, "all systems must belong to the context");
         if (sys[i]->nc != nc) return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched systems must share nc");
-        // the multi-component batched kernel faulted on the GPU and is switched off
-        // until that is understood
+        // the multi-component register/HBM-state kernel faulted on the GPU and is
+        // switched off until that is understood (FP64 batches of up to two components
+        // take the single-workgroup bodies above)
         if (sys[i]->nc != 1)
             return fail(ctx, CDFT_ERR_UNSUPPORTED,
-                        "batched path: one-component systems only for now; use cdft_converge per system");
+                        "batched path: multi-component batches run in FP64 (CDFT_FP64 systems of at most 512 "
+                        "points x components, two components at most); use cdft_converge per system otherwise");
         if (sys[i]->precision != prec) return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched systems must share precision");
-        // a launch running the FP64 map from the first evaluation faulted on the GPU;
-        // switched off until that is understood (mixed precision reaches the FP64 map
-        // after its precision switch)
+        // the register-state kernel running the FP64 map from the first evaluation
+        // faulted on the GPU; switched off until that is understood (FP64 batches of
+        // small systems take the single-workgroup bodies above)
         if (sys[i]->precision != CDFT_MIXED)
             return fail(ctx, CDFT_ERR_UNSUPPORTED,
-                        "batched path: mixed-precision systems only for now; use cdft_converge for FP64");
+                        "batched path: FP64 batches take 1D systems of at most 512 points x components "
+                        "(two components at most, one association variant); use cdft_converge otherwise");
         // the kernels with the Anderson state in HBM share the code of the faulting
         // multi-component kernel and have not run on a GPU: only the register-state
         // kernel (at most 512 points, m <= 5) is enabled

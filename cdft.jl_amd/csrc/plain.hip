@@ -1087,10 +1087,9 @@ __device__ __forceinline__ void sb_sum(double (&v)[NQ], double *wred, double *ou
 }
 
 template <int NC, int AS>
-__global__ __launch_bounds__(SB_T) void k_small_body(SBArgs A, PhiConst P, int nbody)
+__device__ __forceinline__ void sb_run(const SBArgs &A, const PhiConst &P, int nbody, double *sbm)
 {
     constexpr int F = SB_F;
-    extern __shared__ __align__(16) double sbm[];
     __shared__ AACtl C;
     __shared__ double wred[(SB_T / 64) * 16];
     __shared__ double fin[2 * CDFT_NQ + 4];
@@ -1321,7 +1320,52 @@ __global__ __launch_bounds__(SB_T) void k_small_body(SBArgs A, PhiConst P, int n
     }
 }
 
-// Dynamic LDS bytes of k_small_body.
+// Up to nbody bodies of one solve (cdft_converge's plain path).
+template <int NC, int AS>
+__global__ __launch_bounds__(SB_T) void k_small_body(SBArgs A, PhiConst P, int nbody)
+{
+    extern __shared__ __align__(16) double sbm[];
+    sb_run<NC, AS>(A, P, nbody, sbm);
+}
+
+// One system of a batch (cdft_converge_batch of FP64 systems): its buffers and constants.
+struct SBSys {
+    SBArgs a;
+    PhiConst pc;
+};
+
+// The batched form: workgroup b solves system b from its seed (x0 = log rho, the
+// start of converge!, converge.jl:244) to the end, at most maxbodies bodies, and
+// leaves rho = exp(x) and its control state (errcode, counts) in its own buffers;
+// ticks[b] is its time in the workgroup (wall clock).
+template <int NC, int AS>
+__global__ __launch_bounds__(SB_T) void k_small_batch(const SBSys *__restrict__ sys, int maxbodies,
+                                                      unsigned long long *ticks)
+{
+    extern __shared__ __align__(16) double sbm[];
+    __shared__ PhiConst P;
+    const unsigned long long t0 = wall_clock64();
+    const SBSys *S = sys + blockIdx.x;
+    {
+        const int *src = reinterpret_cast<const int *>(&S->pc);
+        int *dst = reinterpret_cast<int *>(&P);
+        for (int i = threadIdx.x; i < (int)(sizeof(PhiConst) / sizeof(int)); i += SB_T) dst[i] = src[i];
+    }
+    const SBArgs A = S->a;
+    const int n = A.N * NC;
+    for (int t = threadIdx.x; t < n; t += SB_T) {
+        const double x = log(A.rho[t]);
+        A.sol[t] = x;
+        A.rho[t] = exp(x);
+    }
+    __syncthreads();
+    sb_run<NC, AS>(A, P, maxbodies, sbm);
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += SB_T) A.rho[t] = exp(A.sol[t]);
+    if (threadIdx.x == 0) ticks[blockIdx.x] = wall_clock64() - t0;
+}
+
+// Dynamic LDS bytes of k_small_body / k_small_batch.
 __host__ __device__ inline size_t sb_lds_bytes(int N, int nc)
 {
     return sizeof(double) * ((size_t)nc * SB_F * N + (size_t)nc * (SB_F - 1) * N + (size_t)nc * (SB_F - 1) * sb_le(N));
