@@ -7,6 +7,9 @@ OpenMP beside it.
 
 A step is one batched solve of all pores from their seed profiles (inputs resident
 in HBM).  Prints one JSON line.  Besides the headline workload the line carries:
+  plain_fp64   the plain FP64 path on the same pores, one after another;
+  batched_fp64 the same FP64 pores in one launch, a workgroup per pore running the
+               plain path's single-workgroup bodies (bit for bit the plain solves);
   termination  per-pore errcodes of the batched, plain and oracle solves, the pores
                where they differ, and the batched and plain results' largest
                relative differences from the oracle (profile, adsorption, tension);
@@ -122,6 +125,36 @@ def run_plain(args, pores):
         out.append(rho.reshape(-1))
         res.append(s.last_result)
     return out, res, t
+
+
+def run_batched_fp64(args, np, torch, _lib, pores, plain_profiles, plain_results):
+    """The FP64 pores in one batched launch (one workgroup per pore running the plain
+    path's single-workgroup bodies), inputs resident on the device, one warm-up; and
+    how many pores give exactly the plain path's bits and counts."""
+    systems = [s.device_system for s, _ in pores]
+    ctx = systems[0].ctx
+    offs = np.concatenate([[0], np.cumsum([s.N * s.nc for s in systems])])
+    seeds = torch.tensor(np.concatenate([r.reshape(-1) for _, r in pores]), dtype=torch.float64, device="cuda")
+    work = torch.empty_like(seeds)
+    ptrs = [work.data_ptr() + 8 * int(o) for o in offs[:-1]]
+    res = None
+    for it in range(2):
+        work.copy_(seeds)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = _lib.converge_batch(ctx, systems, ptrs, tol=args.tol, device_ptrs=True)
+        dt = time.perf_counter() - t0
+    prof = work.cpu().numpy()
+    same = sum(1 for i in range(len(systems))
+               if np.array_equal(prof[offs[i]:offs[i + 1]], plain_profiles[i])
+               and res[i]["nevals"] == plain_results[i]["nevals"] and res[i]["errcode"] == plain_results[i]["errcode"])
+    conv = sum(1 for r in res if r["errcode"] == 0)
+    dev = np.array([r["device_ms"] for r in res])
+    return {"value": conv / dt, "unit": "profiles/s", "seconds": dt, "kernel_ms": ctx.last_kernel_ms(),
+            "converged": conv, "identical_to_plain_fp64": same, "slowest_pore_ms": float(dev.max()),
+            "slowest_pore": int(dev.argmax()), "p50_pore_ms": float(np.percentile(dev, 50)),
+            "sample": f"all {len(systems)} pores in one launch (k_small_batch: a workgroup per pore, "
+                      "the plain path's single-workgroup bodies), FP64"}
 
 
 def run_cpu_baseline(args, np, pores, mus):
@@ -539,6 +572,8 @@ def main():
         out["plain_fp64"] = {"value": plain_rate, "unit": "profiles/s",
                              "sample": f"all {args.npores} pores, one after another", "seconds": t_plain,
                              "converged": nplain, "speedup_batched_vs_plain": value / plain_rate}
+        # ---------------- the same FP64 pores batched (one workgroup each)
+        out["batched_fp64"] = run_batched_fp64(args, np, torch, _lib, plain_pores, pprof, pres)
         # ---------------- CPU oracle, OpenMP over all pores and one thread on a sample
         cpu = run_cpu_baseline(args, np, plain_pores, mus)
         ncpu = sum(1 for c in cpu["errcodes"] if c == 0)
