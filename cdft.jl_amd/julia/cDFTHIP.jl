@@ -138,6 +138,33 @@ end
 
 const HIPSystem = cDFT.DFTSystem{<:Clapeyron.PCSAFTModel,<:Any,<:Any,<:Any,<:Any,<:Any,<:cDFT.DFTOptions{HIPNative}}
 
+# Device systems kept across converge! calls of the same DFTSystem object (keyed by
+# objectid; at most SYS_CACHE_MAX, the oldest destroyed first), so repeated solves do
+# not rebuild the kernels.  A system changed in place needs clear_cache().
+const SYS_CACHE_MAX = 16
+const SYS_CACHE = Vector{Pair{UInt,Ptr{Cvoid}}}()
+
+function _cached_system(system, ctx)
+    key = objectid(system)
+    for (k, h) in SYS_CACHE
+        k == key && return h
+    end
+    h = _system(system, ctx)
+    push!(SYS_CACHE, key => h)
+    if length(SYS_CACHE) > SYS_CACHE_MAX
+        _, old = popfirst!(SYS_CACHE)
+        ccall((:cdft_system_destroy, libcdft), Cint, (Ptr{Cvoid},), old)
+    end
+    return h
+end
+
+"""Destroy the device systems kept for earlier converge! calls."""
+function clear_cache()
+    foreach(p -> ccall((:cdft_system_destroy, libcdft), Cint, (Ptr{Cvoid},), p.second), SYS_CACHE)
+    empty!(SYS_CACHE)
+    return nothing
+end
+
 # F_ideal on the host (cDFT's own, BasicIdeal.jl:10-34) + F_res from the library
 function _free_energy(system, sys, ctx, r)
     fres = Ref{Cdouble}(0.0)
@@ -176,8 +203,8 @@ end
 function cDFT.converge!(prob::cDFT.DFTProblem{<:HIPSystem}, method::cDFT.AASol, rho::AbstractArray)
     system = prob.system
     ctx = _ctx(system.options.device.device)
-    sys = _system(system, ctx)
-    try
+    sys = _cached_system(system, ctx)
+    begin
         every = prob.log_interval > 0 && prob.save_interval > 0 ? gcd(prob.log_interval, prob.save_interval) :
                 max(prob.log_interval, prob.save_interval)
         st = _Progress(prob, sys, ctx, size(rho), nothing)
@@ -200,8 +227,6 @@ function cDFT.converge!(prob::cDFT.DFTProblem{<:HIPSystem}, method::cDFT.AASol, 
             H = _free_energy(system, sys, ctx, buf)
             @info "$msg after $(r.nevals) iterations: err = $(round(r.resnorm; sigdigits=3)) | F = $(round(H; sigdigits=6))"
         end
-    finally
-        ccall((:cdft_system_destroy, libcdft), Cint, (Ptr{Cvoid},), sys)
     end
     return nothing
 end
@@ -292,6 +317,6 @@ function adjoint_params(system, rho::AbstractArray, g::AbstractMatrix, pairs::Ab
     return lam, permutedims(out[1:length(pairs), :])
 end
 
-export HIPNative, converge_batch!, phi_vjp, adjoint_solve, adjoint_params
+export HIPNative, converge_batch!, phi_vjp, adjoint_solve, adjoint_params, clear_cache
 
 end # module
