@@ -161,7 +161,9 @@ constexpr int TB3 = 8;
 // Radix-4 Stockham passes (+ a final radix-2 when log2 L is odd) along j over a tile
 // of TB lines in LDS laid out as in k_fft3 (axis AX), in place.  All 256 threads;
 // the tile must be complete before the call (barrier) and is complete after it.
-// EPT = radix-4 butterflies per thread and pass (L TB / 1024, at least 1).
+// EPT = radix-4 butterflies per thread and pass (L TB / 1024, at least 1).  tw: the
+// line's twiddles W_L^t (staged in LDS by the caller).  The first pass's twiddles are
+// all W^0 = 1 and are skipped (the explicit-fma product by (1, 0) returns its operand).
 template <typename T, int AX, int EPT, int TB = TB3>
 __device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const cx<double> *tw, int L, int lg,
                                                                bool inverse)
@@ -191,7 +193,7 @@ __device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const c
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     cx<T> x = a[lds(jj + r * nb, b)];
-                    if (r) {
+                    if (r && pass > 0) {
                         const cx<double> wd = tw[(r * k * step) & (L - 1)];
                         x = cxmul(x, cx<T>{T(wd.x), inverse ? T(-wd.y) : T(wd.y)});
                     }
@@ -331,8 +333,10 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
             a[lds(j, b)] = v;
         }
     }
+    cx<double> *twl = reinterpret_cast<cx<double> *>(smem3 + (size_t)L * TB * sizeof(cx<T>));
+    for (int t = tid; t < L; t += 256) twl[t] = p.tw[t];
     __syncthreads();
-    tile_fft<T, AX, EPT, TB>(a, p.tw, L, lg, p.inverse != 0);
+    tile_fft<T, AX, EPT, TB>(a, twl, L, lg, p.inverse != 0);
     // ---- store
 #pragma unroll
     for (int q = 0; q < EL; q++) {
@@ -403,6 +407,8 @@ __global__ __launch_bounds__(256) void k_fft3z_accum(Grid3 G, ZPairs3 p)
     const int L = G.n[2], lg = G.lg[2];
     cx<T> *a0 = reinterpret_cast<cx<T> *>(smem3);     // S
     cx<T> *a1 = a0 + (size_t)L * TB;                    // one pair's line tile
+    cx<double> *twl = reinterpret_cast<cx<double> *>(a1 + (size_t)L * TB);   // W_L^t
+    for (int t = tid; t < L; t += 256) twl[t] = p.tw[t];
     const int n0 = G.n[0], tiles_x = n0 / TB;
     const int x0 = (blockIdx.x % tiles_x) * TB, y0 = blockIdx.x / tiles_x;
     const long base = x0 + (long)n0 * y0, lstride = (long)n0 * G.n[1];
@@ -420,7 +426,7 @@ __global__ __launch_bounds__(256) void k_fft3z_accum(Grid3 G, ZPairs3 p)
             a1[l] = reinterpret_cast<const cx<T> *>(p.b[k])[g];
         }
         __syncthreads();
-        tile_fft<T, 2, EPT, TBZ>(a1, p.tw, L, lg, false);
+        tile_fft<T, 2, EPT, TBZ>(a1, twl, L, lg, false);
 #pragma unroll
         for (int q = 0; q < EL; q++) {
             int l;
@@ -430,7 +436,7 @@ __global__ __launch_bounds__(256) void k_fft3z_accum(Grid3 G, ZPairs3 p)
         }
         __syncthreads();                                 // a1 is rewritten by the next pair; a0 complete
     }
-    tile_fft<T, 2, EPT, TBZ>(a0, p.tw, L, lg, true);
+    tile_fft<T, 2, EPT, TBZ>(a0, twl, L, lg, true);
 #pragma unroll
     for (int q = 0; q < EL; q++) {
         int l;
