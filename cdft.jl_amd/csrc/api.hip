@@ -131,7 +131,7 @@ void fft3_launch(const cdft_sys *s, const Pass3 &p)
 {
     const Grid3 &G = s->g3;
     const int L = G.n[AX];
-    const size_t lds = (size_t)L * TB * sizeof(cx<T>) + (size_t)L * sizeof(cx<double>);   // tile, twiddles
+    const size_t lds = (size_t)L * (TB + 1) * sizeof(cx<T>);   // tile, twiddles
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void *)k_fft3<T, AX, IN, OUT, EPT, TB>,
@@ -222,7 +222,7 @@ template <typename T, int EPT>
 void zpairs_launch_ept(const cdft_sys *s, const ZPairs3 &p)
 {
     const Grid3 &G = s->g3;
-    const size_t lds = (size_t)2 * G.n[2] * TBZ * sizeof(cx<T>) + (size_t)G.n[2] * sizeof(cx<double>);
+    const size_t lds = (size_t)G.n[2] * (2 * TBZ + 1) * sizeof(cx<T>);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void *)k_fft3z_accum<T, EPT>,

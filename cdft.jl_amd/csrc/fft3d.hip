@@ -162,10 +162,10 @@ constexpr int TB3 = 8;
 // of TB lines in LDS laid out as in k_fft3 (axis AX), in place.  All 256 threads;
 // the tile must be complete before the call (barrier) and is complete after it.
 // EPT = radix-4 butterflies per thread and pass (L TB / 1024, at least 1).  tw: the
-// line's twiddles W_L^t (staged in LDS by the caller).  The first pass's twiddles are
+// line's twiddles W_L^t in T (staged in LDS by the caller).  The first pass's twiddles are
 // all W^0 = 1 and are skipped (the explicit-fma product by (1, 0) returns its operand).
 template <typename T, int AX, int EPT, int TB = TB3>
-__device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const cx<double> *tw, int L, int lg,
+__device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const cx<T> *tw, int L, int lg,
                                                                bool inverse)
 {
     constexpr int EPR = EPT > 0 ? EPT : 1;
@@ -194,8 +194,8 @@ __device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const c
                 for (int r = 0; r < 4; r++) {
                     cx<T> x = a[lds(jj + r * nb, b)];
                     if (r && pass > 0) {
-                        const cx<double> wd = tw[(r * k * step) & (L - 1)];
-                        x = cxmul(x, cx<T>{T(wd.x), inverse ? T(-wd.y) : T(wd.y)});
+                        const cx<T> wd = tw[(r * k * step) & (L - 1)];
+                        x = cxmul(x, cx<T>{wd.x, inverse ? -wd.y : wd.y});
                     }
                     v[q][r] = x;
                 }
@@ -237,8 +237,8 @@ __device__ __attribute__((always_inline)) inline void tile_fft(cx<T> *a, const c
                 int jj, b;
                 split(e, lg - 1, jj, b);
                 const int k = jj & (Ns - 1);
-                const cx<double> wd = tw[(k * (L / (Ns * 2))) & (L - 1)];
-                const cx<T> w = {T(wd.x), inverse ? T(-wd.y) : T(wd.y)};
+                const cx<T> wd = tw[(k * (L / (Ns * 2))) & (L - 1)];
+                const cx<T> w = {wd.x, inverse ? -wd.y : wd.y};
                 const cx<T> xa = a[lds(jj, b)], xb = cxmul(a[lds(jj + nb, b)], w);
                 v[q][0] = {xa.x + xb.x, xa.y + xb.y};
                 v[q][1] = {xa.x - xb.x, xa.y - xb.y};
@@ -333,8 +333,8 @@ __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
             a[lds(j, b)] = v;
         }
     }
-    cx<double> *twl = reinterpret_cast<cx<double> *>(smem3 + (size_t)L * TB * sizeof(cx<T>));
-    for (int t = tid; t < L; t += 256) twl[t] = p.tw[t];
+    cx<T> *twl = a + (size_t)L * TB;                   // W_L^t in T (rounded once, as at each use)
+    for (int t = tid; t < L; t += 256) twl[t] = cx<T>{T(p.tw[t].x), T(p.tw[t].y)};
     __syncthreads();
     tile_fft<T, AX, EPT, TB>(a, twl, L, lg, p.inverse != 0);
     // ---- store
@@ -407,8 +407,8 @@ __global__ __launch_bounds__(256) void k_fft3z_accum(Grid3 G, ZPairs3 p)
     const int L = G.n[2], lg = G.lg[2];
     cx<T> *a0 = reinterpret_cast<cx<T> *>(smem3);     // S
     cx<T> *a1 = a0 + (size_t)L * TB;                    // one pair's line tile
-    cx<double> *twl = reinterpret_cast<cx<double> *>(a1 + (size_t)L * TB);   // W_L^t
-    for (int t = tid; t < L; t += 256) twl[t] = p.tw[t];
+    cx<T> *twl = a1 + (size_t)L * TB;                   // W_L^t in T
+    for (int t = tid; t < L; t += 256) twl[t] = cx<T>{T(p.tw[t].x), T(p.tw[t].y)};
     const int n0 = G.n[0], tiles_x = n0 / TB;
     const int x0 = (blockIdx.x % tiles_x) * TB, y0 = blockIdx.x / tiles_x;
     const long base = x0 + (long)n0 * y0, lstride = (long)n0 * G.n[1];
