@@ -1068,8 +1068,9 @@ static void aa_ctl_init(AACtl *hc, const cdft_solver_opts *o, bool mixed, int hc
 // converge!; CDFT_SMALL_BODY=0 keeps them on the multi-kernel body.
 static bool small_body_ok(const cdft_sys *s, bool adjoint, bool mixed)
 {
-    if (adjoint || mixed || s->nd != 1 || s->coord || s->nc > 2 || s->F != SB_F || (long)s->N * s->nc > SB_MAXN)
+    if (adjoint || s->nd != 1 || s->coord || s->nc > 2 || s->F != SB_F || (long)s->N * s->nc > SB_MAXN)
         return false;
+    if (mixed && !s->h32) return false;
     const char *e = std::getenv("CDFT_SMALL_BODY");
     return !(e && e[0] == '0');
 }
@@ -1077,32 +1078,46 @@ static bool small_body_ok(const cdft_sys *s, bool adjoint, bool mixed)
 extern "C++" {
 // Dynamic LDS above the default needs the kernel's attribute raised first (to what
 // this system needs; the static LDS comes on top of it).
-template <int NC, int AS>
+template <int NC, int AS, bool MIXED>
 static hipError_t small_body_launch(const cdft_sys *s, const SBArgs &a, int nbody)
 {
     static size_t attr = 0;
-    const size_t lds = sb_lds_bytes(s->N, s->nc);
+    const size_t lds = sb_lds_bytes(s->N, s->nc, MIXED);
     if (lds > attr) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_small_body<NC, AS>,
+        const hipError_t e = hipFuncSetAttribute((const void *)k_small_body<NC, AS, MIXED>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         attr = lds;
     }
-    hipLaunchKernelGGL((k_small_body<NC, AS>), dim3(1), dim3(SB_T), lds, s->ctx->stream, a, s->pc, nbody);
+    hipLaunchKernelGGL((k_small_body<NC, AS, MIXED>), dim3(1), dim3(SB_T), lds, s->ctx->stream, a, s->pc, nbody);
     return hipSuccess;
+}
+
+template <bool MIXED>
+static hipError_t small_body_m(const cdft_sys *s, const SBArgs &a, int nbody)
+{
+    const int as = s->assoc < 2 ? s->assoc : 2;
+    if (s->nc == 1)
+        return as == 0 ? small_body_launch<1, 0, MIXED>(s, a, nbody)
+                       : (as == 1 ? small_body_launch<1, 1, MIXED>(s, a, nbody)
+                                  : small_body_launch<1, 2, MIXED>(s, a, nbody));
+    return as == 0 ? small_body_launch<2, 0, MIXED>(s, a, nbody)
+                   : (as == 1 ? small_body_launch<2, 1, MIXED>(s, a, nbody)
+                              : small_body_launch<2, 2, MIXED>(s, a, nbody));
 }
 }  // extern "C++"
 
+static SBArgs sb_args(const cdft_sys *s, AACtl *c, double *hist)
+{
+    return SBArgs{c,      s->h,   s->vext, s->cpd, s->sol, s->gx,    s->res, s->df, s->dg, s->Q,
+                  s->Qalt, s->DG, s->xsafe, s->rho, hist,  s->N,     s->nc,  s->NA,
+                  s->precision == CDFT_MIXED ? s->h32 : nullptr};
+}
+
 static hipError_t small_body(const cdft_sys *s, AACtl *c, int nbody)
 {
-    const SBArgs a{c,     s->h,  s->vext, s->cpd,   s->sol, s->gx,    s->res, s->df,  s->dg,
-                   s->Q,  s->Qalt, s->DG, s->xsafe, s->rho, s->dhist, s->N,   s->nc,  s->NA};
-    const int as = s->assoc < 2 ? s->assoc : 2;
-    if (s->nc == 1)
-        return as == 0 ? small_body_launch<1, 0>(s, a, nbody)
-                       : (as == 1 ? small_body_launch<1, 1>(s, a, nbody) : small_body_launch<1, 2>(s, a, nbody));
-    return as == 0 ? small_body_launch<2, 0>(s, a, nbody)
-                   : (as == 1 ? small_body_launch<2, 1>(s, a, nbody) : small_body_launch<2, 2>(s, a, nbody));
+    const SBArgs a = sb_args(s, c, s->dhist);
+    return s->precision == CDFT_MIXED ? small_body_m<true>(s, a, nbody) : small_body_m<false>(s, a, nbody);
 }
 
 // aasol (src/utils/anderson.jl:301-443) entirely on the device: every N-length
@@ -1782,15 +1797,32 @@ static int batched_prepare(cdft_ctx *ctx, cdft_sys *s)
 // control state; the same per-body arithmetic as cdft_converge of that system.
 static bool small_batch_ok(const cdft_sys *s)
 {
-    return s->precision == CDFT_FP64 && small_body_ok(s, false, false);
+    return small_body_ok(s, false, s->precision == CDFT_MIXED);
 }
 
 extern "C++" {
-template <int NC, int AS>
-static void small_batch_launch(int nsys, size_t lds, hipStream_t st, const SBSys *db, int maxbodies,
-                               unsigned long long *ticks)
+template <int NC, int AS, bool MIXED>
+static hipError_t small_batch_launch(int nsys, size_t lds, hipStream_t st, const SBSys *db, int maxbodies,
+                                     unsigned long long *ticks)
 {
-    hipLaunchKernelGGL((k_small_batch<NC, AS>), dim3(nsys), dim3(SB_T), lds, st, db, maxbodies, ticks);
+    const hipError_t e = hipFuncSetAttribute((const void *)k_small_batch<NC, AS, MIXED>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_small_batch<NC, AS, MIXED>), dim3(nsys), dim3(SB_T), lds, st, db, maxbodies, ticks);
+    return hipSuccess;
+}
+
+template <bool MIXED>
+static hipError_t small_batch_m(int nc, int as, int nsys, size_t lds, hipStream_t st, const SBSys *db, int maxbodies,
+                                unsigned long long *ticks)
+{
+    if (nc == 1)
+        return as == 0 ? small_batch_launch<1, 0, MIXED>(nsys, lds, st, db, maxbodies, ticks)
+                       : (as == 1 ? small_batch_launch<1, 1, MIXED>(nsys, lds, st, db, maxbodies, ticks)
+                                  : small_batch_launch<1, 2, MIXED>(nsys, lds, st, db, maxbodies, ticks));
+    return as == 0 ? small_batch_launch<2, 0, MIXED>(nsys, lds, st, db, maxbodies, ticks)
+                   : (as == 1 ? small_batch_launch<2, 1, MIXED>(nsys, lds, st, db, maxbodies, ticks)
+                              : small_batch_launch<2, 2, MIXED>(nsys, lds, st, db, maxbodies, ticks));
 }
 }  // extern "C++"
 
@@ -1803,8 +1835,9 @@ static int small_batch_run(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double
         if (std::min(sys[i]->assoc, 2) != as)
             return fail(ctx, CDFT_ERR_UNSUPPORTED, "batched FP64 path: the systems must share the association variant");
     const int hcap = (o->hist && o->hist_cap > 0) ? o->hist_cap : 0;
+    const bool mixed = (sys[0]->precision == CDFT_MIXED);
     size_t lds = 0;
-    for (int i = 0; i < nsys; i++) lds = std::max(lds, sb_lds_bytes(sys[i]->N, nc));
+    for (int i = 0; i < nsys; i++) lds = std::max(lds, sb_lds_bytes(sys[i]->N, nc, mixed));
     // the largest systems first, as in the register-state launch
     std::vector<int> order(nsys);
     for (int i = 0; i < nsys; i++) order[i] = i;
@@ -1826,14 +1859,12 @@ static int small_batch_run(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double
     for (int q = 0; q < nsys; q++) {
         cdft_sys *s = sys[order[q]];
         const long n = (long)s->N * s->nc;
-        aa_ctl_init(&hc[q], o, false, hcap, false, 0);
+        aa_ctl_init(&hc[q], o, mixed, hcap, false, 0);
         if (device_ptrs)
             HIPCHECK_CTX(ctx, hipMemcpyAsync(s->rho, rho[order[q]], sizeof(double) * n, hipMemcpyDeviceToDevice, st));
         else
             HIPCHECK_CTX(ctx, hipMemcpyAsync(s->rho, rho[order[q]], sizeof(double) * n, hipMemcpyHostToDevice, st));
-        hs[q].a = SBArgs{dctl + q, s->h,   s->vext, s->cpd,   s->sol, s->gx,
-                         s->res,   s->df,  s->dg,   s->Q,     s->Qalt, s->DG,
-                         s->xsafe, s->rho, hcap > 0 ? dhist + (size_t)q * hcap : nullptr, s->N, s->nc, s->NA};
+        hs[q].a = sb_args(s, dctl + q, hcap > 0 ? dhist + (size_t)q * hcap : nullptr);
         hs[q].pc = s->pc;
     }
     HIPCHECK_CTX(ctx, hipMemcpyAsync(dctl, hc.data(), sizeof(AACtl) * nsys, hipMemcpyHostToDevice, st));
@@ -1842,19 +1873,9 @@ static int small_batch_run(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double
     // maxit + picard_maxit, plus the first ones (as aasol_run's bound)
     const long bound = 4L * (2L * o->maxit + 4) + 32;
     const int maxbodies = (int)std::min<long>(bound, 0x7fffffff);
-    auto attr = [&](const void *k) { return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); };
-    hipError_t e = hipSuccess;
     HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev0, st));
-    if (nc == 1) {
-        if (as == 0) { e = attr((const void *)k_small_batch<1, 0>); if (e == hipSuccess) small_batch_launch<1, 0>(nsys, lds, st, dsys, maxbodies, dticks); }
-        else if (as == 1) { e = attr((const void *)k_small_batch<1, 1>); if (e == hipSuccess) small_batch_launch<1, 1>(nsys, lds, st, dsys, maxbodies, dticks); }
-        else { e = attr((const void *)k_small_batch<1, 2>); if (e == hipSuccess) small_batch_launch<1, 2>(nsys, lds, st, dsys, maxbodies, dticks); }
-    } else {
-        if (as == 0) { e = attr((const void *)k_small_batch<2, 0>); if (e == hipSuccess) small_batch_launch<2, 0>(nsys, lds, st, dsys, maxbodies, dticks); }
-        else if (as == 1) { e = attr((const void *)k_small_batch<2, 1>); if (e == hipSuccess) small_batch_launch<2, 1>(nsys, lds, st, dsys, maxbodies, dticks); }
-        else { e = attr((const void *)k_small_batch<2, 2>); if (e == hipSuccess) small_batch_launch<2, 2>(nsys, lds, st, dsys, maxbodies, dticks); }
-    }
-    HIPCHECK_CTX(ctx, e);
+    HIPCHECK_CTX(ctx, mixed ? small_batch_m<true>(nc, as, nsys, lds, st, dsys, maxbodies, dticks)
+                            : small_batch_m<false>(nc, as, nsys, lds, st, dsys, maxbodies, dticks));
     HIPCHECK_CTX(ctx, hipGetLastError());
     HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev1, st));
     std::vector<unsigned long long> ticks(nsys);
@@ -1886,7 +1907,7 @@ static int small_batch_run(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double
         r.nhist = hcap > 0 ? c.nh : 0;
         r.resnorm = c.r;
         r.tol = c.tol;
-        r.nevals_fp32 = 0;
+        r.nevals_fp32 = c.nevals32;
         r.reserved = 0;
         r.device_ms = wall_rate_khz > 0 ? (double)ticks[q] / (double)wall_rate_khz : 0.0;
     }
@@ -1910,15 +1931,22 @@ static int batch_impl(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *con
     const int prec = sys[0] ? sys[0]->precision : 0;
     // FP64 batches of small 1D systems: the single-workgroup bodies, one workgroup per
     // system (the allocating entry points)
-    bool small = !noalloc;
-    for (int i = 0; i < nsys && small; i++) {
+    // Routes: one-component mixed batches the register-state kernel takes go there (the
+    // headline kernel); other batches of small 1D systems (FP64, or mixed with two
+    // components, association or a deeper history) take the single-workgroup bodies, one
+    // workgroup per system (the allocating entry points)
+    bool small = !noalloc, reg = true;
+    for (int i = 0; i < nsys && (small || reg); i++) {
         if (!sys[i] || !rho[i]) return fail(ctx, CDFT_ERR_ARG, std::string(who) + ": NULL system or profile");
         if (sys[i]->ctx != ctx) return fail(ctx, CDFT_ERR_ARG, "all systems must belong to the context");
         for (int j = 0; j < i; j++)
             if (sys[j] == sys[i]) return fail(ctx, CDFT_ERR_ARG, "a system appears twice in the batch");
-        small = small_batch_ok(sys[i]) && sys[i]->nc == nc;
+        const cdft_sys *x = sys[i];
+        small = small && small_batch_ok(x) && x->nc == nc && x->precision == prec;
+        reg = reg && x->nc == 1 && x->precision == CDFT_MIXED && x->N <= kRegNPT * BT && o->m <= kRegAM &&
+              x->nd == 1 && !x->coord && !x->assoc;
     }
-    if (small) {
+    if (small && !reg) {
         HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
         return small_batch_run(ctx, nsys, sys, rho, o, results, device_ptrs);
     }

@@ -1018,6 +1018,7 @@ struct SBArgs {
     double *sol, *gx, *res, *df, *dg, *Q0, *Q1, *DG, *xsafe, *rho, *hist;
     int N, nc;
     double NA;
+    const float *h32;              // the mixed-precision kernels' FP32 copy of h (or null)
 };
 
 // Extended periodic source ue[j] = u[j mod N], j < Le, stored decimated by four:
@@ -1025,7 +1026,8 @@ struct SBArgs {
 // four consecutive outputs per lane read consecutive addresses across a wave.
 __host__ __device__ inline int sb_le(int N) { return 4 * ((2 * N + 4 + 3) / 4); }
 
-__device__ __forceinline__ void sb_put(double *ud, int Lq, int N, int i, double v)
+template <typename T>
+__device__ __forceinline__ void sb_put(T *ud, int Lq, int N, int i, T v)
 {
     for (int j = i; j < 4 * Lq; j += N) ud[(j & 3) * Lq + (j >> 2)] = v;
 }
@@ -1066,6 +1068,61 @@ __device__ __forceinline__ void sb_conv4(const double *__restrict__ ud, int Lq, 
     a[0] = a0; a[1] = a1; a[2] = a2; a[3] = a3;
 }
 
+// The FP32 map's form (k_circ1d's compensated sums): blocks of eight products summed
+// plainly, each block sum added with Kahan's correction; the sum is a_r - cc_r.
+__device__ __forceinline__ void sb_conv4c(const float *__restrict__ ud, int Lq, const float *__restrict__ hk, int N,
+                                          int b, float (&a)[4], float (&cc)[4])
+{
+    auto U = [&](int j) __attribute__((always_inline)) { return ud[(j & 3) * Lq + (j >> 2)]; };
+    const int j0 = 4 * b + N;
+    float w0 = U(j0), w1 = U(j0 + 1), w2 = U(j0 + 2), w3 = U(j0 + 3);
+    const float *p0 = ud + ((j0 - 1) & 3) * Lq + ((j0 - 1) >> 2);
+    const float *p1 = ud + ((j0 - 2) & 3) * Lq + ((j0 - 2) >> 2);
+    const float *p2 = ud + ((j0 - 3) & 3) * Lq + ((j0 - 3) >> 2);
+    const float *p3 = ud + ((j0 - 4) & 3) * Lq + ((j0 - 4) >> 2);
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+    float b0, b1, b2, b3;
+    // four taps k .. k + 3 (window index q = k / 4) into the block sums
+    auto step4 = [&](int k, int q) __attribute__((always_inline)) {
+        const float h0 = hk[k], h1 = hk[k + 1], h2 = hk[k + 2], h3 = hk[k + 3];
+        b0 += h0 * w0; b1 += h0 * w1; b2 += h0 * w2; b3 += h0 * w3;
+        const float x = p0[-q];
+        b0 += h1 * x; b1 += h1 * w0; b2 += h1 * w1; b3 += h1 * w2;
+        const float y = p1[-q];
+        b0 += h2 * y; b1 += h2 * x; b2 += h2 * w0; b3 += h2 * w1;
+        const float z = p2[-q];
+        b0 += h3 * z; b1 += h3 * y; b2 += h3 * x; b3 += h3 * w0;
+        const float u = p3[-q];                        // >= 4 b >= 0
+        w3 = x; w2 = y; w1 = z; w0 = u;
+    };
+    auto fold = [&]() __attribute__((always_inline)) {
+        kahan_add(s0, c0, b0); kahan_add(s1, c1, b1); kahan_add(s2, c2, b2); kahan_add(s3, c3, b3);
+    };
+    int k = 0;
+    for (int q = 0; k + 8 <= N; k += 8, q += 2) {
+        b0 = b1 = b2 = b3 = 0.f;
+        step4(k, q);
+        step4(k + 4, q + 1);
+        fold();
+    }
+    if (k < N) {
+        b0 = b1 = b2 = b3 = 0.f;
+        if (k + 4 <= N) {
+            step4(k, k >> 2);
+            k += 4;
+        }
+        for (; k < N; k++) {
+            const float hv = hk[k];
+            b0 += hv * w0; b1 += hv * w1; b2 += hv * w2; b3 += hv * w3;
+            w3 = w2; w2 = w1; w1 = w0;
+            w0 = U(j0 - k - 1);                        // >= 4 b >= 0
+        }
+        fold();
+    }
+    a[0] = s0; a[1] = s1; a[2] = s2; a[3] = s3;
+    cc[0] = c0; cc[1] = c1; cc[2] = c2; cc[3] = c3;
+}
+
 // Workgroup sums of NQ values into out[0..NQ) (LDS), waves added in order.
 template <int NQ>
 __device__ __forceinline__ void sb_sum(double (&v)[NQ], double *wred, double *out)
@@ -1086,7 +1143,7 @@ __device__ __forceinline__ void sb_sum(double (&v)[NQ], double *wred, double *ou
     __syncthreads();
 }
 
-template <int NC, int AS>
+template <int NC, int AS, bool MIXED>
 __device__ __forceinline__ void sb_run(const SBArgs &A, const PhiConst &P, int nbody, double *sbm)
 {
     constexpr int F = SB_F;
@@ -1106,9 +1163,16 @@ __device__ __forceinline__ void sb_run(const SBArgs &A, const PhiConst &P, int n
     double *fld = sbm;                         // [NC F N]
     double *hk = fld + NC * F * N;             // [NC (F-1) N]
     double *ud = hk + NC * (F - 1) * N;        // [NC (F-1) Le]
+    // the FP32 map (MIXED): its fields, sources and kernels in the FP64 map's regions
+    // (as floats) and in hk32; the transposed sums, combined in FP64, in part64
+    float *fld32 = reinterpret_cast<float *>(fld);
+    float *ud32 = reinterpret_cast<float *>(ud);
+    float *hk32 = reinterpret_cast<float *>(ud + NC * (F - 1) * Le);          // [NC (F-1) N]
+    double *part64 = reinterpret_cast<double *>(hk32 + ((NC * (F - 1) * N + 1) & ~1));   // [NC (F-1) N]
     for (int t = tid; t < NC * (F - 1) * N; t += SB_T) {
         const int cf = t / N, k = t - cf * N, c = cf / (F - 1), f = 1 + cf % (F - 1);
         hk[t] = A.h[(size_t)(c * F + f) * N + k];
+        if constexpr (MIXED) hk32[t] = A.h32[(size_t)(c * F + f) * N + k];
     }
 
     // up to nbody bodies, until the solve ends or pauses (C stays in LDS between them)
@@ -1218,26 +1282,55 @@ __device__ __forceinline__ void sb_run(const SBArgs &A, const PhiConst &P, int n
                     r = A.rho[t];
                 }
                 const int c = t / N, i = t - c * N;
-                fld[(c * F) * N + i] = r * A.NA;
-                sb_put(ud + (size_t)c * (F - 1) * Le, Lq, N, i, r);
+                if (MIXED && C.evfp32) {                   // k_circ1d<float, double>, k_phi32
+                    fld32[(c * F) * N + i] = float(r * A.NA);
+                    sb_put(ud32 + (size_t)c * (F - 1) * Le, Lq, N, i, float(r));
+                } else {
+                    fld[(c * F) * N + i] = r * A.NA;
+                    sb_put(ud + (size_t)c * (F - 1) * Le, Lq, N, i, r);
+                }
             }
         }
         __syncthreads();
+        const bool f32 = MIXED && C.evfp32;            // this body's map (uniform)
 
         // ---- weighted densities n_f = N_A h_f (*) rho (k_circ1d)
         const int NB = (N + 3) / 4, NI = NC * (F - 1) * NB;
         for (int it = tid; it < NI; it += SB_T) {
             const int cf = it / NB, b = it - cf * NB, c = cf / (F - 1), f = 1 + cf % (F - 1);
-            double a[4];
-            sb_conv4(ud + (size_t)c * (F - 1) * Le, Lq, hk + (size_t)cf * N, N, b, a);
+            if (f32) {
+                float a[4], cc[4];
+                sb_conv4c(ud32 + (size_t)c * (F - 1) * Le, Lq, hk32 + (size_t)cf * N, N, b, a, cc);
 #pragma unroll
-            for (int r = 0; r < 4; r++)
-                if (4 * b + r < N) fld[(c * F + f) * N + 4 * b + r] = A.NA * a[r];
+                for (int r = 0; r < 4; r++)
+                    if (4 * b + r < N) fld32[(c * F + f) * N + 4 * b + r] = float(A.NA) * (a[r] - cc[r]);
+            } else {
+                double a[4];
+                sb_conv4(ud + (size_t)c * (F - 1) * Le, Lq, hk + (size_t)cf * N, N, b, a);
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (4 * b + r < N) fld[(c * F + f) * N + 4 * b + r] = A.NA * a[r];
+            }
         }
         __syncthreads();
 
         // ---- Phi and dPhi/dn in place (k_phi)
         for (int i = tid; i < N; i += SB_T) {
+            if constexpr (MIXED) {
+                if (f32) {
+                    float nl[NC][F], g[NC][F];
+#pragma unroll
+                    for (int c = 0; c < NC; c++)
+#pragma unroll
+                        for (int f = 0; f < F; f++) nl[c][f] = fld32[(c * F + f) * N + i];
+                    phi_grad<float, double, NC, 1, true, AS, true>(P, nl, g);
+#pragma unroll
+                    for (int c = 0; c < NC; c++)
+#pragma unroll
+                        for (int f = 0; f < F; f++) fld32[(c * F + f) * N + i] = g[c][f];
+                    continue;
+                }
+            }
             double nl[NC][F], g[NC][F];
 #pragma unroll
             for (int c = 0; c < NC; c++)
@@ -1252,7 +1345,8 @@ __device__ __forceinline__ void sb_run(const SBArgs &A, const PhiConst &P, int n
         __syncthreads();
         for (int t = tid; t < NC * (F - 1) * N; t += SB_T) {
             const int cf = t / N, i = t - cf * N, c = cf / (F - 1), f = 1 + cf % (F - 1);
-            sb_put(ud + (size_t)cf * Le, Lq, N, i, fld[(c * F + f) * N + i]);
+            if (f32) sb_put(ud32 + (size_t)cf * Le, Lq, N, i, fld32[(c * F + f) * N + i]);
+            else sb_put(ud + (size_t)cf * Le, Lq, N, i, fld[(c * F + f) * N + i]);
         }
         __syncthreads();
 
@@ -1260,12 +1354,20 @@ __device__ __forceinline__ void sb_run(const SBArgs &A, const PhiConst &P, int n
         // vector slot) into the slots f >= 1
         for (int it = tid; it < NI; it += SB_T) {
             const int cf = it / NB, b = it - cf * NB, c = cf / (F - 1), f = 1 + cf % (F - 1);
-            double a[4];
-            sb_conv4(ud + (size_t)cf * Le, Lq, hk + (size_t)cf * N, N, b, a);
             const double sg = (f == 3) ? -1.0 : 1.0;
+            if (f32) {                                     // k_circ1d<float, float, double>
+                float a[4], cc[4];
+                sb_conv4c(ud32 + (size_t)cf * Le, Lq, hk32 + (size_t)cf * N, N, b, a, cc);
 #pragma unroll
-            for (int r = 0; r < 4; r++)
-                if (4 * b + r < N) fld[(c * F + f) * N + 4 * b + r] = sg * a[r];
+                for (int r = 0; r < 4; r++)
+                    if (4 * b + r < N) part64[(size_t)cf * N + 4 * b + r] = sg * (double(a[r]) - double(cc[r]));
+            } else {
+                double a[4];
+                sb_conv4(ud + (size_t)cf * Le, Lq, hk + (size_t)cf * N, N, b, a);
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (4 * b + r < N) fld[(c * F + f) * N + 4 * b + r] = sg * a[r];
+            }
         }
         __syncthreads();
 
@@ -1276,8 +1378,14 @@ __device__ __forceinline__ void sb_run(const SBArgs &A, const PhiConst &P, int n
             double v[3] = {0.0, 0.0, 0.0};
             for (int t = tid; t < n; t += SB_T) {
                 const int c = t / N, i = t - c * N;
-                double s = fld[(c * F) * N + i];
-                for (int f = 1; f < F; f++) s += fld[(c * F + f) * N + i];
+                double s;
+                if (f32) {                                 // dPhi/dn_rho in FP64, then the transposed sums
+                    s = double(fld32[(c * F) * N + i]);
+                    for (int f = 1; f < F; f++) s += part64[(size_t)(c * (F - 1) + f - 1) * N + i];
+                } else {
+                    s = fld[(c * F) * N + i];
+                    for (int f = 1; f < F; f++) s += fld[(c * F + f) * N + i];
+                }
                 if (A.vext) s += A.vext[t];
                 const double lg = clamp_lng(A.cp[c] - s);
                 const double x = A.sol[t];
@@ -1320,12 +1428,13 @@ __device__ __forceinline__ void sb_run(const SBArgs &A, const PhiConst &P, int n
     }
 }
 
-// Up to nbody bodies of one solve (cdft_converge's plain path).
-template <int NC, int AS>
+// Up to nbody bodies of one solve (cdft_converge's plain path).  MIXED: the system's
+// map is FP32 until the solver's switch, then FP64 (the FP64 map alone otherwise).
+template <int NC, int AS, bool MIXED>
 __global__ __launch_bounds__(SB_T) void k_small_body(SBArgs A, PhiConst P, int nbody)
 {
     extern __shared__ __align__(16) double sbm[];
-    sb_run<NC, AS>(A, P, nbody, sbm);
+    sb_run<NC, AS, MIXED>(A, P, nbody, sbm);
 }
 
 // One system of a batch (cdft_converge_batch of FP64 systems): its buffers and constants.
@@ -1338,7 +1447,7 @@ struct SBSys {
 // start of converge!, converge.jl:244) to the end, at most maxbodies bodies, and
 // leaves rho = exp(x) and its control state (errcode, counts) in its own buffers;
 // ticks[b] is its time in the workgroup (wall clock).
-template <int NC, int AS>
+template <int NC, int AS, bool MIXED>
 __global__ __launch_bounds__(SB_T) void k_small_batch(const SBSys *__restrict__ sys, int maxbodies,
                                                       unsigned long long *ticks)
 {
@@ -1359,16 +1468,25 @@ __global__ __launch_bounds__(SB_T) void k_small_batch(const SBSys *__restrict__ 
         A.rho[t] = exp(x);
     }
     __syncthreads();
-    sb_run<NC, AS>(A, P, maxbodies, sbm);
+    sb_run<NC, AS, MIXED>(A, P, maxbodies, sbm);
     __syncthreads();
-    for (int t = threadIdx.x; t < n; t += SB_T) A.rho[t] = exp(A.sol[t]);
+    // a restore left pending by the last step (back to the last point with a finite
+    // FP32 map value, as aasol_run's closing k_aa_point), then rho = exp(x)
+    const bool restore = MIXED && A.ctl->pmode == 2;
+    for (int t = threadIdx.x; t < n; t += SB_T) {
+        if (restore) A.sol[t] = A.xsafe[t];
+        A.rho[t] = exp(A.sol[t]);
+    }
     if (threadIdx.x == 0) ticks[blockIdx.x] = wall_clock64() - t0;
 }
 
-// Dynamic LDS bytes of k_small_body / k_small_batch.
-__host__ __device__ inline size_t sb_lds_bytes(int N, int nc)
+// Dynamic LDS bytes of k_small_body / k_small_batch (mixed: plus the FP32 kernels and
+// the FP64 transposed sums).
+__host__ __device__ inline size_t sb_lds_bytes(int N, int nc, bool mixed = false)
 {
-    return sizeof(double) * ((size_t)nc * SB_F * N + (size_t)nc * (SB_F - 1) * N + (size_t)nc * (SB_F - 1) * sb_le(N));
+    const size_t m = (size_t)nc * (SB_F - 1) * N;
+    return sizeof(double) * ((size_t)nc * SB_F * N + m + (size_t)nc * (SB_F - 1) * sb_le(N)) +
+           (mixed ? sizeof(float) * ((m + 1) & ~(size_t)1) + sizeof(double) * m : 0);
 }
 
 }  // namespace cdft

@@ -195,25 +195,25 @@ def test_batched_mixed_binary_interface():
 
 
 def test_batched_refusals(pores):
-    """What neither batched kernel takes raises before any launch: mixed-precision
-    multi-component batches and mixed ones needing the HBM-state kernel (each shares
-    code with a kernel that faulted the GPU once, cause not shown), FP64 systems above
-    the single-workgroup body's 512 points x components, and FP64 batches mixing
-    association variants."""
+    """What neither batched kernel takes raises before any launch: systems above the
+    single-workgroup body's 512 points x components (in FP64 and in mixed precision:
+    the register-state kernel's HBM-state and multi-component variants share code with
+    a kernel that faulted the GPU once, cause not shown), batches mixing association
+    variants, and batches mixing precisions."""
     big64, r64 = configs.c1c10_interface(ngrid=512, box=10.0)          # 1024 points x components
     with pytest.raises(_lib.CDFTError):
         batch([big64], [r64.copy()], tol=1e-8)
+    bigm, rm = configs.c1c10_interface(ngrid=512, box=10.0, options=DFTOptions(precision="mixed"))
+    with pytest.raises(_lib.CDFTError):
+        batch([bigm], [rm.copy()], tol=1e-8)
     from test_gpu_small_body import _interface
     sa, ra = _interface(["ethanol", "hexane"], [0.5, 0.5], 298.15, 64)  # one association pair
     sb, rb = configs.c1c10_interface(ngrid=64, box=6.0)                 # none
     with pytest.raises(_lib.CDFTError):
         batch([sa, sb], [ra.copy(), rb.copy()], tol=1e-8)
-    sm, rm = configs.c1c10_interface(ngrid=64, box=6.0, options=DFTOptions(precision="mixed"))
+    sm, rm2 = configs.c1c10_interface(ngrid=64, box=6.0, options=DFTOptions(precision="mixed"))
     with pytest.raises(_lib.CDFTError):
-        batch([sm], [rm.copy()], tol=1e-8)
-    big = configs.slit_pores(n=1, seed=0, options=DFTOptions(precision="mixed"))[0]
-    with pytest.raises(_lib.CDFTError):                   # anderson m = 6: state in HBM
-        batch([big[0]], [big[1].copy()], tol=1e-8, m=6)
+        batch([sm, sb], [rm2.copy(), rb.copy()], tol=1e-8)
 
 
 # Pores of the 512-pore workload (seed 0) for the tight oracle comparison: smallest and
@@ -419,3 +419,27 @@ def test_fp64_batch_matches_oracle_tight():
         assert e_ < 1e-10, e_
         assert abs(ga - oa) <= 1e-9 * abs(oa)
         assert abs(gt - ot) <= 1e-9 * abs(ot)
+
+
+def test_mixed_batches_on_single_workgroup_bodies_match_single_solves():
+    """Mixed-precision batches the register-state kernel does not take run on the
+    single-workgroup bodies (FP32 map to the switch, then FP64): two-component
+    interfaces, and one-component pores with an Anderson depth of 6.  Bit for bit what
+    cdft_converge gives for each system alone, FP32 evaluation counts included."""
+    mixed = DFTOptions(precision="mixed")
+    batches = {
+        "c1c10": ([configs.c1c10_interface(ngrid=128, box=10.0, options=mixed),
+                   configs.c1c10_interface(ngrid=100, box=8.0, options=mixed)], {}),
+        "pores_m6": (configs.slit_pores(n=512, seed=0, indices=[269, 343, 18], options=mixed), {"m": 6}),
+    }
+    for name, (sel, kw) in batches.items():
+        say(f"mixed batch ({name}) on single-workgroup bodies vs single solves, tol=1e-8")
+        rb = [r.copy() for _, r in sel]
+        res = batch([s for s, _ in sel], rb, tol=1e-8, maxit=3000, **kw)
+        for (s, r0), b, rr in zip(sel, rb, res):
+            r1, ra = _single(s, r0, tol=1e-8, maxit=3000, anderson_m=kw.get("m", 5))
+            say(f"  N={s.N} nc={s.nc}: batch errcode {rr['errcode']} evals {rr['nevals']} (fp32 {rr['nevals_fp32']}) | "
+                f"single errcode {ra['errcode']} evals {ra['nevals']} (fp32 {ra['nevals_fp32']})")
+            assert (rr["errcode"], rr["nevals"], rr["nevals_fp32"]) == (ra["errcode"], ra["nevals"], ra["nevals_fp32"])
+            assert rr["nevals_fp32"] > 0
+            assert np.array_equal(b, r1)

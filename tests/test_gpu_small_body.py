@@ -6,7 +6,7 @@ import sys
 import numpy as np
 import pytest
 
-from cdft_amd import PCSAFT, DFTSystem, TwoPhase1DCart, configs, converge_
+from cdft_amd import PCSAFT, DFTOptions, DFTSystem, TwoPhase1DCart, configs, converge_
 from cdft_amd.methods import bulk_chemical_potential
 from cdft_amd.system import initialize_profiles
 from helpers import oracle_of
@@ -32,9 +32,13 @@ def _interface(comps, x, T, N, combining=None):
     return s, initialize_profiles(s)
 
 
+MIXED = DFTOptions(precision="mixed")
+
 CASES = {
     # (system factory, free interface)
     "methane_wall_201": (lambda: configs.methane_wall(), False),
+    "methane_wall_201_mixed": (lambda: configs.methane_wall(options=MIXED), False),
+    "c1c10_128_mixed": (lambda: configs.c1c10_interface(ngrid=128, box=10.0, options=MIXED), True),
     "c1c10_128": (lambda: configs.c1c10_interface(ngrid=128, box=10.0), True),
     "ethanol_hexane_201": (lambda: _interface(["ethanol", "hexane"], [0.5, 0.5], 298.15, 201), True),
     "water_ethanol_cr1_128": (lambda: _interface(["water", "ethanol"], [0.9, 0.1], 350.0, 128, "cr1"), True),
