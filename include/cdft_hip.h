@@ -180,11 +180,12 @@ int cdft_converge(cdft_sys *sys, double *rho_inout, const cdft_solver_opts *opts
  * result[i].nhist of them recorded; the allocation-free entry point ignores it);
  * progress is ignored.  Two kernels serve batches: one-component CDFT_MIXED 1D
  * systems of at most 512 points with anderson m <= 5 take the register-state kernel
- * (FP32 map to its noise floor, then FP64); CDFT_FP64 1D Cartesian systems of one or
- * two components with N nc <= 512 (association allowed, one variant per batch) take
- * one workgroup each running cdft_converge's single-workgroup bodies to the end (the
- * same bits as cdft_converge of each system).  Other batches return
- * CDFT_ERR_UNSUPPORTED (use cdft_converge per system). */
+ * (FP32 map to its noise floor, then FP64); other 1D Cartesian systems of one or two
+ * components with N nc <= 512 (CDFT_FP64 or CDFT_MIXED, one precision and one
+ * association variant per batch) take one workgroup each running cdft_converge's
+ * single-workgroup bodies to the end (the same bits as cdft_converge of each
+ * system).  Other batches return CDFT_ERR_UNSUPPORTED (use cdft_converge per
+ * system). */
 int cdft_converge_batch(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_inout,
                         const cdft_solver_opts *opts, cdft_result *results);
 /* Same with rho_inout as device pointers (inputs resident in HBM). */
