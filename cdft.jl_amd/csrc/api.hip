@@ -56,6 +56,14 @@ double simpson1d(const double *f, int n, double dz)
     return s * (dz / 3.0);
 }
 
+// Captured solver bodies (one graph per map variant).  They hold the line-FFT
+// variant they were captured with, so a new variant drops them too.
+static void drop_body_graphs(cdft_sys *s)
+{
+    for (hipGraphExec_t &e : s->aa_exec)
+        if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
+}
+
 void free_sys_buffers(cdft_sys *s)
 {
     double **bufs[] = {&s->h, &s->hT, &s->vext, &s->cpd, &s->rho, &s->n, &s->dphi, &s->part, &s->fval, &s->out, &s->sol,
@@ -64,7 +72,7 @@ void free_sys_buffers(cdft_sys *s)
     for (double **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (s->H64) { (void)hipFree(s->H64); s->H64 = nullptr; }
-    if (s->aa_exec) { (void)hipGraphExecDestroy(s->aa_exec); s->aa_exec = nullptr; }
+    drop_body_graphs(s);
     if (s->ctl) { (void)hipFree(s->ctl); s->ctl = nullptr; }
     if (s->dhist) { (void)hipFree(s->dhist); s->dhist = nullptr; s->dhist_cap = 0; }
     double **b3[] = {&s->G3, &s->A3, &s->B3, &s->S3, &s->B3x[0], &s->B3x[1], &s->B3x[2]};
@@ -1178,8 +1186,18 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
 
     AACtl *c = s->ctl;
     const bool small = small_body_ok(s, adj != nullptr, mixed);
+    // Mixed precision with multi-kernel bodies: the FP32 phase's bodies carry the FP32
+    // map only and those after the switch the FP64 map only (no gated launches of the
+    // other map); a body that needs the FP64 map while only FP32 ones are queued stops
+    // the chunk (paused = 2) and the host continues with FP64 bodies.
+    const bool split = mixed && !small;
+    bool only32 = split && hc->fp32;
+    if (only32) {
+        hc->graph_map = 1;
+        HIPCHECK_CTX(ctx, hipMemcpyAsync(s->ctl, hc, sizeof(AACtl), hipMemcpyHostToDevice, st));
+    }
     hipError_t sb_err = hipSuccess;
-    auto body = [&]() {
+    auto body = [&](bool map64, bool map32) {
         if (small) {                              // one body (the first, eager)
             const hipError_t e = small_body(s, c, 1);
             if (e != hipSuccess) sb_err = e;
@@ -1202,10 +1220,12 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
         hipLaunchKernelGGL(k_aa_point, dim3(ne), dim3(256), 0, st, (int)n, (const AACtl *)c, (const double *)s->df,
                            s->Q, s->Qalt, s->sol, (const double *)s->gx, (const double *)s->dg, s->DG, s->xsafe, ptgt,
                            pmask);
-        s->gate = &c->g_f64;
-        if (adj) launch_hvp(s, adj->vbuf);
-        else launch_functional(s);
-        if (mixed) {
+        if (map64) {
+            s->gate = &c->g_f64;
+            if (adj) launch_hvp(s, adj->vbuf);
+            else launch_functional(s);
+        }
+        if (map32) {
             s->gate = &c->g_f32;
             launch_functional32(s);
         }
@@ -1227,7 +1247,7 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
 
     std::vector<double> prho(prog ? n : 0);       // the paused evaluation's profile
     HIPCHECK_CTX(ctx, hipEventRecord(ctx->ev0, st));
-    body();                                       // eager first body
+    body(true, mixed);                            // eager first body, both maps
     HIPCHECK_CTX(ctx, sb_err);
     HIPCHECK_CTX(ctx, hipGetLastError());
     long bodies = 1, pauses = 0;
@@ -1235,16 +1255,18 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
         HIPCHECK_CTX(ctx, hipMemcpyAsync(hc, c, sizeof(AACtl), hipMemcpyDeviceToHost, st));
         HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
         if (hc->paused) {
-            // the evaluation just finished used rho = exp(x) in s->rho, and nothing
-            // has run since (every gate closed at the pause)
-            HIPCHECK_CTX(ctx, hipMemcpyAsync(prho.data(), s->rho, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-            HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
-            if (o->progress(o->progress_user, hc->nevals, prho.data(), n) != 0) {
-                std::memcpy(rho_inout, prho.data(), sizeof(double) * n);
-                return fail(ctx, CDFT_ERR_STOPPED, "cdft_converge: stopped by the progress callback");
+            if (hc->paused & STOP_PROGRESS) {
+                // the evaluation just finished used rho = exp(x) in s->rho, and only
+                // the control step has run since (every gate closed at the stop)
+                HIPCHECK_CTX(ctx, hipMemcpyAsync(prho.data(), s->rho, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+                HIPCHECK_CTX(ctx, hipStreamSynchronize(st));
+                if (o->progress(o->progress_user, hc->nevals, prho.data(), n) != 0) {
+                    std::memcpy(rho_inout, prho.data(), sizeof(double) * n);
+                    return fail(ctx, CDFT_ERR_STOPPED, "cdft_converge: stopped by the progress callback");
+                }
             }
-            if (small) hipLaunchKernelGGL(k_aa_resume<true>, dim3(1), dim3(64), 0, st, c);
-            else hipLaunchKernelGGL(k_aa_resume<false>, dim3(1), dim3(1), 0, st, c);
+            if (hc->paused & STOP_MAP) only32 = false;   // the FP64 map from here
+            hipLaunchKernelGGL(k_aa_resume, dim3(1), dim3(1), 0, st, c);
             HIPCHECK_CTX(ctx, hipGetLastError());
             pauses++;
             continue;
@@ -1256,13 +1278,15 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
             HIPCHECK_CTX(ctx, small_body(s, c, K));
             HIPCHECK_CTX(ctx, hipGetLastError());
         } else if (use_graph) {
-            bool same = s->aa_exec != nullptr;
-            for (int j = 0; j < 5 && same; j++) same = (s->aa_key[j] == key[j]);
+            const int v = only32 ? 1 : 0;
+            hipGraphExec_t &exec = s->aa_exec[v];
+            bool same = exec != nullptr;
+            for (int j = 0; j < 5 && same; j++) same = (s->aa_key[v][j] == key[j]);
             if (!same) {
-                if (s->aa_exec) { (void)hipGraphExecDestroy(s->aa_exec); s->aa_exec = nullptr; }
+                if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
                 hipGraph_t g = nullptr;
                 HIPCHECK_CTX(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-                for (int b = 0; b < K; b++) body();
+                for (int b = 0; b < K; b++) body(!only32, only32);
                 hipError_t e1 = hipGetLastError();
                 hipError_t e2 = hipStreamEndCapture(st, &g);
                 if (e1 != hipSuccess || e2 != hipSuccess) {
@@ -1270,17 +1294,17 @@ static int aasol_run(cdft_sys *s, double *rho_inout, const cdft_solver_opts *o, 
                     return fail(ctx, CDFT_ERR_HIP, std::string("aasol graph capture: ") +
                                                        hipGetErrorString(e1 != hipSuccess ? e1 : e2));
                 }
-                hipError_t e3 = hipGraphInstantiate(&s->aa_exec, g, nullptr, nullptr, 0);
+                hipError_t e3 = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 if (e3 != hipSuccess) {
-                    s->aa_exec = nullptr;
+                    exec = nullptr;
                     return fail(ctx, CDFT_ERR_HIP, std::string("aasol graph instantiate: ") + hipGetErrorString(e3));
                 }
-                for (int j = 0; j < 5; j++) s->aa_key[j] = key[j];
+                for (int j = 0; j < 5; j++) s->aa_key[v][j] = key[j];
             }
-            HIPCHECK_CTX(ctx, hipGraphLaunch(s->aa_exec, st));
+            HIPCHECK_CTX(ctx, hipGraphLaunch(exec, st));
         } else {
-            for (int b = 0; b < K; b++) body();
+            for (int b = 0; b < K; b++) body(!only32, only32);
             HIPCHECK_CTX(ctx, hipGetLastError());
         }
         bodies += K;
@@ -1363,6 +1387,16 @@ static std::vector<long> tune_key(const cdft_sys *s)
     return {s->g3.n[0], s->g3.n[1], s->g3.n[2], s->nc, s->precision};
 }
 
+static void set_tune(cdft_sys *s, const cdft_tuning &t)
+{
+    const cdft_tuning &o = s->tune;
+    bool same = true;
+    for (int pr = 0; pr < 2; pr++)
+        same = same && o.lines_per_tile[pr] == t.lines_per_tile[pr] && o.fused_z[pr] == t.fused_z[pr];
+    if (!same) drop_body_graphs(s);
+    s->tune = t;
+}
+
 static int tune_spectral(cdft_sys *s, bool force)
 {
     if (!spectral(s)) return CDFT_OK;
@@ -1372,11 +1406,12 @@ static int tune_spectral(cdft_sys *s, bool force)
         if (s->tune.tuned) return CDFT_OK;
         auto it = ctx->tune_cache.find(key);
         if (it != ctx->tune_cache.end()) {
-            s->tune = it->second;
+            set_tune(s, it->second);
             return CDFT_OK;
         }
     }
     HIPCHECK_CTX(ctx, hipSetDevice(ctx->device));
+    const cdft_tuning before = s->tune;
     cdft_tuning best = s->tune;
     const int nprec = (s->precision == CDFT_MIXED) ? 2 : 1;
     const int reps = s->N >= (1L << 22) ? 2 : 4;
@@ -1391,7 +1426,7 @@ static int tune_spectral(cdft_sys *s, bool force)
                 double ms = 0.0;
                 int rc = time_spectral_seq(s, pr == 1, reps, &ms);
                 if (rc != CDFT_OK) {
-                    s->tune = best;
+                    s->tune = before;
                     return rc;
                 }
                 if (tbest < 0 || ms < tbest) {
@@ -1405,7 +1440,8 @@ static int tune_spectral(cdft_sys *s, bool force)
         s->tune = best;
     }
     best.tuned = 1;
-    s->tune = best;
+    s->tune = before;
+    set_tune(s, best);
     ctx->tune_cache[key] = best;
     return CDFT_OK;
 }
@@ -1437,7 +1473,7 @@ int cdft_system_set_tuning(cdft_sys *s, const cdft_tuning *in)
             return fail(s->ctx, CDFT_ERR_ARG, "cdft_system_set_tuning: no all-pairs z pass on this grid");
     }
     t.tuned = 1;                               // an explicit choice: cdft_converge keeps it
-    s->tune = t;
+    set_tune(s, t);
     return CDFT_OK;
 }
 

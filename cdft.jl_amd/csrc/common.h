@@ -103,8 +103,9 @@ struct cdft_sys {
     AACtl *ctl = nullptr;
     double *dhist = nullptr;
     int dhist_cap = 0;
-    hipGraphExec_t aa_exec = nullptr;
-    const void *aa_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // [0]: bodies with the FP64 map (or the adjoint's), [1]: the FP32 map only
+    hipGraphExec_t aa_exec[2] = {nullptr, nullptr};
+    const void *aa_key[2][5] = {};
     const int *gate = nullptr;  // gate of the kernels being enqueued (null: always run)
     cdft_tuning tune = {{8, 8}, {1, 1}, {0.0, 0.0}, 0};   // line-FFT variant per map precision
 };

@@ -33,7 +33,9 @@ struct AACtl {
     int g_proj;        // no Gram-Schmidt projections in this body
     int g_f64;         // this body's map is not the FP64 one
     int g_f32;         // this body's map is not the FP32 one
-    int paused;        // stopped after an evaluation for the host's progress callback
+    int paused;        // stopped after a control step (gates kept in sg), bits:
+                       // 1 for the host's progress callback, 2 before a body whose
+                       // map the host's queued bodies do not carry
     // work of the body in flight
     int umode;         // k_update mode (1 first/Picard, 2 Anderson)
     int pmode;         // point kernel: 0 none, 1 sol = gx, 2 sol = xsafe, 3 Anderson point
@@ -50,6 +52,8 @@ struct AACtl {
     int fp32, fp32_failed, toosoon, since_best, restore, errcode, idid;
     int m, maxit, picard_maxit, hist_cap, adjoint;
     int pause_every;   // pause after every this many evaluations (0: never)
+    int graph_map;     // 1: the host's bodies carry the FP32 map only (0: both maps)
+    int sg[4];         // g_done, g_proj, g_f64, g_f32 of the body after a stop
     double r, dfn2, dfres, tol, up_bd, picard_tol, best;
     double rtol, atol, picard_rtol, picard_atol, beta, picard_beta;
     double R[CDFT_MAXM * CDFT_MAXM];

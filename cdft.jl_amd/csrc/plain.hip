@@ -789,12 +789,24 @@ __device__ inline bool aa_pre(AACtl &c)
     return dd;
 }
 
-// Pause after an evaluation (converge!'s per-evaluation logger, converge.jl:256-267):
-// every gate closes, so the bodies still queued do nothing, and the work of the next
-// body (aa_pre) waits for k_aa_resume.  The state aa_post left is untouched.
-__device__ inline void aa_pause(AACtl &c)
+// Stop after a control step, with the next body's work already set by aa_pre:
+// STOP_PROGRESS after every pause_every-th evaluation (converge!'s per-evaluation
+// logger, converge.jl:256-267); STOP_MAP when the next body needs the FP64 map while
+// the host's bodies carry the FP32 map only (the switch, or an FP32 failure).  Every
+// gate closes, so the bodies still queued do nothing; the next body's gates wait in
+// sg for k_aa_resume.  The arithmetic of the solve stays where it is without a stop,
+// so a stop changes no bit of it.
+enum { STOP_PROGRESS = 1, STOP_MAP = 2 };
+__device__ inline void aa_stop(AACtl &c)
 {
-    c.paused = 1;
+    int why = (c.pause_every > 0 && c.nevals % c.pause_every == 0) ? STOP_PROGRESS : 0;
+    if (c.graph_map == 1 && !c.g_done && !c.g_f64) why |= STOP_MAP;
+    if (!why) return;
+    c.sg[0] = c.g_done;
+    c.sg[1] = c.g_proj;
+    c.sg[2] = c.g_f64;
+    c.sg[3] = c.g_f32;
+    c.paused = why;
     c.g_done = 1;
     c.g_proj = 1;
     c.g_f64 = 1;
@@ -862,8 +874,8 @@ __global__ __launch_bounds__(256) void k_aa_fin(const double *part, int nblk, do
     if (STEP == 1) aa_mid(*ctl, fin);
     else {
         aa_post(*ctl, fin, hist);
-        if (ctl->pause_every > 0 && ctl->nevals % ctl->pause_every == 0) aa_pause(*ctl);
-        else aa_pre(*ctl);
+        aa_pre(*ctl);
+        aa_stop(*ctl);
     }
 }
 
@@ -871,23 +883,17 @@ __global__ __launch_bounds__(256) void k_aa_fin(const double *part, int nblk, do
 template <int MX>
 __device__ inline void wave_downdate(double *R, int m, double *Qx);
 
-// Resumes a paused solve: the deferred aa_pre of the next body, with the downdate
-// of the body that paused (WAVE: k_small_body's whole-wave one; launch one wave),
-// so that a pause changes no bit of the solve.
-template <bool WAVE>
+// Resumes a stopped solve: the next body's gates back; after STOP_MAP the host's
+// bodies carry the FP64 map (the FP32 one is not used again).
 __global__ void k_aa_resume(AACtl *ctl)
 {
-    __shared__ int dd;
-    if (threadIdx.x == 0) {
-        dd = 0;
-        if (ctl->paused) {
-            ctl->paused = 0;
-            if constexpr (WAVE) dd = aa_pre<true>(*ctl);
-            else aa_pre(*ctl);
-        }
-    }
-    __syncthreads();
-    if (WAVE && dd) wave_downdate<CDFT_MAXM>(ctl->R, ctl->m, ctl->qx.a);
+    if (threadIdx.x != 0 || !ctl->paused) return;
+    if (ctl->paused & STOP_MAP) ctl->graph_map = 0;
+    ctl->g_done = ctl->sg[0];
+    ctl->g_proj = ctl->sg[1];
+    ctl->g_f64 = ctl->sg[2];
+    ctl->g_f32 = ctl->sg[3];
+    ctl->paused = 0;
 }
 
 // pass 1: optional downdate Q' = Q Qx, then h1 = Q'^T df and t = Q'^T res.
@@ -1413,12 +1419,11 @@ __device__ __forceinline__ void sb_run(const SBArgs &A, const PhiConst &P, int n
         __shared__ int dd;
         if (tid == 0) {
             aa_post(C, fin, A.hist);
-            dd = 0;
-            if (C.pause_every > 0 && C.nevals % C.pause_every == 0) aa_pause(C);
-            else dd = aa_pre<true>(C);
+            dd = aa_pre<true>(C);
         }
         __syncthreads();
         if (dd && tid < 64) wave_downdate<CDFT_MAXM>(C.R, C.m, C.qx.a);
+        if (tid == 0) aa_stop(C);              // after the downdate, in wave 0's order
         __syncthreads();
     }
     {

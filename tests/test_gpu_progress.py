@@ -125,6 +125,40 @@ def test_mixed_precision_unchanged_by_pauses():
     assert its == list(range(3, r1["nevals"] + 1, 3))
 
 
+CASES_PAUSE = {
+    # (system factory, precision)
+    "interface_512x2_mixed": (lambda o: configs.c1c10_interface(ngrid=512, box=10.0, options=o), "mixed"),
+    "interface_512x2_fp64": (lambda o: configs.c1c10_interface(ngrid=512, box=10.0, options=o), "fp64"),
+    "wall_mixed": (lambda o: configs.methane_wall(options=o), "mixed"),
+    "wall_fp64": (lambda o: configs.methane_wall(options=o), "fp64"),
+}
+
+
+@pytest.mark.parametrize("every", [1, 2, 5])
+@pytest.mark.parametrize("name", list(CASES_PAUSE))
+def test_solve_unchanged_by_pauses(name, every):
+    """A pause after every (or every 2nd, 5th) evaluation leaves the solve bitwise
+    unchanged, on the multi-kernel bodies (methane/n-decane interface, 512 points x 2;
+    in mixed precision FP32 bodies until the switch, then FP64 bodies, so the switch
+    also falls on a pause) and on the single-workgroup bodies (methane wall)."""
+    make, prec = CASES_PAUSE[name]
+    s, rho0 = make(DFTOptions(precision="mixed") if prec == "mixed" else None)
+    say(f"progress: {name}, save every {every}")
+    ref = rho0.copy()
+    converge_(s, ref, tol=1e-8)
+    r0 = s.last_result
+    its = []
+    got = rho0.copy()
+    converge_(s, got, tol=1e-8, save_interval=every, save_callback=lambda it, r: its.append(it))
+    r1 = s.last_result
+    say(f"  {r1['nevals']} evaluations ({r1['nevals_fp32']} FP32), errcode {r1['errcode']}")
+    assert (r1["nevals"], r1["nevals_fp32"], r1["errcode"]) == (r0["nevals"], r0["nevals_fp32"], r0["errcode"])
+    if prec == "mixed":
+        assert 0 < r1["nevals_fp32"] < r1["nevals"]
+    assert np.array_equal(got, ref)
+    assert its == list(range(every, r1["nevals"] + 1, every))
+
+
 def test_progress_on_3d_grid():
     """The spectral path (3D, 32^3, three components around two hard spheres): saved
     every 7 evaluations, same solve as without."""
