@@ -150,7 +150,8 @@ void fft3_launch(const cdft_sys *s, const Pass3 &p)
     if (AX == 0) nblk = (G.N / G.n[0]) / TB;
     else if (AX == 1) nblk = (long)(G.n[0] / TB) * G.n[2];
     else nblk = (long)(G.n[0] / TB) * G.n[1];
-    hipLaunchKernelGGL((k_fft3<T, AX, IN, OUT, EPT, TB>), dim3((unsigned)nblk), dim3(256), lds, s->ctx->stream, G, p);
+    hipLaunchKernelGGL((k_fft3<T, AX, IN, OUT, EPT, TB>), dim3((unsigned)nblk, std::max(p.ncomp, 1)), dim3(256), lds,
+                       s->ctx->stream, G, p);
 }
 
 // EPT = L TB / 1024 radix-4 butterflies per thread (0: short lines, L TB <= 512,
@@ -251,51 +252,62 @@ void zpairs_launch(const cdft_sys *s, const ZPairs3 &p)
 }
 
 // Weighted densities of src (FP64, layout [c N + k]) into nout[(c F + f) N + k],
-// f >= 1, computed in T.
+// f >= 1, computed in T.  Every pass takes all components in one launch
+// (blockIdx.y = component; A3 and B3 hold one slice per component).
 template <typename T>
 void launch_weighted3d(cdft_sys *s, const double *src, T *nout)
 {
-    const int N = s->N, F = s->F;
+    const int N = s->N, F = s->F, nc = s->nc;
     const bool plane = (s->g3.n[2] == 1);            // a 2D grid: no z passes
-    for (int c = 0; c < s->nc; c++) {
-        Pass3 p;
-        std::memset(&p, 0, sizeof(p));
-        p.in = src + (size_t)c * N;
-        p.out = s->A3;
-        fft3_pass<T, 0, IN_RHO, OUT_CPLX>(s, p, false);
-        p.in = s->A3;
-        fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, p, false);
-        if (!plane) fft3_pass<T, 2, IN_CPLX, OUT_CPLX>(s, p, false);
-        for (int q = 0; q < 4; q++) {
-            Pass3 pq = pair_pass<T>(s, c, q);
-            pq.in = s->A3;
-            pq.out = s->B3;
-            if (plane) {
-                fft3_pass<T, 1, IN_SPEC, OUT_CPLX>(s, pq, true);
-            } else {
-                fft3_pass<T, 2, IN_SPEC, OUT_CPLX>(s, pq, true);
-                pq.in = s->B3;
-                fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, pq, true);
-            }
+    const long cplx = (long)N * sizeof(cx<T>), field = (long)F * N * sizeof(T);
+    Pass3 p;
+    std::memset(&p, 0, sizeof(p));
+    p.ncomp = nc;
+    p.in = src;
+    p.cs[0] = (long)N * sizeof(double);
+    p.out = s->A3;
+    p.cs[2] = cplx;
+    fft3_pass<T, 0, IN_RHO, OUT_CPLX>(s, p, false);
+    p.in = s->A3;
+    p.cs[0] = cplx;
+    fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, p, false);
+    if (!plane) fft3_pass<T, 2, IN_CPLX, OUT_CPLX>(s, p, false);
+    for (int q = 0; q < 4; q++) {
+        Pass3 pq = pair_pass<T>(s, 0, q);
+        pq.ncomp = nc;
+        pq.cs[5] = pq.cs[6] = (long)8 * N * sizeof(T);   // spectra: 8 slots per component
+        pq.in = s->A3;
+        pq.cs[0] = cplx;
+        pq.out = s->B3;
+        pq.cs[2] = cplx;
+        if (plane) {
+            fft3_pass<T, 1, IN_SPEC, OUT_CPLX>(s, pq, true);
+        } else {
+            fft3_pass<T, 2, IN_SPEC, OUT_CPLX>(s, pq, true);
             pq.in = s->B3;
-            pq.out = nout + (size_t)(c * F + kPair3[q][0]) * N;
-            pq.out2 = nout + (size_t)(c * F + kPair3[q][1]) * N;
-            pq.scale = s->NA / (double)N;
-            fft3_pass<T, 0, IN_CPLX, OUT_FIELDS>(s, pq, true);
+            fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, pq, true);
         }
+        pq.in = s->B3;
+        pq.out = nout + (size_t)kPair3[q][0] * N;
+        pq.out2 = nout + (size_t)kPair3[q][1] * N;
+        pq.cs[2] = pq.cs[3] = field;
+        pq.scale = s->NA / (double)N;
+        fft3_pass<T, 0, IN_CPLX, OUT_FIELDS>(s, pq, true);
     }
 }
 
 // dtot[c N + k] = dphi[(c F) N + k] + sum_f (transposed convolution of dphi_f), in T.
-// With the extra work arrays the four accumulating z passes and the first inverse z
-// pass are one all-pairs launch.
+// Every pass takes all components in one launch (blockIdx.y = component; B3 and S3
+// hold one slice per component).  With the extra work arrays (the tuner's fused_z)
+// the four accumulating z passes and the first inverse z pass of a component are one
+// all-pairs launch instead, component by component.
 template <typename T>
 void launch_transposed3d(cdft_sys *s, const T *dphi, double *dtot)
 {
-    const int N = s->N, F = s->F;
+    const int N = s->N, F = s->F, nc = s->nc;
     const bool plane = (s->g3.n[2] == 1);            // a 2D grid: no z passes
-    for (int c = 0; c < s->nc; c++) {
-        if (s->B3x[2] && s->tune.fused_z[sizeof(T) == sizeof(float) ? 1 : 0]) {
+    if (s->B3x[2] && s->tune.fused_z[sizeof(T) == sizeof(float) ? 1 : 0]) {
+        for (int c = 0; c < nc; c++) {
             ZPairs3 zp = zpairs_for<T>(s, c);
             double *bq[4] = {s->B3, s->B3x[0], s->B3x[1], s->B3x[2]};
             for (int q = 0; q < 4; q++) {
@@ -318,36 +330,48 @@ void launch_transposed3d(cdft_sys *s, const T *dphi, double *dtot)
             p.base = dphi + (size_t)(c * F) * N;
             p.scale = 1.0 / (double)N;
             fft3_pass<T, 0, IN_CPLX, OUT_REAL>(s, p, true);
-            continue;
         }
-        for (int q = 0; q < 4; q++) {
-            Pass3 pq = pair_pass<T>(s, c, q);
-            pq.in = dphi + (size_t)(c * F + kPair3[q][0]) * N;
-            pq.in2 = dphi + (size_t)(c * F + kPair3[q][1]) * N;
-            pq.out = s->B3;
-            fft3_pass<T, 0, IN_PAIR, OUT_CPLX>(s, pq, false);
-            pq.in = s->B3;
-            pq.first = (q == 0);
-            if (plane) {
-                pq.out = s->S3;
-                fft3_pass<T, 1, IN_CPLX, OUT_ACCUM>(s, pq, false);
-            } else {
-                fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, pq, false);
-                pq.out = s->S3;
-                fft3_pass<T, 2, IN_CPLX, OUT_ACCUM>(s, pq, false);
-            }
-        }
-        Pass3 p;
-        std::memset(&p, 0, sizeof(p));
-        p.in = s->S3;
-        p.out = s->S3;
-        if (!plane) fft3_pass<T, 2, IN_CPLX, OUT_CPLX>(s, p, true);
-        fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, p, true);
-        p.out = dtot + (size_t)c * N;
-        p.base = dphi + (size_t)(c * F) * N;
-        p.scale = 1.0 / (double)N;
-        fft3_pass<T, 0, IN_CPLX, OUT_REAL>(s, p, true);
+        return;
     }
+    const long cplx = (long)N * sizeof(cx<T>), field = (long)F * N * sizeof(T);
+    for (int q = 0; q < 4; q++) {
+        Pass3 pq = pair_pass<T>(s, 0, q);
+        pq.ncomp = nc;
+        pq.cs[5] = pq.cs[6] = (long)8 * N * sizeof(T);   // spectra: 8 slots per component
+        pq.in = dphi + (size_t)kPair3[q][0] * N;
+        pq.in2 = dphi + (size_t)kPair3[q][1] * N;
+        pq.cs[0] = pq.cs[1] = field;
+        pq.out = s->B3;
+        pq.cs[2] = cplx;
+        fft3_pass<T, 0, IN_PAIR, OUT_CPLX>(s, pq, false);
+        pq.in = s->B3;
+        pq.in2 = nullptr;
+        pq.cs[0] = cplx;
+        pq.cs[1] = 0;
+        pq.first = (q == 0);
+        if (plane) {
+            pq.out = s->S3;
+            fft3_pass<T, 1, IN_CPLX, OUT_ACCUM>(s, pq, false);
+        } else {
+            fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, pq, false);
+            pq.out = s->S3;
+            fft3_pass<T, 2, IN_CPLX, OUT_ACCUM>(s, pq, false);
+        }
+    }
+    Pass3 p;
+    std::memset(&p, 0, sizeof(p));
+    p.ncomp = nc;
+    p.in = s->S3;
+    p.out = s->S3;
+    p.cs[0] = p.cs[2] = cplx;
+    if (!plane) fft3_pass<T, 2, IN_CPLX, OUT_CPLX>(s, p, true);
+    fft3_pass<T, 1, IN_CPLX, OUT_CPLX>(s, p, true);
+    p.out = dtot;
+    p.cs[2] = (long)N * sizeof(double);
+    p.base = dphi;
+    p.cs[4] = field;
+    p.scale = 1.0 / (double)N;
+    fft3_pass<T, 0, IN_CPLX, OUT_REAL>(s, p, true);
 }
 
 template <int NC, int AS>
@@ -869,8 +893,10 @@ int cdft_system_create(cdft_ctx *ctx, const cdft_system_desc *d, cdft_sys **out)
         (e = alloc(&s->n, (size_t)nc * F * N)) != hipSuccess || (e = alloc(&s->dphi, (size_t)nc * F * N)) != hipSuccess ||
         (e = alloc(&s->part, d3 ? (size_t)n : (size_t)nc * (F - 1) * N)) != hipSuccess ||
         (e = alloc(&s->fval, N)) != hipSuccess ||
-        (d3 && ((e = alloc(&s->G3, (size_t)8 * nc * N)) != hipSuccess || (e = alloc(&s->A3, (size_t)2 * N)) != hipSuccess ||
-                (e = alloc(&s->B3, (size_t)2 * N)) != hipSuccess || (e = alloc(&s->S3, (size_t)2 * N)) != hipSuccess)) ||
+        (d3 && ((e = alloc(&s->G3, (size_t)8 * nc * N)) != hipSuccess ||
+                (e = alloc(&s->A3, (size_t)2 * nc * N)) != hipSuccess ||       // one slice per component
+                (e = alloc(&s->B3, (size_t)2 * nc * N)) != hipSuccess ||
+                (e = alloc(&s->S3, (size_t)2 * nc * N)) != hipSuccess)) ||
         (!d3 && d->precision == CDFT_MIXED &&
          ((e = hipMalloc((void **)&s->h32, sizeof(float) * (size_t)F * nc * N)) != hipSuccess ||
           (e = hipMalloc((void **)&s->n32, sizeof(float) * (size_t)F * nc * N)) != hipSuccess ||

@@ -54,7 +54,27 @@ struct Pass3 {
     double scale;             // OUT_FIELDS / OUT_REAL / OUT_DDFT*
     int mkind;                // IN_MULT: 0 the Laplacian map, 1 + a the gradient map along axis a
     const int *gate;          // the pass returns at once when *gate is nonzero (device-side solver)
+    // Several components in one launch (blockIdx.y = component c, ncomp of them):
+    // component c's in, in2, out, out2, base, g1, g2 lie cs[0..6] bytes times c
+    // further on.  Zero strides and ncomp = 0 or 1: one component.
+    int ncomp;
+    long cs[7];
 };
+
+// Pass p of component c (null pointers stay null).
+__device__ __attribute__((always_inline)) inline void pass_component(Pass3 &p, long c)
+{
+    auto adv = [c](auto *q, long st) { return q ? reinterpret_cast<decltype(q)>(
+                                                      reinterpret_cast<const char *>(q) + c * st)
+                                                : q; };
+    p.in = adv(p.in, p.cs[0]);
+    p.in2 = adv(p.in2, p.cs[1]);
+    p.out = const_cast<void *>(adv(static_cast<const void *>(p.out), p.cs[2]));
+    p.out2 = const_cast<void *>(adv(static_cast<const void *>(p.out2), p.cs[3]));
+    p.base = adv(p.base, p.cs[4]);
+    p.g1 = adv(p.g1, p.cs[5]);
+    p.g2 = adv(p.g2, p.cs[6]);
+}
 
 // The DDFT derivative maps at grid index g (ext/SciMLBasecDFTExt.jl:16-19): the
 // Laplacian -(2 pi)^2 |k|^2 (real) or the gradient 2 pi i k_a, k the reduced
@@ -269,6 +289,7 @@ template <typename T, int AX, int IN, int OUT, int EPT, int TB = TB3>
 __global__ __launch_bounds__(256) void k_fft3(Grid3 G, Pass3 p)
 {
     if (p.gate && *p.gate) return;
+    if (blockIdx.y) pass_component(p, blockIdx.y);
     constexpr int EPR = EPT > 0 ? EPT : 1;     // radix-4 butterflies per thread and pass
     extern __shared__ __align__(16) unsigned char smem3[];
     cx<T> *a = reinterpret_cast<cx<T> *>(smem3);
