@@ -371,9 +371,11 @@ class DeviceSystem:
 
 def converge_batch(ctx, systems, rhos, *, maxit=10000, beta=1e-2, tol=1e-4, anderson_start=1e-1,
                    m=5, device_ptrs=False, noalloc=False, hist_cap=0):
-    """Batched persistent solve of many one-component mixed-precision systems of at
-    most 512 points, m <= 5 (cdft_converge_batch[_device]); other batches raise
-    CDFTError.
+    """Batched persistent solve (cdft_converge_batch[_device]): one-component
+    mixed-precision 1D systems of at most 512 points with m <= 5 take the
+    register-state kernel; other 1D Cartesian systems of one or two components with
+    N nc <= 512 (FP64 or mixed, one precision and one association variant per batch)
+    run the single-workgroup bodies; other batches raise CDFTError.
 
     rhos: list of float64 numpy arrays (in/out), or device pointers (ints) when
     device_ptrs is True.  noalloc (device pointers only): the allocation-free entry

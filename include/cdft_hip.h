@@ -184,8 +184,9 @@ int cdft_converge(cdft_sys *sys, double *rho_inout, const cdft_solver_opts *opts
  * components with N nc <= 512 (CDFT_FP64 or CDFT_MIXED, one precision and one
  * association variant per batch) take one workgroup each running cdft_converge's
  * single-workgroup bodies to the end (the same bits as cdft_converge of each
- * system).  Other batches return CDFT_ERR_UNSUPPORTED (use cdft_converge per
- * system). */
+ * system, except mixed-precision systems with association: same termination, profiles
+ * equal to rounding).  Other batches return CDFT_ERR_UNSUPPORTED (use cdft_converge
+ * per system). */
 int cdft_converge_batch(cdft_ctx *ctx, int nsys, cdft_sys *const *sys, double *const *rho_inout,
                         const cdft_solver_opts *opts, cdft_result *results);
 /* Same with rho_inout as device pointers (inputs resident in HBM). */
