@@ -139,17 +139,23 @@ __global__ void k_ddft_combine(const double *d, int N, int nc, double *deta)
 
 // Circular convolution dst[i] = scale * sum_k h[k] src[(i - k) mod N] for a batch
 // of (src, h, dst) triples selected by blockIdx.y = c*(F-1) + (f-1).  A workgroup
-// owns CIRC_TW consecutive outputs; each of its four waves sums one quarter of the
-// k range (in LDS chunks: h chunk with broadcast reads, the matching window of src
-// with consecutive reads per lane), and the quarters are added in a fixed order,
-// so the result is reproducible and many workgroups share the N^2 work.
+// owns CIRC_TW consecutive outputs; its 256 threads are 16 tap groups of 16 lanes,
+// group g summing the g-th sixteenth of the k range in chunks of CIRC_KC taps, lane l
+// the four outputs 4l .. 4l + 3 from a sliding window held in registers: per tap one
+// broadcast read of h and one LDS read of src for four multiply-adds.  Each chunk's
+// window of src is staged in LDS decimated by four (element j at [j & 3][j >> 2]), so
+// the sixteen lanes of a group read consecutive words.  The groups' partial sums are
+// added in a fixed order, so the result is reproducible.
 // The sums run in T, loads convert from TS and the result is stored as TD.  COMP
 // (the FP32 map): compensated summation, blocks of eight products summed plainly
-// and the block sums added with Kahan's correction, the four waves' sums and
+// and the block sums added with Kahan's correction, the groups' sums and
 // corrections combined in TD; the sum's rounding error then no longer grows with
 // the number of taps (a plain FP32 sum over N taps loses up to ~N eps of the
 // largest partial sum), at 1.5 extra operations per tap.
 constexpr int CIRC_TW = 64;
+constexpr int CIRC_KC = 64;
+constexpr int CIRC_G = 16;
+constexpr int CIRC_LQ = (CIRC_TW + CIRC_KC + 8) / 4;     // window row length (4 rows)
 
 template <typename T>
 __device__ __forceinline__ void kahan_add(T &s, T &c, T x)
@@ -160,6 +166,65 @@ __device__ __forceinline__ void kahan_add(T &s, T &c, T x)
     s = t;
 }
 
+// s_r (+ compensation c_r with COMP) += sum_{k < kcn} hs[k] W(j0 + r - k), r < 4, taps
+// in order, W decimated by four with rows of Lq; every index read is >= j0 - kcn.
+template <typename T, bool COMP>
+__device__ __forceinline__ void circ_chunk4(const T *W, int Lq, const T *hs, int kcn, int j0, T (&sr)[4], T (&cr)[4])
+{
+    auto U = [&](int j) __attribute__((always_inline)) { return W[(j & 3) * Lq + (j >> 2)]; };
+    T w0 = U(j0), w1 = U(j0 + 1), w2 = U(j0 + 2), w3 = U(j0 + 3);
+    // W(j0 - 1 - s - 4q) for s < 4 sits at p_s[-q]: stepping k by four keeps j & 3
+    const T *p0 = W + ((j0 - 1) & 3) * Lq + ((j0 - 1) >> 2);
+    const T *p1 = W + ((j0 - 2) & 3) * Lq + ((j0 - 2) >> 2);
+    const T *p2 = W + ((j0 - 3) & 3) * Lq + ((j0 - 3) >> 2);
+    const T *p3 = W + ((j0 - 4) & 3) * Lq + ((j0 - 4) >> 2);
+    T b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+    if constexpr (!COMP) {
+        b0 = sr[0]; b1 = sr[1]; b2 = sr[2]; b3 = sr[3];
+    }
+    auto step4 = [&](int k, int q) __attribute__((always_inline)) {
+        const T h0 = hs[k], h1 = hs[k + 1], h2 = hs[k + 2], h3 = hs[k + 3];
+        b0 += h0 * w0; b1 += h0 * w1; b2 += h0 * w2; b3 += h0 * w3;
+        const T x = p0[-q];
+        b0 += h1 * x; b1 += h1 * w0; b2 += h1 * w1; b3 += h1 * w2;
+        const T y = p1[-q];
+        b0 += h2 * y; b1 += h2 * x; b2 += h2 * w0; b3 += h2 * w1;
+        const T z = p2[-q];
+        b0 += h3 * z; b1 += h3 * y; b2 += h3 * x; b3 += h3 * w0;
+        const T u = p3[-q];
+        w3 = x; w2 = y; w1 = z; w0 = u;
+    };
+    auto fold = [&]() __attribute__((always_inline)) {
+        kahan_add(sr[0], cr[0], b0); kahan_add(sr[1], cr[1], b1);
+        kahan_add(sr[2], cr[2], b2); kahan_add(sr[3], cr[3], b3);
+        b0 = b1 = b2 = b3 = 0;
+    };
+    int k = 0;
+    if constexpr (COMP) {
+        for (; k + 8 <= kcn; k += 8) {
+            step4(k, k >> 2);
+            step4(k + 4, (k >> 2) + 1);
+            fold();
+        }
+        if (k + 4 <= kcn) {
+            step4(k, k >> 2);
+            k += 4;
+        }
+    } else {
+        for (; k + 4 <= kcn; k += 4) step4(k, k >> 2);
+    }
+    for (; k < kcn; k++) {
+        const T hv = hs[k];
+        b0 += hv * w0; b1 += hv * w1; b2 += hv * w2; b3 += hv * w3;
+        w3 = w2; w2 = w1; w1 = w0;
+        w0 = U(j0 - k - 1);
+    }
+    if constexpr (COMP) fold();
+    else {
+        sr[0] = b0; sr[1] = b1; sr[2] = b2; sr[3] = b3;
+    }
+}
+
 template <typename T, typename TS = T, typename TD = T, bool COMP = false>
 __global__ __launch_bounds__(256) void k_circ1d(const TS *__restrict__ src, long s_cstride, long s_fstride,
                                                 const T *__restrict__ h, TD *__restrict__ dst, long d_cstride,
@@ -167,11 +232,11 @@ __global__ __launch_bounds__(256) void k_circ1d(const TS *__restrict__ src, long
                                                 const int *gate)
 {
     if (gate && *gate) return;
-    constexpr int TW = CIRC_TW, NG = 4, KC = 256;
-    __shared__ T hs[NG][KC];
-    __shared__ T ws[NG][TW + KC];
-    __shared__ T part[NG][TW];
-    __shared__ T partc[COMP ? NG : 1][TW];
+    constexpr int TW = CIRC_TW, KC = CIRC_KC, G = CIRC_G, LQ = CIRC_LQ;
+    __shared__ T ws[G][4 * LQ];
+    __shared__ T hs[G][KC];
+    __shared__ T part[G][TW];
+    __shared__ T partc[COMP ? G : 1][TW];
     const int y = blockIdx.y;
     const int c = y / (F - 1), f = 1 + y % (F - 1);
     const TS *s = src + c * s_cstride + f * s_fstride;
@@ -180,64 +245,49 @@ __global__ __launch_bounds__(256) void k_circ1d(const TS *__restrict__ src, long
     T sc = scale;
     if (vec_sign && f == 3) sc = -scale;   // 1D vector slot is f = 3
     const int i0 = blockIdx.x * TW;
-    const int g = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int Kg = (N + NG - 1) / NG;                   // taps per wave
+    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
+    const int Kg = (N + G - 1) / G;                     // taps per group
     const int kb = g * Kg, ke = min(N, kb + Kg);
-    T acc = 0, accc = 0;
-    for (int kc0 = 0; kc0 < Kg; kc0 += KC) {            // the same trip count in every wave
+    T sr[4] = {0, 0, 0, 0}, cr[4] = {0, 0, 0, 0};
+    for (int kc0 = 0; kc0 < Kg; kc0 += KC) {            // the same trip count in every group
         const int kc = kb + kc0;
         const int kcn = max(0, min(KC, ke - kc));
         __syncthreads();
-        for (int t = l; t < kcn; t += TW) hs[g][t] = hk[kc + t];
-        const int base = i0 - kc - kcn + 1;
-        const int wn = kcn > 0 ? TW + kcn - 1 : 0;
-        for (int t = l; t < wn; t += TW) {
-            int gi = (base + t) % N;
-            if (gi < 0) gi += N;
-            ws[g][t] = T(s[gi]);
-        }
-        __syncthreads();
         if (kcn > 0) {
-            const T *wp = &ws[g][l + kcn - 1];
-            int kk = 0;
-            if (COMP) {
-                for (; kk + 7 < kcn; kk += 8) {
-                    T a0 = hs[g][kk] * wp[-kk], a1 = hs[g][kk + 1] * wp[-kk - 1];
-                    a0 += hs[g][kk + 2] * wp[-kk - 2];
-                    a1 += hs[g][kk + 3] * wp[-kk - 3];
-                    a0 += hs[g][kk + 4] * wp[-kk - 4];
-                    a1 += hs[g][kk + 5] * wp[-kk - 5];
-                    a0 += hs[g][kk + 6] * wp[-kk - 6];
-                    a1 += hs[g][kk + 7] * wp[-kk - 7];
-                    kahan_add(acc, accc, a0 + a1);
-                }
-                T a0 = 0;
-                for (; kk < kcn; kk++) a0 += hs[g][kk] * wp[-kk];
-                kahan_add(acc, accc, a0);
-            } else {
-                T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-                for (; kk + 3 < kcn; kk += 4) {
-                    a0 += hs[g][kk] * wp[-kk];
-                    a1 += hs[g][kk + 1] * wp[-kk - 1];
-                    a2 += hs[g][kk + 2] * wp[-kk - 2];
-                    a3 += hs[g][kk + 3] * wp[-kk - 3];
-                }
-                for (; kk < kcn; kk++) a0 += hs[g][kk] * wp[-kk];
-                acc += (a0 + a1) + (a2 + a3);
+            for (int t = l; t < kcn; t += 16) hs[g][t] = hk[kc + t];
+            // window position j holds src[i0 - kc - kcn - 3 + j]: output 4l + r, tap
+            // kc + k reads position 4l + r + kcn + 3 - k, from 4l + 3 up to 66 + kcn
+            const int wbase = i0 - kc - kcn - 3;
+            for (int t = l; t < 4 * LQ; t += 16) {
+                int gi = (wbase + t) % N;
+                if (gi < 0) gi += N;
+                ws[g][(t & 3) * LQ + (t >> 2)] = T(s[gi]);
             }
         }
+        __syncthreads();
+        if (kcn > 0) circ_chunk4<T, COMP>(ws[g], LQ, hs[g], kcn, 4 * l + kcn + 3, sr, cr);
     }
-    part[g][l] = acc;
-    if (COMP) partc[g][l] = accc;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        part[g][4 * l + r] = sr[r];
+        if (COMP) partc[g][4 * l + r] = cr[r];
+    }
     __syncthreads();
-    const int i = i0 + l;
-    if (g == 0 && i < N) {
+    const int o = threadIdx.x, i = i0 + o;
+    if (o < TW && i < N) {
         if (COMP) {
-            const TD hi = (TD(part[0][l]) + TD(part[1][l])) + (TD(part[2][l]) + TD(part[3][l]));
-            const TD lo = (TD(partc[0][l]) + TD(partc[1][l])) + (TD(partc[2][l]) + TD(partc[3][l]));
+            TD hi = 0, lo = 0;
+#pragma unroll
+            for (int q = 0; q < G; q++) {
+                hi += TD(part[q][o]);
+                lo += TD(partc[q][o]);
+            }
             d[i] = TD(sc) * (hi - lo);
         } else {
-            d[i] = TD(sc * ((part[0][l] + part[1][l]) + (part[2][l] + part[3][l])));
+            T acc = 0;
+#pragma unroll
+            for (int q = 0; q < G; q++) acc += part[q][o];
+            d[i] = TD(sc * acc);
         }
     }
 }
