@@ -606,45 +606,88 @@ __global__ __launch_bounds__(256) void k_update(const double *__restrict__ dphi,
 __host__ __device__ inline void householder_downdate(double *R, int m, double *Qx)
 {
 #pragma clang fp contract(off)
-    int nr = m, ncol = m - 1;
-    double A[CDFT_MAXM * CDFT_MAXM], tau[CDFT_MAXM], v[CDFT_MAXM];
-    for (int j = 0; j < ncol; j++)
-        for (int i = 0; i < nr; i++) A[j * CDFT_MAXM + i] = R[(j + 1) * m + i];
-    for (int j = 0; j < ncol; j++) {
-        double alpha = A[j * CDFT_MAXM + j], xn = 0.0;
-        for (int i = j + 1; i < nr; i++) xn += A[j * CDFT_MAXM + i] * A[j * CDFT_MAXM + i];
+    // every loop unrolled to CDFT_MAXM with the live entries selected, so the arrays
+    // stay in registers; the live operations and their order are those of the plain
+    // loops over j < m - 1, i < m
+    constexpr int MX = CDFT_MAXM;
+    const int nr = m, ncol = m - 1;
+    double A[MX][MX], tau[MX], v[MX], Q[MX][MX];
+#pragma unroll
+    for (int j = 0; j < MX; j++)
+#pragma unroll
+        for (int i = 0; i < MX; i++) {
+            const int t = (j + 1) * m + i;
+            A[j][i] = (j < ncol && i < nr) ? R[t < MX * MX ? t : 0] : 0.0;
+        }
+#pragma unroll
+    for (int j = 0; j < MX; j++) {
+        tau[j] = 0.0;
+        if (j >= ncol) continue;
+        const double alpha = A[j][j];
+        double xn = 0.0;
+#pragma unroll
+        for (int i = j + 1; i < MX; i++)
+            if (i < nr) xn += A[j][i] * A[j][i];
         xn = sqrt(xn);
-        if (xn == 0.0) { tau[j] = 0.0; continue; }
-        double beta = -copysign(hypot(alpha, xn), alpha);
+        if (xn == 0.0) continue;
+        const double beta = -copysign(hypot(alpha, xn), alpha);
         tau[j] = (beta - alpha) / beta;
-        double sc = 1.0 / (alpha - beta);
-        for (int i = j + 1; i < nr; i++) A[j * CDFT_MAXM + i] *= sc;
-        A[j * CDFT_MAXM + j] = beta;
-        for (int jj = j + 1; jj < ncol; jj++) {
-            double w = A[jj * CDFT_MAXM + j];
-            for (int i = j + 1; i < nr; i++) w += A[j * CDFT_MAXM + i] * A[jj * CDFT_MAXM + i];
+        const double sc = 1.0 / (alpha - beta);
+#pragma unroll
+        for (int i = j + 1; i < MX; i++)
+            if (i < nr) A[j][i] *= sc;
+        A[j][j] = beta;
+#pragma unroll
+        for (int jj = j + 1; jj < MX; jj++) {
+            if (jj >= ncol) continue;
+            double w = A[jj][j];
+#pragma unroll
+            for (int i = j + 1; i < MX; i++)
+                if (i < nr) w += A[j][i] * A[jj][i];
             w *= tau[j];
-            A[jj * CDFT_MAXM + j] -= w;
-            for (int i = j + 1; i < nr; i++) A[jj * CDFT_MAXM + i] -= w * A[j * CDFT_MAXM + i];
+            A[jj][j] -= w;
+#pragma unroll
+            for (int i = j + 1; i < MX; i++)
+                if (i < nr) A[jj][i] -= w * A[j][i];
         }
     }
-    for (int j = 0; j < ncol; j++)
-        for (int i = 0; i < nr; i++) Qx[j * nr + i] = (i == j) ? 1.0 : 0.0;
-    for (int h = ncol - 1; h >= 0; h--) {
-        if (tau[h] == 0.0) continue;
+#pragma unroll
+    for (int j = 0; j < MX; j++)
+#pragma unroll
+        for (int i = 0; i < MX; i++) Q[j][i] = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+    for (int h = MX - 1; h >= 0; h--) {
+        if (h >= ncol || tau[h] == 0.0) continue;
         v[h] = 1.0;
-        for (int i = h + 1; i < nr; i++) v[i] = A[h * CDFT_MAXM + i];
-        for (int j = 0; j < ncol; j++) {
+#pragma unroll
+        for (int i = h + 1; i < MX; i++) v[i] = A[h][i];
+#pragma unroll
+        for (int j = 0; j < MX; j++) {
+            if (j >= ncol) continue;
             double w = 0.0;
-            for (int i = h; i < nr; i++) w += v[i] * Qx[j * nr + i];
+#pragma unroll
+            for (int i = h; i < MX; i++)
+                if (i < nr) w += v[i] * Q[j][i];
             w *= tau[h];
-            for (int i = h; i < nr; i++) Qx[j * nr + i] -= w * v[i];
+#pragma unroll
+            for (int i = h; i < MX; i++)
+                if (i < nr) Q[j][i] -= w * v[i];
         }
     }
+#pragma unroll
+    for (int j = 0; j < MX; j++)
+#pragma unroll
+        for (int i = 0; i < MX; i++)
+            if (j < ncol && i < nr) Qx[j * nr + i] = Q[j][i];
     // R[1:m-1, 1:m-1] = Rd; R[:, m] = 0
-    for (int j = 0; j < ncol; j++)
-        for (int i = 0; i < ncol; i++) R[j * m + i] = (i <= j) ? A[j * CDFT_MAXM + i] : 0.0;
-    for (int i = 0; i < m; i++) R[(m - 1) * m + i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < MX; j++)
+#pragma unroll
+        for (int i = 0; i < MX; i++)
+            if (j < ncol && i < ncol) R[j * m + i] = (i <= j) ? A[j][i] : 0.0;
+#pragma unroll
+    for (int i = 0; i < MX; i++)
+        if (i < m) R[(m - 1) * m + i] = 0.0;
 }
 
 __device__ inline void aa_hist(AACtl &c, double *hist, double v)
